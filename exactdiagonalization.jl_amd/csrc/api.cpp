@@ -1,0 +1,1600 @@
+// The edhip C ABI (include/edhip.h): contexts, spaces, operators, apply,
+// iterators, sparse export, Lanczos, term expectations and MINRES.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/edhip.h"
+#include "kernels.hpp"
+
+using namespace edhip;
+
+// ============================================================================
+// Errors
+// ============================================================================
+namespace {
+
+struct Error {
+  int code;
+  std::string msg;
+};
+
+inline size_t elem_size(int prec) {
+  switch (prec) {
+    case EDHIP_F64:
+      return 8;
+    case EDHIP_C128:
+      return 16;
+    case EDHIP_F32:
+      return 4;
+    case EDHIP_C64:
+      return 8;
+    default:
+      return 0;
+  }
+}
+
+inline bool prec_is_complex(int prec) { return prec == EDHIP_C128 || prec == EDHIP_C64; }
+
+#define HIP_OR_THROW(expr)                                                                                      \
+  do {                                                                                                          \
+    hipError_t e_ = (expr);                                                                                     \
+    if (e_ != hipSuccess) {                                                                                     \
+      int code_ = (e_ == hipErrorOutOfMemory || e_ == hipErrorMemoryAllocation) ? EDHIP_ERR_MEMORY              \
+                                                                                : EDHIP_ERR_DEVICE;             \
+      throw Error{code_, std::string(#expr) + ": " + hipGetErrorString(e_)};                                    \
+    }                                                                                                           \
+  } while (0)
+
+[[noreturn]] void fail(int code, const std::string& msg) { throw Error{code, msg}; }
+
+// Device buffer owned by a handle.
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  void alloc(size_t n) {
+    release();
+    if (n == 0) n = 16;
+    HIP_OR_THROW(hipMalloc(&p, n));
+    bytes = n;
+  }
+  void ensure(size_t n) {
+    if (bytes < n) alloc(n);
+  }
+  template <typename T>
+  T* as() const {
+    return (T*)p;
+  }
+};
+
+template <typename T>
+void upload(DevBuf& b, const std::vector<T>& v, hipStream_t st) {
+  b.alloc(v.size() * sizeof(T));
+  if (!v.empty()) HIP_OR_THROW(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st));
+}
+
+}  // namespace
+
+// ============================================================================
+// Context
+// ============================================================================
+struct edhip_ctx {
+  int device = 0;
+  hipStream_t own = nullptr;
+  hipStream_t stream = nullptr;
+  std::string last_error;
+  DevBuf partial;  // reduction partials
+  DevBuf small;    // small results
+  hipEvent_t ev[16] = {};
+
+  double* partial_buf(size_t doubles) {
+    partial.ensure(doubles * sizeof(double));
+    return partial.as<double>();
+  }
+  double* small_buf(size_t doubles) {
+    small.ensure(doubles * sizeof(double));
+    return small.as<double>();
+  }
+  void sync() { HIP_OR_THROW(hipStreamSynchronize(stream)); }
+};
+
+namespace {
+
+template <typename F>
+int guarded(edhip_ctx* ctx, F&& f) {
+  try {
+    if (ctx) HIP_OR_THROW(hipSetDevice(ctx->device));
+    f();
+    if (ctx) ctx->last_error.clear();
+    return EDHIP_OK;
+  } catch (const Error& e) {
+    if (ctx) ctx->last_error = e.msg;
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    if (ctx) ctx->last_error = "host allocation failed";
+    return EDHIP_ERR_MEMORY;
+  } catch (const std::exception& e) {
+    if (ctx) ctx->last_error = e.what();
+    return EDHIP_ERR_DEVICE;
+  }
+}
+
+// binomial-based completion counts W[i*(n+1)+ones]
+std::vector<uint64_t> completion_counts(int n, const std::vector<int>& pops) {
+  std::vector<std::vector<uint64_t>> C(n + 1, std::vector<uint64_t>(n + 1, 0));
+  for (int i = 0; i <= n; ++i) {
+    C[i][0] = 1;
+    for (int k = 1; k <= i; ++k) C[i][k] = C[i - 1][k - 1] + (k <= i - 1 ? C[i - 1][k] : 0);
+  }
+  std::vector<uint64_t> W((size_t)(n + 1) * (n + 1), 0);
+  for (int i = 0; i <= n; ++i)
+    for (int ones = 0; ones <= n; ++ones) {
+      uint64_t s = 0;
+      for (int p : pops) {
+        int need = p - ones;
+        if (need >= 0 && need <= i) s += C[i][need];
+      }
+      W[(size_t)i * (n + 1) + ones] = s;
+    }
+  return W;
+}
+
+inline int bit_length(uint64_t v) { return v == 0 ? 0 : 64 - __builtin_clzll(v); }
+
+}  // namespace
+
+// ============================================================================
+// Spaces
+// ============================================================================
+struct GroupHost {
+  int G = 1;
+  int mode = kGroupNone;
+  int Lx = 0, Ly = 0;
+  std::vector<int32_t> perm;  // G x n_bits
+  std::vector<std::complex<double>> chi;
+  std::vector<int32_t> inv;
+  bool complex_phase = false;
+  // device tables
+  DevBuf nib, minrot, rot_hi, rot_lo, g_of_ab, inv_d, trivial, ph_re, ph_im, invsqrt;
+  GroupDev dev{};
+  GroupLdsLayout lay{};
+};
+
+struct edhip_space {
+  edhip_ctx* ctx = nullptr;
+  int n_bits = 0;
+  int64_t dim = 0;
+  // sector description (popcount sector)
+  bool is_sector = false;
+  std::vector<int> popcounts;
+  uint64_t pop_allowed = 0;
+  DevBuf W;  // completion counts
+  std::vector<uint64_t> W_host;
+  // basis
+  mutable bool materialized = false;
+  mutable DevBuf keys;
+  // sorted-search index
+  mutable DevBuf prefix;
+  mutable int shift = 0;
+  mutable uint64_t n_buckets = 0;
+  mutable bool index_built = false;
+  // reduction
+  const edhip_space* parent = nullptr;
+  std::unique_ptr<GroupHost> group;
+  DevBuf orbit;
+
+  void materialize() const {
+    if (materialized) return;
+    if (!is_sector) fail(EDHIP_ERR_ARGUMENT, "space has no basis");
+    if ((uint64_t)dim > (1ull << 34)) fail(EDHIP_ERR_MEMORY, "sector too large to materialise");
+    keys.alloc((size_t)dim * 8);
+    HIP_OR_THROW(launch_enumerate_sector(0, (uint64_t)dim, n_bits, W.as<uint64_t>(), keys.as<uint64_t>(), ctx->stream));
+    materialized = true;
+  }
+
+  void build_index() const {
+    if (index_built) return;
+    materialize();
+    if (dim == 0) {
+      shift = 0;
+      n_buckets = 1;
+      std::vector<uint32_t> z(2, 0);
+      prefix.alloc(8);
+      HIP_OR_THROW(hipMemcpyAsync(prefix.p, z.data(), 8, hipMemcpyHostToDevice, ctx->stream));
+      index_built = true;
+      return;
+    }
+    if ((uint64_t)dim >= 0xffffffffull) fail(EDHIP_ERR_ARGUMENT, "more than 2^32-1 basis states");
+    uint64_t max_key = 0;
+    HIP_OR_THROW(hipMemcpyAsync(&max_key, keys.as<uint64_t>() + (dim - 1), 8, hipMemcpyDeviceToHost, ctx->stream));
+    ctx->sync();
+    int bl = std::max(1, bit_length(max_key));
+    int want = std::max(1, bit_length((uint64_t)dim) + 1);
+    int k = std::min({bl, want, 26});
+    shift = bl - k;
+    n_buckets = (max_key >> shift) + 1;
+    prefix.alloc((size_t)(n_buckets + 1) * 4);
+    HIP_OR_THROW(launch_build_prefix(keys.as<uint64_t>(), dim, shift, n_buckets, prefix.as<uint32_t>(), ctx->stream));
+    index_built = true;
+  }
+
+  // Index of canonical words in this space.
+  IndexDev index_dev() const {
+    IndexDev ix{};
+    ix.dim = dim;
+    ix.n_bits = n_bits;
+    if (is_sector && !group) {
+      ix.rank_mode = 1;
+      ix.pop_allowed = pop_allowed;
+      ix.W = W.as<uint64_t>();
+      materialize();
+      ix.keys = keys.as<uint64_t>();
+      return ix;
+    }
+    build_index();
+    ix.keys = keys.as<uint64_t>();
+    ix.prefix = prefix.as<uint32_t>();
+    ix.n_buckets = n_buckets;
+    ix.shift = shift;
+    return ix;
+  }
+
+  ReduceSource source() const {
+    ReduceSource s{};
+    s.n_words = (uint64_t)dim;
+    s.n_bits = n_bits;
+    if (is_sector) {
+      s.list = nullptr;
+      s.single_popcount = (popcounts.size() == 1 && popcounts[0] > 0) ? 1 : 0;
+      s.W = W.as<uint64_t>();
+    } else {
+      materialize();
+      s.list = keys.as<uint64_t>();
+    }
+    return s;
+  }
+};
+
+namespace {
+
+// ---- group analysis --------------------------------------------------------
+uint32_t align_up(uint32_t v, uint32_t a) { return (v + a - 1) / a * a; }
+
+bool match_torus(int nb, int G, const std::vector<int32_t>& perm, int Lx, int Ly, std::vector<int32_t>& g_of_ab) {
+  if (Lx * Ly != nb || G != nb) return false;
+  g_of_ab.assign((size_t)nb, -1);
+  for (int g = 0; g < G; ++g) {
+    int img0 = perm[(size_t)g * nb + 0];
+    int a = img0 % Lx, b = img0 / Lx;
+    for (int y = 0; y < Ly; ++y)
+      for (int x = 0; x < Lx; ++x) {
+        int src = x + Lx * y;
+        int dst = ((x + a) % Lx) + Lx * ((y + b) % Ly);
+        if (perm[(size_t)g * nb + src] != dst) return false;
+      }
+    if (g_of_ab[a + Lx * b] != -1) return false;
+    g_of_ab[a + Lx * b] = g;
+  }
+  return true;
+}
+
+void build_group(GroupHost& gh, int nb, double tol, hipStream_t st) {
+  const int G = gh.G;
+  // identity first
+  for (int i = 0; i < nb; ++i)
+    if (gh.perm[i] != i) fail(EDHIP_ERR_ARGUMENT, "the first symmetry operation must be the identity");
+  // each element is a permutation of the bits
+  for (int g = 0; g < G; ++g) {
+    std::vector<char> seen(nb, 0);
+    for (int i = 0; i < nb; ++i) {
+      int j = gh.perm[(size_t)g * nb + i];
+      if (j < 0 || j >= nb || seen[j]) fail(EDHIP_ERR_ARGUMENT, "symmetry operation is not a permutation");
+      seen[j] = 1;
+    }
+  }
+  // |chi| == 1 (isapprox(abs(y), 1) with the default rtol)
+  const double rtol = std::sqrt(2.220446049250313e-16);
+  for (int g = 0; g < G; ++g) {
+    double a = std::abs(gh.chi[g]);
+    if (!(std::fabs(a - 1.0) <= rtol * std::max(a, 1.0))) fail(EDHIP_ERR_ARGUMENT, "all amplitudes need to have norm 1");
+  }
+  // closure and inverses
+  std::map<std::vector<int32_t>, int> index;
+  for (int g = 0; g < G; ++g) {
+    std::vector<int32_t> p(gh.perm.begin() + (size_t)g * nb, gh.perm.begin() + (size_t)(g + 1) * nb);
+    if (index.count(p)) fail(EDHIP_ERR_ARGUMENT, "repeated symmetry operation");
+    index[p] = g;
+  }
+  gh.inv.assign(G, -1);
+  std::vector<int32_t> comp(nb);
+  for (int g1 = 0; g1 < G; ++g1)
+    for (int g2 = 0; g2 < G; ++g2) {
+      for (int i = 0; i < nb; ++i) comp[i] = gh.perm[(size_t)g1 * nb + gh.perm[(size_t)g2 * nb + i]];
+      auto it = index.find(comp);
+      if (it == index.end()) fail(EDHIP_ERR_ARGUMENT, "symmetry operations must form a group");
+      if (it->second == 0) gh.inv[g1] = g2;
+    }
+  // canonicaliser choice
+  std::vector<int32_t> g_of_ab;
+  int bestLx = 0, bestLy = 0;
+  std::vector<int32_t> best_map;
+  for (int Ly = 1; Ly <= nb; ++Ly) {
+    if (nb % Ly) continue;
+    int Lx = nb / Ly;
+    if (Lx < 2) continue;
+    if (!match_torus(nb, G, gh.perm, Lx, Ly, g_of_ab)) continue;
+    bool better = false;
+    if (bestLx == 0)
+      better = true;
+    else {
+      bool t_new = Lx <= 12, t_old = bestLx <= 12;
+      better = (t_new && !t_old) || (t_new == t_old && Lx > bestLx);
+    }
+    if (better) {
+      bestLx = Lx;
+      bestLy = Ly;
+      best_map = g_of_ab;
+    }
+  }
+  gh.mode = (bestLx > 0) ? kGroupTorus : kGroupGeneric;
+  if (G == 1) gh.mode = kGroupGeneric;
+
+  // phases, triviality, 1/sqrt(N)
+  std::vector<double> pr(G), pi(G), isq(G + 1, 0.0);
+  std::vector<uint8_t> triv(align_up((uint32_t)G, 4), 0);
+  gh.complex_phase = false;
+  for (int h = 0; h < G; ++h) {
+    std::complex<double> ph = std::conj(gh.chi[gh.inv[h]]);
+    pr[h] = ph.real();
+    pi[h] = ph.imag();
+    if (std::fabs(pi[h]) > 1e-14) gh.complex_phase = true;
+    triv[h] = (std::abs(gh.chi[h] - 1.0) <= tol) ? 1 : 0;
+  }
+  for (int n = 1; n <= G; ++n) isq[n] = 1.0 / std::sqrt((double)n);
+
+  GroupDev& d = gh.dev;
+  d = GroupDev{};
+  d.mode = gh.mode;
+  d.G = G;
+  d.nb = nb;
+  d.full_mask = (nb >= 64) ? ~0ull : ((1ull << nb) - 1);
+  GroupLdsLayout& L = gh.lay;
+  L = GroupLdsLayout{};
+  uint32_t off = 0;
+  auto place = [&](uint32_t& o, uint32_t bytes) {
+    off = align_up(off, 16);
+    o = off;
+    off += bytes;
+  };
+  upload(gh.ph_re, pr, st);
+  upload(gh.ph_im, pi, st);
+  upload(gh.invsqrt, isq, st);
+  upload(gh.inv_d, gh.inv, st);
+  upload(gh.trivial, triv, st);
+  d.ph_re = gh.ph_re.as<double>();
+  d.ph_im = gh.ph_im.as<double>();
+  d.invsqrt = gh.invsqrt.as<double>();
+  d.inv = gh.inv_d.as<int32_t>();
+  d.trivial = gh.trivial.as<uint8_t>();
+  L.bytes_ph = (uint32_t)(G * 8);
+  L.bytes_invsqrt = (uint32_t)((G + 1) * 8);
+  L.bytes_inv = (uint32_t)(G * 4);
+  L.bytes_trivial = (uint32_t)triv.size();
+  place(L.off_ph_re, L.bytes_ph);
+  place(L.off_ph_im, L.bytes_ph);
+  place(L.off_invsqrt, L.bytes_invsqrt);
+  place(L.off_inv, L.bytes_inv);
+  place(L.off_trivial, L.bytes_trivial);
+
+  if (gh.mode == kGroupTorus) {
+    gh.Lx = bestLx;
+    gh.Ly = bestLy;
+    d.Lx = bestLx;
+    d.Ly = bestLy;
+    d.row_mask = (bestLx >= 64) ? ~0ull : ((1ull << bestLx) - 1);
+    d.minrot_table = bestLx <= 12 ? 1 : 0;
+    std::vector<uint64_t> hi(bestLx, 0), lo(bestLx, 0);
+    for (int a = 0; a < bestLx; ++a)
+      for (int y = 0; y < bestLy; ++y) {
+        uint64_t rowbits_hi = ((d.row_mask << a) & d.row_mask);
+        uint64_t rowbits_lo = (a == 0) ? 0 : ((1ull << a) - 1);
+        hi[a] |= rowbits_hi << (bestLx * y);
+        lo[a] |= rowbits_lo << (bestLx * y);
+      }
+    upload(gh.rot_hi, hi, st);
+    upload(gh.rot_lo, lo, st);
+    upload(gh.g_of_ab, best_map, st);
+    d.rot_hi = gh.rot_hi.as<uint64_t>();
+    d.rot_lo = gh.rot_lo.as<uint64_t>();
+    d.g_of_ab = gh.g_of_ab.as<int32_t>();
+    L.bytes_rot = (uint32_t)(bestLx * 8);
+    L.bytes_g_of_ab = (uint32_t)(bestLx * bestLy * 4);
+    place(L.off_rot_hi, L.bytes_rot);
+    place(L.off_rot_lo, L.bytes_rot);
+    place(L.off_g_of_ab, L.bytes_g_of_ab);
+    if (d.minrot_table) {
+      int nv = 1 << bestLx;
+      std::vector<uint32_t> mr(nv);
+      for (int v = 0; v < nv; ++v) {
+        uint32_t mv = (uint32_t)v, am = 1;
+        for (int a = 1; a < bestLx; ++a) {
+          uint32_t r = (uint32_t)((((uint64_t)v << a) | ((uint64_t)v >> (bestLx - a))) & d.row_mask);
+          if (r < mv) {
+            mv = r;
+            am = 1u << a;
+          } else if (r == mv) {
+            am |= 1u << a;
+          }
+        }
+        mr[v] = mv | (am << 16);
+      }
+      upload(gh.minrot, mr, st);
+      d.minrot = gh.minrot.as<uint32_t>();
+      L.bytes_minrot = (uint32_t)(nv * 4);
+      place(L.off_minrot, L.bytes_minrot);
+    }
+  } else {
+    int n_nib = (nb + 3) / 4;
+    d.n_nib = n_nib;
+    std::vector<uint64_t> tab((size_t)G * n_nib * 16, 0);
+    for (int g = 0; g < G; ++g)
+      for (int k = 0; k < n_nib; ++k)
+        for (int v = 0; v < 16; ++v) {
+          uint64_t o = 0;
+          for (int q = 0; q < 4; ++q) {
+            int bit = 4 * k + q;
+            if (bit < nb && ((v >> q) & 1)) o |= 1ull << gh.perm[(size_t)g * nb + bit];
+          }
+          tab[((size_t)g * n_nib + k) * 16 + v] = o;
+        }
+    upload(gh.nib, tab, st);
+    d.nib = gh.nib.as<uint64_t>();
+    L.bytes_nib = (uint32_t)(tab.size() * 8);
+    place(L.off_nib, L.bytes_nib);
+  }
+  L.total = align_up(off, 16);
+}
+
+GroupDev trivial_group_dev() {
+  GroupDev d{};
+  d.mode = kGroupNone;
+  d.G = 1;
+  return d;
+}
+
+}  // namespace
+
+// ============================================================================
+// Operators
+// ============================================================================
+struct edhip_op {
+  edhip_ctx* ctx = nullptr;
+  const edhip_space* space = nullptr;
+  int64_t n_terms = 0;
+  std::vector<uint64_t> mask, row, col;
+  std::vector<double> are, aim;
+  bool complex_amp = false;
+  // device terms
+  DevBuf off_row, off_col, diag, all_row, all_col;
+  int n_off = 0, n_diag = 0;
+  std::vector<int64_t> diag_ids, off_ids;  // original term index of each device term
+  OpDev dev_row{}, dev_col{};
+  // scratch
+  DevBuf hx, hy, iter_rows, iter_idx, iter_re, iter_im;
+  // sparse result
+  int64_t sp_nnz = -1;
+  DevBuf sp_colptr, sp_rowval, sp_re, sp_im;
+  // expectation scratch
+  DevBuf ex_partial, ex_out;
+
+  bool is_complex() const { return complex_amp || (space->group && space->group->complex_phase); }
+};
+
+namespace {
+
+void check_space(const edhip_space* s) {
+  if (!s) fail(EDHIP_ERR_ARGUMENT, "null space");
+}
+
+OpDev make_opdev(edhip_op* op, bool transpose) {
+  const edhip_space* sp = op->space;
+  OpDev d{};
+  d.ix = sp->index_dev();
+  d.dim = sp->dim;
+  sp->materialize();
+  d.rows = sp->keys.as<uint64_t>();
+  if (sp->group) {
+    d.g = sp->group->dev;
+    d.lay = sp->group->lay;
+    d.orbit = sp->orbit.as<uint16_t>();
+  } else {
+    d.g = trivial_group_dev();
+    d.lay = GroupLdsLayout{};
+    d.orbit = nullptr;
+  }
+  d.n_off = op->n_off;
+  d.n_diag = op->n_diag;
+  d.off = transpose ? op->off_col.as<OffTerm>() : op->off_row.as<OffTerm>();
+  d.diag = op->diag.as<DiagTerm>();
+  uint32_t off = align_up(d.lay.total, 16);
+  d.off_lds = off;
+  off += (uint32_t)(op->n_off * sizeof(OffTerm));
+  off = align_up(off, 16);
+  d.w_lds = off;
+  d.w_bytes = d.ix.rank_mode ? (uint32_t)((sp->n_bits + 1) * (sp->n_bits + 1) * 8) : 0u;
+  off += d.w_bytes;
+  d.lds_total = align_up(off, 16);
+  if (d.lds_total > 64 * 1024)
+    fail(EDHIP_ERR_ARGUMENT, "operator and symmetry tables exceed the 64 KiB LDS budget of the matvec kernels (" +
+                                 std::to_string(d.lds_total) + " bytes)");
+  return d;
+}
+
+void check_prec_for_op(const edhip_op* op, int prec) {
+  if (elem_size(prec) == 0) fail(EDHIP_ERR_ARGUMENT, "unknown precision code");
+  if (!prec_is_complex(prec) && op->is_complex())
+    fail(EDHIP_ERR_ARGUMENT, "complex operator or characters need a complex vector type");
+}
+
+}  // namespace
+
+// ============================================================================
+// Lanczos (thick restart, full reorthogonalisation) and its host linear algebra
+// ============================================================================
+namespace {
+
+// Householder reduction of a dense symmetric matrix (row-major, n x n) to
+// tridiagonal form A = Q T Q^T; returns d (diagonal), e (e[i] = T[i][i+1]).
+void householder_tridiag(int n, std::vector<double>& A, std::vector<double>& Q, std::vector<double>& d,
+                         std::vector<double>& e) {
+  Q.assign((size_t)n * n, 0.0);
+  for (int i = 0; i < n; ++i) Q[(size_t)i * n + i] = 1.0;
+  std::vector<double> v(n), p(n), w(n);
+  for (int k = 0; k + 2 < n; ++k) {
+    double nx = 0.0;
+    for (int i = k + 1; i < n; ++i) nx += A[(size_t)i * n + k] * A[(size_t)i * n + k];
+    nx = std::sqrt(nx);
+    if (nx == 0.0) continue;
+    double x0 = A[(size_t)(k + 1) * n + k];
+    double alpha = (x0 >= 0) ? -nx : nx;
+    for (int i = 0; i < n; ++i) v[i] = 0.0;
+    for (int i = k + 1; i < n; ++i) v[i] = A[(size_t)i * n + k];
+    v[k + 1] -= alpha;
+    double vn = 0.0;
+    for (int i = k + 1; i < n; ++i) vn += v[i] * v[i];
+    vn = std::sqrt(vn);
+    if (vn == 0.0) continue;
+    for (int i = k + 1; i < n; ++i) v[i] /= vn;
+    // p = A v ; K = v^T p ; w = p - K v ; A -= 2 (v w^T + w v^T)
+    for (int i = 0; i < n; ++i) {
+      double s = 0.0;
+      for (int j = k + 1; j < n; ++j) s += A[(size_t)i * n + j] * v[j];
+      p[i] = s;
+    }
+    double K = 0.0;
+    for (int i = k + 1; i < n; ++i) K += v[i] * p[i];
+    for (int i = 0; i < n; ++i) w[i] = p[i] - K * v[i];
+    for (int i = 0; i < n; ++i)
+      for (int j = 0; j < n; ++j) A[(size_t)i * n + j] -= 2.0 * (v[i] * w[j] + w[i] * v[j]);
+    // Q = Q H
+    for (int i = 0; i < n; ++i) {
+      double s = 0.0;
+      for (int j = k + 1; j < n; ++j) s += Q[(size_t)i * n + j] * v[j];
+      for (int j = k + 1; j < n; ++j) Q[(size_t)i * n + j] -= 2.0 * s * v[j];
+    }
+  }
+  d.assign(n, 0.0);
+  e.assign(n, 0.0);
+  for (int i = 0; i < n; ++i) d[i] = A[(size_t)i * n + i];
+  for (int i = 0; i + 1 < n; ++i) e[i] = A[(size_t)i * n + i + 1];
+}
+
+// Implicit QL with Wilkinson-type shifts on a symmetric tridiagonal matrix
+// (d diagonal, e[i] coupling i and i+1).  Z (row-major n x n) accumulates the
+// rotations: on entry the basis the tridiagonal matrix is expressed in, on
+// exit the eigenvectors as columns.  Eigenvalues are sorted ascending.
+bool tridiag_ql(int n, std::vector<double>& d, std::vector<double>& e, std::vector<double>& Z) {
+  const double eps = 2.220446049250313e-16;
+  for (int l = 0; l < n; ++l) {
+    int iter = 0;
+    while (true) {
+      int m = l;
+      for (; m < n - 1; ++m) {
+        double dd = std::fabs(d[m]) + std::fabs(d[m + 1]);
+        if (std::fabs(e[m]) <= eps * dd) break;
+      }
+      if (m == l) break;
+      if (++iter > 200) return false;
+      double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+      double r = std::hypot(g, 1.0);
+      g = d[m] - d[l] + e[l] / (g + (g >= 0 ? r : -r));
+      double s = 1.0, c = 1.0, p = 0.0;
+      int i = m - 1;
+      bool early = false;
+      for (; i >= l; --i) {
+        double f = s * e[i];
+        double b = c * e[i];
+        r = std::hypot(f, g);
+        e[i + 1] = r;
+        if (r == 0.0) {
+          d[i + 1] -= p;
+          e[m] = 0.0;
+          early = true;
+          break;
+        }
+        s = f / r;
+        c = g / r;
+        g = d[i + 1] - p;
+        r = (d[i] - g) * s + 2.0 * c * b;
+        p = s * r;
+        d[i + 1] = g + p;
+        g = c * r - b;
+        for (int k = 0; k < n; ++k) {
+          double zf = Z[(size_t)k * n + i + 1];
+          Z[(size_t)k * n + i + 1] = s * Z[(size_t)k * n + i] + c * zf;
+          Z[(size_t)k * n + i] = c * Z[(size_t)k * n + i] - s * zf;
+        }
+      }
+      if (early) continue;
+      d[l] -= p;
+      e[l] = g;
+      e[m] = 0.0;
+    }
+  }
+  // sort ascending
+  std::vector<int> ord(n);
+  for (int i = 0; i < n; ++i) ord[i] = i;
+  std::sort(ord.begin(), ord.end(), [&](int a, int b) { return d[a] < d[b]; });
+  std::vector<double> d2(n), Z2((size_t)n * n);
+  for (int j = 0; j < n; ++j) {
+    d2[j] = d[ord[j]];
+    for (int k = 0; k < n; ++k) Z2[(size_t)k * n + j] = Z[(size_t)k * n + ord[j]];
+  }
+  d = d2;
+  Z = Z2;
+  return true;
+}
+
+// Eigen-decomposition of the symmetric m x m matrix T (row-major).
+void sym_eig(int m, const std::vector<double>& T, bool tridiagonal, std::vector<double>& w, std::vector<double>& Z) {
+  std::vector<double> d(m), e(m, 0.0);
+  if (tridiagonal) {
+    for (int i = 0; i < m; ++i) d[i] = T[(size_t)i * m + i];
+    for (int i = 0; i + 1 < m; ++i) e[i] = T[(size_t)i * m + i + 1];
+    Z.assign((size_t)m * m, 0.0);
+    for (int i = 0; i < m; ++i) Z[(size_t)i * m + i] = 1.0;
+  } else {
+    std::vector<double> A = T;
+    householder_tridiag(m, A, Z, d, e);
+  }
+  if (!tridiag_ql(m, d, e, Z)) fail(EDHIP_ERR_NOCONV, "tridiagonal eigensolver did not converge");
+  w = d;
+}
+
+struct Timer {
+  std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+  double sec() const { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); }
+};
+
+// Thick-restart Lanczos on one operator in one storage precision.
+struct TRLanczos {
+  edhip_op* op;
+  int prec;        // storage precision of the Krylov vectors
+  int64_t n;
+  int m;           // basis size
+  DevBuf V;        // (m + 1) vectors
+  DevBuf tmp;      // restart combination scratch
+  DevBuf coef_d;   // device coefficients
+  const void* locked = nullptr;  // locked (deflated) vectors, same precision
+  int n_locked = 0;
+  int matvecs = 0;
+  double t_matvec = 0.0;
+  int iterations = 0, restarts = 0;
+
+  size_t es() const { return elem_size(prec); }
+  void* vec(int j) const { return (char*)V.p + (size_t)j * n * es(); }
+
+  void matvec(const void* x, void* y) {
+    edhip_ctx* ctx = op->ctx;
+    Timer t;
+    HIP_OR_THROW(launch_matvec(op->dev_row, prec, 0, 0, x, y, ctx->stream));
+    ++matvecs;
+    if (n > (1 << 22)) {  // time only the large products (avoids a sync per small step)
+      ctx->sync();
+      t_matvec += t.sec();
+    }
+  }
+
+  // <x, y> in double
+  std::complex<double> dot(const void* x, const void* y) {
+    edhip_ctx* ctx = op->ctx;
+    double* part = ctx->partial_buf(2 * (size_t)vec_partial_blocks(n));
+    double* out = ctx->small_buf(2);
+    HIP_OR_THROW(vec_dot(prec, n, x, y, part, out, ctx->stream));
+    double h[2];
+    HIP_OR_THROW(hipMemcpyAsync(h, out, 16, hipMemcpyDeviceToHost, ctx->stream));
+    ctx->sync();
+    return {h[0], h[1]};
+  }
+
+  // w -= B (B^H w) twice over vectors B_0..B_{k-1} (stride n); returns the
+  // first-pass coefficients (host).
+  std::vector<double> orthogonalize(const void* B, int k, void* w) {
+    edhip_ctx* ctx = op->ctx;
+    std::vector<double> c1(2 * (size_t)k, 0.0);
+    if (k <= 0) return c1;
+    double* part = ctx->partial_buf((size_t)2 * k * 1024);
+    coef_d.ensure((size_t)2 * k * sizeof(double));
+    for (int pass = 0; pass < 2; ++pass) {
+      HIP_OR_THROW(vec_multi_dot(prec, n, k, B, n, w, part, coef_d.as<double>(), ctx->stream));
+      HIP_OR_THROW(vec_multi_axpy(prec, n, k, B, n, coef_d.as<double>(), w, ctx->stream));
+      if (pass == 0) {
+        HIP_OR_THROW(hipMemcpyAsync(c1.data(), coef_d.p, 2 * (size_t)k * sizeof(double), hipMemcpyDeviceToHost,
+                                    ctx->stream));
+      }
+    }
+    ctx->sync();
+    return c1;
+  }
+
+  void scale(void* x, double s) { HIP_OR_THROW(vec_axpby(prec, n, s, 0.0, x, 0.0, 0.0, x, op->ctx->stream)); }
+
+  // Runs the iteration from the unit vector in vec(0).  Returns the lowest
+  // nev Ritz values; Ritz vectors (if out != nullptr) are written in
+  // precision out_prec with stride n.  res: residual norms.
+  bool run(int nev, double tol, int max_restarts, int check_every, std::vector<double>& theta, std::vector<double>& res,
+           void* out, int out_prec) {
+    edhip_ctx* ctx = op->ctx;
+    const int64_t dim = n;
+    int mm = (int)std::min<int64_t>(m, dim - n_locked);
+    if (mm < 1) mm = 1;
+    nev = std::min(nev, mm);
+    std::vector<double> T((size_t)mm * mm, 0.0);
+    int k = 0;  // kept Ritz vectors after a restart
+    bool converged = false;
+    std::vector<double> w, Z;
+    int j = 0;
+    double beta_last = 0.0;
+    int size = 0;  // current basis size used for the final Ritz step
+    for (;;) {
+      bool arrow = (k > 0);
+      for (j = k; j < mm; ++j) {
+        void* vj = vec(j);
+        void* wv = vec(j + 1);
+        matvec(vj, wv);
+        ++iterations;
+        if (n_locked) orthogonalize(locked, n_locked, wv);
+        std::vector<double> c = orthogonalize(V.p, j + 1, wv);
+        T[(size_t)j * mm + j] = c[2 * (size_t)j];
+        std::complex<double> nn = dot(wv, wv);
+        double beta = std::sqrt(std::max(nn.real(), 0.0));
+        beta_last = beta;
+        size = j + 1;
+        // convergence check on the leading (j+1) x (j+1) block
+        bool last = (j + 1 == mm);
+        bool tiny = beta <= 1e-13 * std::max(1.0, std::fabs(T[(size_t)j * mm + j]));
+        bool check = last || tiny || ((j + 1 - k) % std::max(1, check_every) == 0 && j + 1 >= nev);
+        if (check) {
+          std::vector<double> Tj((size_t)(j + 1) * (j + 1));
+          for (int a = 0; a <= j; ++a)
+            for (int b = 0; b <= j; ++b) Tj[(size_t)a * (j + 1) + b] = T[(size_t)a * mm + b];
+          sym_eig(j + 1, Tj, !arrow, w, Z);
+          bool ok = (j + 1 >= nev);
+          res.assign(nev, 0.0);
+          for (int i = 0; i < nev && i <= j; ++i) {
+            res[i] = std::fabs(beta * Z[(size_t)j * (j + 1) + i]);
+            if (res[i] > tol * std::max(1.0, std::fabs(w[i]))) ok = false;
+          }
+          if (ok || tiny) {
+            converged = ok || tiny;
+            break;
+          }
+        }
+        if (j + 1 < mm) {
+          T[(size_t)(j + 1) * mm + j] = T[(size_t)j * mm + j + 1] = beta;
+          scale(wv, 1.0 / beta);
+        }
+      }
+      if (converged) break;
+      // thick restart
+      if (restarts >= max_restarts) break;
+      ++restarts;
+      std::vector<double> Tm((size_t)mm * mm);
+      for (int a = 0; a < mm; ++a)
+        for (int b = 0; b < mm; ++b) Tm[(size_t)a * mm + b] = T[(size_t)a * mm + b];
+      sym_eig(mm, Tm, !arrow, w, Z);
+      int keep = std::min(mm - 1, std::max(nev + 2, std::min(mm / 3, nev + 16)));
+      // new basis: V'_i = sum_j Z[j][i] V_j (i < keep), V'_keep = normalised residual
+      std::vector<double> coef((size_t)mm * keep);
+      for (int a = 0; a < mm; ++a)
+        for (int i = 0; i < keep; ++i) coef[(size_t)a * keep + i] = Z[(size_t)a * mm + i];
+      coef_d.ensure(coef.size() * sizeof(double));
+      HIP_OR_THROW(hipMemcpyAsync(coef_d.p, coef.data(), coef.size() * sizeof(double), hipMemcpyHostToDevice,
+                                  ctx->stream));
+      tmp.ensure((size_t)keep * n * es());
+      HIP_OR_THROW(vec_combine(prec, prec, n, mm, V.p, n, keep, coef_d.as<double>(), tmp.p, n, ctx->stream));
+      HIP_OR_THROW(hipMemcpyAsync(V.p, tmp.p, (size_t)keep * n * es(), hipMemcpyDeviceToDevice, ctx->stream));
+      // residual direction: w = vec(mm) (unnormalised), beta_last its norm
+      scale(vec(mm), 1.0 / beta_last);
+      HIP_OR_THROW(hipMemcpyAsync(vec(keep), vec(mm), (size_t)n * es(), hipMemcpyDeviceToDevice, ctx->stream));
+      std::fill(T.begin(), T.end(), 0.0);
+      for (int i = 0; i < keep; ++i) {
+        T[(size_t)i * mm + i] = w[i];
+        double s = beta_last * Z[(size_t)(mm - 1) * mm + i];
+        T[(size_t)i * mm + keep] = T[(size_t)keep * mm + i] = s;
+      }
+      k = keep;
+      ctx->sync();
+    }
+    // Ritz values / vectors from the current (size x size) block
+    theta.assign(w.begin(), w.begin() + std::min<size_t>(nev, w.size()));
+    if (out) {
+      int sz = size;
+      std::vector<double> coef((size_t)sz * nev);
+      for (int a = 0; a < sz; ++a)
+        for (int i = 0; i < nev; ++i) coef[(size_t)a * nev + i] = Z[(size_t)a * sz + i];
+      coef_d.ensure(coef.size() * sizeof(double));
+      HIP_OR_THROW(hipMemcpyAsync(coef_d.p, coef.data(), coef.size() * sizeof(double), hipMemcpyHostToDevice,
+                                  ctx->stream));
+      HIP_OR_THROW(vec_combine(prec, out_prec, n, sz, V.p, n, nev, coef_d.as<double>(), out, n, ctx->stream));
+      ctx->sync();
+    }
+    (void)dim;
+    return converged;
+  }
+};
+
+}  // namespace
+
+// ============================================================================
+// C ABI
+// ============================================================================
+extern "C" {
+
+int edhip_create(int device, edhip_ctx** out) {
+  if (!out) return EDHIP_ERR_ARGUMENT;
+  *out = nullptr;
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n == 0) return EDHIP_ERR_DEVICE;
+  if (device < 0 || device >= n) return EDHIP_ERR_ARGUMENT;
+  auto* ctx = new edhip_ctx();
+  ctx->device = device;
+  int rc = guarded(ctx, [&] {
+    HIP_OR_THROW(hipStreamCreateWithFlags(&ctx->own, hipStreamNonBlocking));
+    ctx->stream = ctx->own;
+    for (auto& ev : ctx->ev) HIP_OR_THROW(hipEventCreate(&ev));
+  });
+  if (rc != EDHIP_OK) {
+    delete ctx;
+    return rc;
+  }
+  *out = ctx;
+  return EDHIP_OK;
+}
+
+void edhip_destroy(edhip_ctx* ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  for (auto& ev : ctx->ev)
+    if (ev) (void)hipEventDestroy(ev);
+  ctx->partial.release();
+  ctx->small.release();
+  if (ctx->own) (void)hipStreamDestroy(ctx->own);
+  delete ctx;
+}
+
+const char* edhip_last_error(const edhip_ctx* ctx) { return ctx ? ctx->last_error.c_str() : "null context"; }
+
+int edhip_set_stream(edhip_ctx* ctx, void* s) {
+  if (!ctx) return EDHIP_ERR_ARGUMENT;
+  return guarded(ctx, [&] {
+    ctx->sync();
+    ctx->stream = s ? (hipStream_t)s : ctx->own;
+  });
+}
+
+int edhip_synchronize(edhip_ctx* ctx) {
+  if (!ctx) return EDHIP_ERR_ARGUMENT;
+  return guarded(ctx, [&] { ctx->sync(); });
+}
+
+int edhip_malloc(edhip_ctx* ctx, uint64_t bytes, void** out) {
+  if (!ctx || !out) return EDHIP_ERR_ARGUMENT;
+  return guarded(ctx, [&] { HIP_OR_THROW(hipMalloc(out, bytes ? bytes : 16)); });
+}
+
+int edhip_free(edhip_ctx* ctx, void* p) {
+  if (!ctx) return EDHIP_ERR_ARGUMENT;
+  return guarded(ctx, [&] {
+    ctx->sync();
+    HIP_OR_THROW(hipFree(p));
+  });
+}
+
+int edhip_memcpy(edhip_ctx* ctx, void* dst, const void* src, uint64_t bytes, int kind) {
+  if (!ctx) return EDHIP_ERR_ARGUMENT;
+  return guarded(ctx, [&] {
+    HIP_OR_THROW(hipMemcpyAsync(dst, src, bytes, (hipMemcpyKind)kind, ctx->stream));
+    ctx->sync();
+  });
+}
+
+// ---- spaces ----------------------------------------------------------------
+int edhip_space_sector(edhip_ctx* ctx, int n_bits, int n_pop, const int32_t* pops, edhip_space** out) {
+  if (!ctx || !out) return EDHIP_ERR_ARGUMENT;
+  *out = nullptr;
+  auto sp = std::make_unique<edhip_space>();
+  int rc = guarded(ctx, [&] {
+    if (n_bits < 0 || n_bits > 63) fail(EDHIP_ERR_ARGUMENT, "type UInt64 not enough to represent the hilbert space");
+    if (n_pop < 0 || (n_pop > 0 && !pops)) fail(EDHIP_ERR_ARGUMENT, "bad popcount list");
+    std::vector<int> p;
+    for (int i = 0; i < n_pop; ++i)
+      if (pops[i] >= 0 && pops[i] <= n_bits && std::find(p.begin(), p.end(), (int)pops[i]) == p.end())
+        p.push_back(pops[i]);
+    std::sort(p.begin(), p.end());
+    sp->ctx = ctx;
+    sp->n_bits = n_bits;
+    sp->is_sector = true;
+    sp->popcounts = p;
+    for (int q : p) sp->pop_allowed |= 1ull << q;
+    sp->W_host = completion_counts(n_bits, p);
+    sp->dim = (int64_t)sp->W_host[(size_t)n_bits * (n_bits + 1) + 0];
+    upload(sp->W, sp->W_host, ctx->stream);
+    ctx->sync();
+  });
+  if (rc == EDHIP_OK) *out = sp.release();
+  return rc;
+}
+
+int edhip_space_from_list(edhip_ctx* ctx, int n_bits, int64_t n, const uint64_t* basis, edhip_space** out) {
+  if (!ctx || !out) return EDHIP_ERR_ARGUMENT;
+  *out = nullptr;
+  auto sp = std::make_unique<edhip_space>();
+  int rc = guarded(ctx, [&] {
+    if (n_bits < 0 || n_bits > 63) fail(EDHIP_ERR_ARGUMENT, "type UInt64 not enough to represent the hilbert space");
+    if (n < 0 || (n > 0 && !basis)) fail(EDHIP_ERR_ARGUMENT, "bad basis list");
+    for (int64_t i = 1; i < n; ++i)
+      if (basis[i - 1] > basis[i]) fail(EDHIP_ERR_ARGUMENT, "vals must be sorted");
+    for (int64_t i = 1; i < n; ++i)
+      if (basis[i - 1] == basis[i]) fail(EDHIP_ERR_ARGUMENT, "vals contains duplicates");
+    sp->ctx = ctx;
+    sp->n_bits = n_bits;
+    sp->dim = n;
+    sp->keys.alloc((size_t)std::max<int64_t>(n, 1) * 8);
+    if (n) HIP_OR_THROW(hipMemcpyAsync(sp->keys.p, basis, (size_t)n * 8, hipMemcpyHostToDevice, ctx->stream));
+    sp->materialized = true;
+    sp->build_index();
+    ctx->sync();
+  });
+  if (rc == EDHIP_OK) *out = sp.release();
+  return rc;
+}
+
+int edhip_space_reduce(edhip_ctx* ctx, const edhip_space* parent, int n_group, const int32_t* bit_perm,
+                       const double* chi_re, const double* chi_im, double tol, edhip_space** out) {
+  if (!ctx || !out) return EDHIP_ERR_ARGUMENT;
+  *out = nullptr;
+  auto sp = std::make_unique<edhip_space>();
+  int rc = guarded(ctx, [&] {
+    check_space(parent);
+    if (parent->group) fail(EDHIP_ERR_ARGUMENT, "the parent space is already reduced");
+    if (n_group < 1 || !bit_perm || !chi_re || !chi_im) fail(EDHIP_ERR_ARGUMENT, "empty symmetry group");
+    if (n_group > 4096) fail(EDHIP_ERR_ARGUMENT, "symmetry group too large");
+    const int nb = parent->n_bits;
+    auto gh = std::make_unique<GroupHost>();
+    gh->G = n_group;
+    gh->perm.assign(bit_perm, bit_perm + (size_t)n_group * nb);
+    gh->chi.resize(n_group);
+    for (int g = 0; g < n_group; ++g) gh->chi[g] = {chi_re[g], chi_im[g]};
+    build_group(*gh, nb, tol, ctx->stream);
+    {
+      uint32_t need = gh->lay.total + (uint32_t)((nb + 1) * (nb + 1) * 8);
+      if (need > 64 * 1024)
+        fail(EDHIP_ERR_ARGUMENT, "symmetry tables exceed the 64 KiB LDS budget (" + std::to_string(need) + " bytes)");
+    }
+    ReduceSource src = parent->source();
+    // chunking: ~4M threads
+    const uint64_t NW = src.n_words;
+    uint64_t n_threads = std::min<uint64_t>(std::max<uint64_t>((NW + 15) / 16, 1), 1ull << 22);
+    uint64_t chunk = (NW + n_threads - 1) / n_threads;
+    if (chunk == 0) chunk = 1;
+    n_threads = (NW + chunk - 1) / chunk;
+    sp->ctx = ctx;
+    sp->n_bits = nb;
+    sp->parent = parent;
+    if (n_threads == 0) {
+      sp->dim = 0;
+    } else {
+      DevBuf counts, offsets, tmp;
+      counts.alloc(n_threads * 4);
+      offsets.alloc(n_threads * 8);
+      HIP_OR_THROW(launch_reduce_count(src, chunk, n_threads, gh->dev, gh->lay, nullptr, counts.as<uint32_t>(),
+                                       ctx->stream));
+      size_t tb = 0;
+      HIP_OR_THROW(exclusive_scan_u32_to_u64(counts.as<uint32_t>(), offsets.as<uint64_t>(), n_threads, nullptr, tb,
+                                             ctx->stream));
+      tmp.alloc(tb);
+      HIP_OR_THROW(exclusive_scan_u32_to_u64(counts.as<uint32_t>(), offsets.as<uint64_t>(), n_threads, tmp.p, tb,
+                                             ctx->stream));
+      uint64_t last_off = 0;
+      uint32_t last_cnt = 0;
+      HIP_OR_THROW(hipMemcpyAsync(&last_off, offsets.as<uint64_t>() + (n_threads - 1), 8, hipMemcpyDeviceToHost,
+                                  ctx->stream));
+      HIP_OR_THROW(hipMemcpyAsync(&last_cnt, counts.as<uint32_t>() + (n_threads - 1), 4, hipMemcpyDeviceToHost,
+                                  ctx->stream));
+      ctx->sync();
+      uint64_t D = last_off + last_cnt;
+      sp->dim = (int64_t)D;
+      sp->keys.alloc(std::max<uint64_t>(D, 1) * 8);
+      sp->orbit.alloc(std::max<uint64_t>(D, 1) * 2);
+      HIP_OR_THROW(launch_reduce_write(src, chunk, n_threads, gh->dev, gh->lay, offsets.as<uint64_t>(),
+                                       sp->keys.as<uint64_t>(), sp->orbit.as<uint16_t>(), ctx->stream));
+      ctx->sync();
+    }
+    sp->materialized = true;
+    sp->group = std::move(gh);
+    sp->build_index();
+    ctx->sync();
+  });
+  if (rc == EDHIP_OK) *out = sp.release();
+  return rc;
+}
+
+void edhip_space_destroy(edhip_space* s) {
+  if (!s) return;
+  if (s->ctx) {
+    (void)hipSetDevice(s->ctx->device);
+    (void)hipStreamSynchronize(s->ctx->stream);
+  }
+  delete s;
+}
+
+int edhip_space_dim(const edhip_space* s, int64_t* dim) {
+  if (!s || !dim) return EDHIP_ERR_ARGUMENT;
+  *dim = s->dim;
+  return EDHIP_OK;
+}
+
+int edhip_space_copy_basis(const edhip_space* s, uint64_t* out) {
+  if (!s || (!out && s->dim)) return EDHIP_ERR_ARGUMENT;
+  return guarded(s->ctx, [&] {
+    s->materialize();
+    if (s->dim)
+      HIP_OR_THROW(hipMemcpyAsync(out, s->keys.p, (size_t)s->dim * 8, hipMemcpyDeviceToHost, s->ctx->stream));
+    s->ctx->sync();
+  });
+}
+
+int edhip_space_device_basis(const edhip_space* s, const uint64_t** out) {
+  if (!s || !out) return EDHIP_ERR_ARGUMENT;
+  return guarded(s->ctx, [&] {
+    s->materialize();
+    *out = s->keys.as<uint64_t>();
+  });
+}
+
+int edhip_space_copy_orbit(const edhip_space* s, uint16_t* out) {
+  if (!s || (!out && s->dim)) return EDHIP_ERR_ARGUMENT;
+  return guarded(s->ctx, [&] {
+    if (!s->group) {
+      for (int64_t i = 0; i < s->dim; ++i) out[i] = 1;
+      return;
+    }
+    if (s->dim)
+      HIP_OR_THROW(hipMemcpyAsync(out, s->orbit.p, (size_t)s->dim * 2, hipMemcpyDeviceToHost, s->ctx->stream));
+    s->ctx->sync();
+  });
+}
+
+int edhip_space_lookup(const edhip_space* s, int64_t n, const uint64_t* keys, int64_t* idx) {
+  if (!s || n < 0 || (n && (!keys || !idx))) return EDHIP_ERR_ARGUMENT;
+  return guarded(s->ctx, [&] {
+    if (n == 0) return;
+    DevBuf dk, di;
+    dk.alloc((size_t)n * 8);
+    di.alloc((size_t)n * 8);
+    HIP_OR_THROW(hipMemcpyAsync(dk.p, keys, (size_t)n * 8, hipMemcpyHostToDevice, s->ctx->stream));
+    HIP_OR_THROW(launch_lookup(s->index_dev(), n, dk.as<uint64_t>(), di.as<int64_t>(), s->ctx->stream));
+    HIP_OR_THROW(hipMemcpyAsync(idx, di.p, (size_t)n * 8, hipMemcpyDeviceToHost, s->ctx->stream));
+    s->ctx->sync();
+  });
+}
+
+int edhip_space_export_mapping(const edhip_space* s, int64_t* idx, double* are, double* aim) {
+  if (!s) return EDHIP_ERR_ARGUMENT;
+  return guarded(s->ctx, [&] {
+    if (!s->group || !s->parent) fail(EDHIP_ERR_ARGUMENT, "space is not symmetry reduced");
+    const edhip_space* p = s->parent;
+    const int64_t np = p->dim;
+    if (np == 0) return;
+    if (!idx || !are || !aim) fail(EDHIP_ERR_ARGUMENT, "null output");
+    DevBuf di, dr, dm;
+    di.alloc((size_t)np * 8);
+    dr.alloc((size_t)np * 8);
+    dm.alloc((size_t)np * 8);
+    ReduceSource src = p->source();
+    HIP_OR_THROW(launch_export_mapping(src, s->group->dev, s->group->lay, s->index_dev(), s->orbit.as<uint16_t>(),
+                                       di.as<int64_t>(), dr.as<double>(), dm.as<double>(), s->ctx->stream));
+    HIP_OR_THROW(hipMemcpyAsync(idx, di.p, (size_t)np * 8, hipMemcpyDeviceToHost, s->ctx->stream));
+    HIP_OR_THROW(hipMemcpyAsync(are, dr.p, (size_t)np * 8, hipMemcpyDeviceToHost, s->ctx->stream));
+    HIP_OR_THROW(hipMemcpyAsync(aim, dm.p, (size_t)np * 8, hipMemcpyDeviceToHost, s->ctx->stream));
+    s->ctx->sync();
+  });
+}
+
+int edhip_space_info(const edhip_space* s, int* reduced, int* group_order, int* canon_kind) {
+  if (!s) return EDHIP_ERR_ARGUMENT;
+  if (reduced) *reduced = s->group ? 1 : 0;
+  if (group_order) *group_order = s->group ? s->group->G : 1;
+  if (canon_kind) *canon_kind = s->group ? s->group->mode : 0;
+  return EDHIP_OK;
+}
+
+// ---- operators --------------------------------------------------------------
+int edhip_op_create(edhip_ctx* ctx, const edhip_space* space, int64_t n_terms, const uint64_t* mask,
+                    const uint64_t* row, const uint64_t* col, const double* are, const double* aim, edhip_op** out) {
+  if (!ctx || !out) return EDHIP_ERR_ARGUMENT;
+  *out = nullptr;
+  auto op = std::make_unique<edhip_op>();
+  int rc = guarded(ctx, [&] {
+    check_space(space);
+    if (n_terms < 0 || (n_terms > 0 && (!mask || !row || !col || !are || !aim)))
+      fail(EDHIP_ERR_ARGUMENT, "bad term arrays");
+    op->ctx = ctx;
+    op->space = space;
+    op->n_terms = n_terms;
+    op->mask.assign(mask, mask + n_terms);
+    op->row.assign(row, row + n_terms);
+    op->col.assign(col, col + n_terms);
+    op->are.assign(are, are + n_terms);
+    op->aim.assign(aim, aim + n_terms);
+    std::vector<OffTerm> offr, offc, allr, allc;
+    std::vector<DiagTerm> dg;
+    for (int64_t t = 0; t < n_terms; ++t) {
+      if ((~mask[t]) & row[t]) fail(EDHIP_ERR_ARGUMENT, "every bit of bitrow not in bitmask should be set to zero");
+      if ((~mask[t]) & col[t]) fail(EDHIP_ERR_ARGUMENT, "every bit of bitcol not in bitmask should be set to zero");
+      if (aim[t] != 0.0) op->complex_amp = true;
+      allr.push_back(OffTerm{mask[t], row[t], col[t], are[t], aim[t]});
+      allc.push_back(OffTerm{mask[t], col[t], row[t], are[t], aim[t]});
+      if (row[t] == col[t]) {
+        dg.push_back(DiagTerm{mask[t], row[t], are[t], aim[t]});
+        op->diag_ids.push_back(t);
+      } else {
+        offr.push_back(OffTerm{mask[t], row[t], col[t], are[t], aim[t]});
+        offc.push_back(OffTerm{mask[t], col[t], row[t], are[t], aim[t]});
+        op->off_ids.push_back(t);
+      }
+    }
+    op->n_off = (int)offr.size();
+    op->n_diag = (int)dg.size();
+    upload(op->off_row, offr, ctx->stream);
+    upload(op->off_col, offc, ctx->stream);
+    upload(op->diag, dg, ctx->stream);
+    upload(op->all_row, allr, ctx->stream);
+    upload(op->all_col, allc, ctx->stream);
+    op->dev_row = make_opdev(op.get(), false);
+    op->dev_col = make_opdev(op.get(), true);
+    ctx->sync();
+  });
+  if (rc == EDHIP_OK) *out = op.release();
+  return rc;
+}
+
+void edhip_op_destroy(edhip_op* op) {
+  if (!op) return;
+  (void)hipSetDevice(op->ctx->device);
+  (void)hipStreamSynchronize(op->ctx->stream);
+  delete op;
+}
+
+int edhip_op_is_complex(const edhip_op* op, int* c) {
+  if (!op || !c) return EDHIP_ERR_ARGUMENT;
+  *c = op->is_complex() ? 1 : 0;
+  return EDHIP_OK;
+}
+
+int edhip_apply(edhip_op* op, int prec, int transpose, int accumulate, int64_t n_x, const void* x, int64_t n_y,
+                void* y) {
+  if (!op) return EDHIP_ERR_ARGUMENT;
+  return guarded(op->ctx, [&] {
+    check_prec_for_op(op, prec);
+    const int64_t d = op->space->dim;
+    if (n_y != d) fail(EDHIP_ERR_DIMENSION, "out has length " + std::to_string(n_y) + " != dimension " + std::to_string(d));
+    if (n_x != d)
+      fail(EDHIP_ERR_DIMENSION, "state has length " + std::to_string(n_x) + " != dimension " + std::to_string(d));
+    HIP_OR_THROW(launch_matvec(transpose ? op->dev_col : op->dev_row, prec, transpose ? 1 : 0, accumulate ? 1 : 0, x, y,
+                               op->ctx->stream));
+  });
+}
+
+int edhip_apply_host(edhip_op* op, int prec, int transpose, int accumulate, int64_t n_x, const void* x, int64_t n_y,
+                     void* y) {
+  if (!op) return EDHIP_ERR_ARGUMENT;
+  return guarded(op->ctx, [&] {
+    check_prec_for_op(op, prec);
+    const int64_t d = op->space->dim;
+    if (n_y != d) fail(EDHIP_ERR_DIMENSION, "out has length " + std::to_string(n_y) + " != dimension " + std::to_string(d));
+    if (n_x != d)
+      fail(EDHIP_ERR_DIMENSION, "state has length " + std::to_string(n_x) + " != dimension " + std::to_string(d));
+    const size_t bytes = (size_t)d * elem_size(prec);
+    op->hx.ensure(bytes);
+    op->hy.ensure(bytes);
+    hipStream_t st = op->ctx->stream;
+    if (d) {
+      HIP_OR_THROW(hipMemcpyAsync(op->hx.p, x, bytes, hipMemcpyHostToDevice, st));
+      if (accumulate) HIP_OR_THROW(hipMemcpyAsync(op->hy.p, y, bytes, hipMemcpyHostToDevice, st));
+      HIP_OR_THROW(launch_matvec(transpose ? op->dev_col : op->dev_row, prec, transpose ? 1 : 0, accumulate ? 1 : 0,
+                                 op->hx.p, op->hy.p, st));
+      HIP_OR_THROW(hipMemcpyAsync(y, op->hy.p, bytes, hipMemcpyDeviceToHost, st));
+    }
+    op->ctx->sync();
+  });
+}
+
+int edhip_iterate(edhip_op* op, int transpose, int64_t n_rows, const int64_t* rows, int64_t* idx, double* re,
+                  double* im) {
+  if (!op) return EDHIP_ERR_ARGUMENT;
+  return guarded(op->ctx, [&] {
+    if (n_rows < 0 || (n_rows && (!rows || !idx || !re || !im))) fail(EDHIP_ERR_ARGUMENT, "bad arguments");
+    for (int64_t k = 0; k < n_rows; ++k)
+      if (rows[k] < 0 || rows[k] >= op->space->dim) fail(EDHIP_ERR_BOUNDS, "row index out of bounds");
+    if (n_rows == 0 || op->n_terms == 0) {
+      for (int64_t k = 0; k < n_rows * op->n_terms; ++k) idx[k] = -2;
+      return;
+    }
+    hipStream_t st = op->ctx->stream;
+    const size_t nt = (size_t)n_rows * op->n_terms;
+    op->iter_rows.ensure((size_t)n_rows * 8);
+    op->iter_idx.ensure(nt * 8);
+    op->iter_re.ensure(nt * 8);
+    op->iter_im.ensure(nt * 8);
+    HIP_OR_THROW(hipMemcpyAsync(op->iter_rows.p, rows, (size_t)n_rows * 8, hipMemcpyHostToDevice, st));
+    HIP_OR_THROW(launch_iterate(transpose ? op->dev_col : op->dev_row,
+                                transpose ? op->all_col.as<OffTerm>() : op->all_row.as<OffTerm>(), (int)op->n_terms,
+                                transpose, n_rows, op->iter_rows.as<int64_t>(), op->iter_idx.as<int64_t>(),
+                                op->iter_re.as<double>(), op->iter_im.as<double>(), st));
+    HIP_OR_THROW(hipMemcpyAsync(idx, op->iter_idx.p, nt * 8, hipMemcpyDeviceToHost, st));
+    HIP_OR_THROW(hipMemcpyAsync(re, op->iter_re.p, nt * 8, hipMemcpyDeviceToHost, st));
+    HIP_OR_THROW(hipMemcpyAsync(im, op->iter_im.p, nt * 8, hipMemcpyDeviceToHost, st));
+    op->ctx->sync();
+  });
+}
+
+int edhip_sparse_build(edhip_op* op, double tol, int64_t* nnz) {
+  if (!op || !nnz) return EDHIP_ERR_ARGUMENT;
+  return guarded(op->ctx, [&] {
+    hipStream_t st = op->ctx->stream;
+    const int64_t d = op->space->dim;
+    const int slot = 1 + op->n_off;
+    DevBuf sidx, sre, sim, cnt, tmp;
+    sidx.alloc((size_t)d * slot * 8);
+    sre.alloc((size_t)d * slot * 8);
+    sim.alloc((size_t)d * slot * 8);
+    cnt.alloc((size_t)(d + 1) * 8);
+    HIP_OR_THROW(hipMemsetAsync(cnt.p, 0, (size_t)(d + 1) * 8, st));
+    HIP_OR_THROW(launch_sparse_columns(op->dev_col, tol, slot, sidx.as<int64_t>(), sre.as<double>(), sim.as<double>(),
+                                       cnt.as<int64_t>(), st));
+    op->sp_colptr.alloc((size_t)(d + 1) * 8);
+    size_t tb = 0;
+    HIP_OR_THROW(exclusive_scan_i64(cnt.as<int64_t>(), op->sp_colptr.as<int64_t>(), (uint64_t)(d + 1), nullptr, tb, st));
+    tmp.alloc(tb);
+    HIP_OR_THROW(exclusive_scan_i64(cnt.as<int64_t>(), op->sp_colptr.as<int64_t>(), (uint64_t)(d + 1), tmp.p, tb, st));
+    int64_t total = 0;
+    HIP_OR_THROW(hipMemcpyAsync(&total, op->sp_colptr.as<int64_t>() + d, 8, hipMemcpyDeviceToHost, st));
+    op->ctx->sync();
+    op->sp_rowval.alloc((size_t)std::max<int64_t>(total, 1) * 8);
+    op->sp_re.alloc((size_t)std::max<int64_t>(total, 1) * 8);
+    op->sp_im.alloc((size_t)std::max<int64_t>(total, 1) * 8);
+    HIP_OR_THROW(launch_sparse_compact(d, slot, op->sp_colptr.as<int64_t>(), sidx.as<int64_t>(), sre.as<double>(),
+                                       sim.as<double>(), op->sp_rowval.as<int64_t>(), op->sp_re.as<double>(),
+                                       op->sp_im.as<double>(), st));
+    op->ctx->sync();
+    op->sp_nnz = total;
+    *nnz = total;
+  });
+}
+
+int edhip_sparse_export(edhip_op* op, int64_t* colptr, int64_t* rowval, double* re, double* im) {
+  if (!op) return EDHIP_ERR_ARGUMENT;
+  return guarded(op->ctx, [&] {
+    if (op->sp_nnz < 0) fail(EDHIP_ERR_ARGUMENT, "call edhip_sparse_build first");
+    hipStream_t st = op->ctx->stream;
+    const int64_t d = op->space->dim;
+    HIP_OR_THROW(hipMemcpyAsync(colptr, op->sp_colptr.p, (size_t)(d + 1) * 8, hipMemcpyDeviceToHost, st));
+    if (op->sp_nnz) {
+      HIP_OR_THROW(hipMemcpyAsync(rowval, op->sp_rowval.p, (size_t)op->sp_nnz * 8, hipMemcpyDeviceToHost, st));
+      HIP_OR_THROW(hipMemcpyAsync(re, op->sp_re.p, (size_t)op->sp_nnz * 8, hipMemcpyDeviceToHost, st));
+      HIP_OR_THROW(hipMemcpyAsync(im, op->sp_im.p, (size_t)op->sp_nnz * 8, hipMemcpyDeviceToHost, st));
+    }
+    op->ctx->sync();
+  });
+}
+
+// ---- eigensolver -----------------------------------------------------------------
+void edhip_lanczos_default_opts(edhip_lanczos_opts* o) {
+  if (!o) return;
+  std::memset(o, 0, sizeof(*o));
+  o->nev = 1;
+  o->max_basis = 120;
+  o->max_restarts = 50;
+  o->mode = EDHIP_LANCZOS_FP64;
+  o->want_vectors = 0;
+  o->refine_basis = 8;
+  o->max_refine = 10;
+  o->check_every = 5;
+  o->seed = 20241202ull;
+  o->tol = 1e-10;
+}
+
+int edhip_lanczos(edhip_op* op, const edhip_lanczos_opts* opts, double* evals, void* evecs, edhip_lanczos_stats* stats) {
+  if (!op || !opts || !evals) return EDHIP_ERR_ARGUMENT;
+  return guarded(op->ctx, [&] {
+    Timer total;
+    edhip_ctx* ctx = op->ctx;
+    const int64_t n = op->space->dim;
+    const bool cx = op->is_complex();
+    const int nev = opts->nev;
+    if (nev < 1 || nev > 16) fail(EDHIP_ERR_ARGUMENT, "nev must be in 1..16");
+    if (n < nev) fail(EDHIP_ERR_ARGUMENT, "nev exceeds the dimension");
+    if (opts->max_basis < nev + 2 && opts->max_basis < n) fail(EDHIP_ERR_ARGUMENT, "max_basis too small for nev");
+    const int prec64 = cx ? EDHIP_C128 : EDHIP_F64;
+    const int prec_store = (opts->mode == EDHIP_LANCZOS_MIXED) ? (cx ? EDHIP_C64 : EDHIP_F32) : prec64;
+    edhip_lanczos_stats st{};
+    std::vector<double> theta, res;
+
+    TRLanczos L;
+    L.op = op;
+    L.prec = prec_store;
+    L.n = n;
+    L.m = (int)std::min<int64_t>(opts->max_basis, n);
+    L.V.alloc((size_t)(L.m + 1) * n * elem_size(prec_store));
+    HIP_OR_THROW(vec_gaussian(prec_store, n, opts->seed, L.vec(0), ctx->stream));
+    double nrm = std::sqrt(L.dot(L.vec(0), L.vec(0)).real());
+    L.scale(L.vec(0), 1.0 / nrm);
+
+    // Ritz vectors of the main run, FP64 (or C128)
+    DevBuf ritz;
+    ritz.alloc((size_t)nev * n * elem_size(prec64));
+    bool conv = L.run(nev, opts->mode == EDHIP_LANCZOS_MIXED ? std::max(opts->tol, 1e-6) : opts->tol,
+                      opts->max_restarts, opts->check_every, theta, res, ritz.p, prec64);
+    st.iterations = L.iterations;
+    st.restarts = L.restarts;
+    st.matvecs = L.matvecs;
+    st.t_matvec = L.t_matvec;
+    for (int i = 0; i < nev && i < 16; ++i) st.unrefined[i] = theta[i];
+    L.V.release();
+    L.tmp.release();
+
+    if (opts->mode == EDHIP_LANCZOS_MIXED) {
+      Timer tr;
+      conv = true;
+      // FP64 refinement with locking: eigenpair i is refined by short FP64
+      // thick-restart cycles started from its mixed-precision Ritz vector,
+      // orthogonal to the already refined ones.
+      for (int i = 0; i < nev; ++i) {
+        TRLanczos R;
+        R.op = op;
+        R.prec = prec64;
+        R.n = n;
+        R.m = (int)std::min<int64_t>(std::max(opts->refine_basis, 2), n - i);
+        R.V.alloc((size_t)(R.m + 1) * n * elem_size(prec64));
+        R.locked = ritz.p;
+        R.n_locked = i;
+        void* start = (char*)ritz.p + (size_t)i * n * elem_size(prec64);
+        HIP_OR_THROW(hipMemcpyAsync(R.vec(0), start, (size_t)n * elem_size(prec64), hipMemcpyDeviceToDevice,
+                                    ctx->stream));
+        if (i) R.orthogonalize(ritz.p, i, R.vec(0));
+        double nr = std::sqrt(R.dot(R.vec(0), R.vec(0)).real());
+        R.scale(R.vec(0), 1.0 / nr);
+        std::vector<double> th1, rs1;
+        bool ok = R.run(1, opts->tol, std::max(opts->max_refine - 1, 0), std::max(1, R.m), th1, rs1, start, prec64);
+        theta[i] = th1[0];
+        res[i] = rs1.empty() ? 0.0 : rs1[0];
+        st.matvecs_fp64 += R.matvecs;
+        conv = conv && ok;
+      }
+      st.t_refine = tr.sec();
+    }
+    for (int i = 0; i < nev; ++i) {
+      evals[i] = theta[i];
+      if (i < 16) st.residual[i] = res.size() > (size_t)i ? res[i] / std::max(1.0, std::fabs(theta[i])) : 0.0;
+    }
+    if (evecs && opts->want_vectors)
+      HIP_OR_THROW(hipMemcpyAsync(evecs, ritz.p, (size_t)nev * n * elem_size(prec64), hipMemcpyDeviceToDevice,
+                                  ctx->stream));
+    ctx->sync();
+    st.converged = conv ? 1 : 0;
+    st.t_total = total.sec();
+    if (stats) *stats = st;
+    if (!conv) fail(EDHIP_ERR_NOCONV, "Lanczos did not converge within max_restarts");
+  });
+}
+
+// ---- differentiation ------------------------------------------------------------------
+int edhip_term_expect(edhip_op* op, int prec, const void* phi, const void* psi, double* out) {
+  if (!op || !out) return EDHIP_ERR_ARGUMENT;
+  return guarded(op->ctx, [&] {
+    check_prec_for_op(op, prec);
+    const int nt = op->n_diag + op->n_off;
+    if (nt == 0) return;
+    int64_t nb = term_expect_blocks(op->dev_row);
+    op->ex_partial.ensure((size_t)nb * nt * 2 * 8);
+    op->ex_out.ensure((size_t)nt * 2 * 8);
+    uint32_t lds = op->dev_row.lds_total + (uint32_t)(4 * nt * 2 * 8);
+    if (lds > 64 * 1024) fail(EDHIP_ERR_ARGUMENT, "too many terms for the fused expectation kernel");
+    HIP_OR_THROW(launch_term_expect(op->dev_row, prec, phi, psi, op->ex_partial.as<double>(), nb,
+                                    op->ex_out.as<double>(), op->ctx->stream));
+    std::vector<double> h((size_t)nt * 2);
+    HIP_OR_THROW(hipMemcpyAsync(h.data(), op->ex_out.p, h.size() * 8, hipMemcpyDeviceToHost, op->ctx->stream));
+    op->ctx->sync();
+    for (int64_t t = 0; t < op->n_terms; ++t) out[2 * t] = out[2 * t + 1] = 0.0;
+    for (int i = 0; i < op->n_diag; ++i) {
+      out[2 * op->diag_ids[i]] = h[2 * (size_t)i];
+      out[2 * op->diag_ids[i] + 1] = h[2 * (size_t)i + 1];
+    }
+    for (int i = 0; i < op->n_off; ++i) {
+      out[2 * op->off_ids[i]] = h[2 * (size_t)(op->n_diag + i)];
+      out[2 * op->off_ids[i] + 1] = h[2 * (size_t)(op->n_diag + i) + 1];
+    }
+  });
+}
+
+int edhip_minres(edhip_op* op, int prec, double shift, int n_deflate, const void* deflate, const void* b, void* x,
+                 double tol, int max_iter, edhip_minres_stats* stats) {
+  if (!op || !b || !x) return EDHIP_ERR_ARGUMENT;
+  return guarded(op->ctx, [&] {
+    if (prec != EDHIP_F64 && prec != EDHIP_C128) fail(EDHIP_ERR_ARGUMENT, "MINRES runs in FP64 or ComplexF64");
+    check_prec_for_op(op, prec);
+    if (n_deflate < 0 || (n_deflate > 0 && !deflate)) fail(EDHIP_ERR_ARGUMENT, "bad deflation vectors");
+    edhip_ctx* ctx = op->ctx;
+    hipStream_t s = ctx->stream;
+    const int64_t n = op->space->dim;
+    const size_t bytes = (size_t)n * elem_size(prec);
+    TRLanczos H;  // reuse the vector helpers
+    H.op = op;
+    H.prec = prec;
+    H.n = n;
+    DevBuf buf;
+    buf.alloc(bytes * 7);
+    auto V = [&](int i) { return (void*)((char*)buf.p + (size_t)i * bytes); };
+    void *r1 = V(0), *r2 = V(1), *y = V(2), *v = V(3), *w = V(4), *w1 = V(5), *w2 = V(6);
+    auto project = [&](void* z) {
+      if (n_deflate) H.orthogonalize(deflate, n_deflate, z);
+    };
+    auto apply_A = [&](const void* in, void* out) {
+      HIP_OR_THROW(launch_matvec(op->dev_row, prec, 0, 0, in, out, s));
+      if (shift != 0.0) HIP_OR_THROW(vec_axpby(prec, n, -shift, 0.0, in, 1.0, 0.0, out, s));
+      project(out);
+    };
+    // x = 0; r1 = P b
+    HIP_OR_THROW(vec_fill_zero(prec, n, x, s));
+    HIP_OR_THROW(hipMemcpyAsync(r1, b, bytes, hipMemcpyDeviceToDevice, s));
+    project(r1);
+    HIP_OR_THROW(hipMemcpyAsync(y, r1, bytes, hipMemcpyDeviceToDevice, s));
+    HIP_OR_THROW(hipMemcpyAsync(r2, r1, bytes, hipMemcpyDeviceToDevice, s));
+    HIP_OR_THROW(vec_fill_zero(prec, n, w, s));
+    HIP_OR_THROW(vec_fill_zero(prec, n, w2, s));
+    double beta1 = std::sqrt(std::max(H.dot(r1, r1).real(), 0.0));
+    edhip_minres_stats out{};
+    if (beta1 == 0.0) {
+      out.converged = 1;
+      if (stats) *stats = out;
+      return;
+    }
+    double oldb = 0.0, beta = beta1, dbar = 0.0, epsln = 0.0, phibar = beta1, cs = -1.0, sn = 0.0;
+    int itn = 0;
+    bool conv = false;
+    for (itn = 1; itn <= max_iter; ++itn) {
+      // v = y / beta ; y = A v - (beta/oldb) r1
+      HIP_OR_THROW(vec_axpby(prec, n, 1.0 / beta, 0.0, y, 0.0, 0.0, v, s));
+      apply_A(v, y);
+      if (itn >= 2) HIP_OR_THROW(vec_axpby(prec, n, -beta / oldb, 0.0, r1, 1.0, 0.0, y, s));
+      double alfa = H.dot(v, y).real();
+      HIP_OR_THROW(vec_axpby(prec, n, -alfa / beta, 0.0, r2, 1.0, 0.0, y, s));
+      // r1 = r2 ; r2 = y
+      std::swap(r1, r2);
+      HIP_OR_THROW(hipMemcpyAsync(r2, y, bytes, hipMemcpyDeviceToDevice, s));
+      oldb = beta;
+      beta = std::sqrt(std::max(H.dot(y, y).real(), 0.0));
+      // plane rotation
+      double oldeps = epsln;
+      double delta = cs * dbar + sn * alfa;
+      double gbar = sn * dbar - cs * alfa;
+      epsln = sn * beta;
+      dbar = -cs * beta;
+      double gamma = std::max(std::hypot(gbar, beta), 1e-300);
+      cs = gbar / gamma;
+      sn = beta / gamma;
+      double phi = cs * phibar;
+      phibar = sn * phibar;
+      // w1 = w2 ; w2 = w ; w = (v - oldeps w1 - delta w2) / gamma ; x += phi w
+      std::swap(w1, w2);  // w1 <- old w2
+      std::swap(w2, w);   // w2 <- old w, w <- old w1 (scratch)
+      HIP_OR_THROW(vec_axpby(prec, n, 1.0 / gamma, 0.0, v, 0.0, 0.0, w, s));
+      HIP_OR_THROW(vec_axpby(prec, n, -oldeps / gamma, 0.0, w1, 1.0, 0.0, w, s));
+      HIP_OR_THROW(vec_axpby(prec, n, -delta / gamma, 0.0, w2, 1.0, 0.0, w, s));
+      HIP_OR_THROW(vec_axpby(prec, n, phi, 0.0, w, 1.0, 0.0, x, s));
+      out.residual = phibar / beta1;
+      if (phibar <= tol * beta1) {
+        conv = true;
+        break;
+      }
+      if (beta == 0.0) {
+        conv = true;
+        break;
+      }
+    }
+    project(x);
+    ctx->sync();
+    out.iterations = std::min(itn, max_iter);
+    out.converged = conv ? 1 : 0;
+    if (stats) *stats = out;
+    if (!conv) fail(EDHIP_ERR_NOCONV, "MINRES did not converge");
+  });
+}
+
+// ---- vector helpers ------------------------------------------------------------
+int edhip_vec_dot(edhip_ctx* ctx, int prec, int64_t n, const void* x, const void* y, double* out) {
+  if (!ctx || !out) return EDHIP_ERR_ARGUMENT;
+  return guarded(ctx, [&] {
+    if (elem_size(prec) == 0) fail(EDHIP_ERR_ARGUMENT, "unknown precision code");
+    double* part = ctx->partial_buf(2 * (size_t)vec_partial_blocks(n));
+    double* o = ctx->small_buf(2);
+    HIP_OR_THROW(vec_dot(prec, n, x, y, part, o, ctx->stream));
+    HIP_OR_THROW(hipMemcpyAsync(out, o, 16, hipMemcpyDeviceToHost, ctx->stream));
+    ctx->sync();
+  });
+}
+
+int edhip_vec_gaussian(edhip_ctx* ctx, int prec, int64_t n, uint64_t seed, void* x) {
+  if (!ctx) return EDHIP_ERR_ARGUMENT;
+  return guarded(ctx, [&] {
+    if (elem_size(prec) == 0) fail(EDHIP_ERR_ARGUMENT, "unknown precision code");
+    HIP_OR_THROW(vec_gaussian(prec, n, seed, x, ctx->stream));
+  });
+}
+
+int edhip_vec_convert(edhip_ctx* ctx, int pi, int po, int64_t n, const void* x, void* y) {
+  if (!ctx) return EDHIP_ERR_ARGUMENT;
+  return guarded(ctx, [&] {
+    if (elem_size(pi) == 0 || elem_size(po) == 0) fail(EDHIP_ERR_ARGUMENT, "unknown precision code");
+    HIP_OR_THROW(vec_convert(pi, po, n, x, y, ctx->stream));
+  });
+}
+
+int edhip_event_record(edhip_ctx* ctx, int slot) {
+  if (!ctx || slot < 0 || slot >= 16) return EDHIP_ERR_ARGUMENT;
+  return guarded(ctx, [&] { HIP_OR_THROW(hipEventRecord(ctx->ev[slot], ctx->stream)); });
+}
+
+int edhip_event_elapsed(edhip_ctx* ctx, int s0, int s1, double* ms) {
+  if (!ctx || !ms || s0 < 0 || s0 >= 16 || s1 < 0 || s1 >= 16) return EDHIP_ERR_ARGUMENT;
+  return guarded(ctx, [&] {
+    HIP_OR_THROW(hipEventSynchronize(ctx->ev[s1]));
+    float f = 0.f;
+    HIP_OR_THROW(hipEventElapsedTime(&f, ctx->ev[s0], ctx->ev[s1]));
+    *ms = f;
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,399 @@
+// Device-side building blocks shared by the edhip kernels (gfx950, wave64).
+//
+//  * sector unranking / ranking by completion counts (the ascending order of
+//    popcount-constrained words, i.e. the order hs_get_basis_list produces,
+//    src/Representation/hilbert_space_representation.jl:127-206);
+//  * the sorted-key lookup (FrozenSortedArrayIndex semantics,
+//    src/frozensortedarray.jl:29-48) as prefix table + branch-free search;
+//  * the two canonicalisers of a word under a symmetry group: a generic one
+//    that applies bit permutations through 4-bit lookup tables staged in LDS,
+//    and a torus-translation one with top-row pruning.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace edhip {
+
+// ----------------------------------------------------------------------------
+// Scalars
+// ----------------------------------------------------------------------------
+template <typename R>
+struct Cx {
+  R re, im;
+};
+
+template <typename V>
+struct VT;
+template <>
+struct VT<double> {
+  using R = double;
+  static constexpr bool cx = false;
+};
+template <>
+struct VT<float> {
+  using R = float;
+  static constexpr bool cx = false;
+};
+template <>
+struct VT<Cx<double>> {
+  using R = double;
+  static constexpr bool cx = true;
+};
+template <>
+struct VT<Cx<float>> {
+  using R = float;
+  static constexpr bool cx = true;
+};
+
+template <typename V>
+__device__ __forceinline__ V vzero() {
+  if constexpr (VT<V>::cx) {
+    return V{0, 0};
+  } else {
+    return V(0);
+  }
+}
+
+// acc += w * x, w a complex weight given as (wr, wi)
+template <typename V, typename R>
+__device__ __forceinline__ void vfma(V& acc, R wr, R wi, const V& x) {
+  if constexpr (VT<V>::cx) {
+    acc.re += wr * x.re - wi * x.im;
+    acc.im += wr * x.im + wi * x.re;
+  } else {
+    (void)wi;
+    acc += wr * x;
+  }
+}
+
+template <typename V>
+__device__ __forceinline__ V vadd(const V& a, const V& b) {
+  if constexpr (VT<V>::cx) {
+    return V{a.re + b.re, a.im + b.im};
+  } else {
+    return a + b;
+  }
+}
+
+// ----------------------------------------------------------------------------
+// Sector counting: W[i*(n+1)+ones] = number of fillings of bits [0,i) such
+// that ones + popcount(filling) is an allowed popcount.
+// ----------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t unrank_word(uint64_t rank, int n, const uint64_t* W) {
+  uint64_t w = 0;
+  int ones = 0;
+  for (int i = n - 1; i >= 0; --i) {
+    uint64_t c0 = W[i * (n + 1) + ones];
+    if (rank >= c0) {
+      rank -= c0;
+      w |= (1ull << i);
+      ++ones;
+    }
+  }
+  return w;
+}
+
+// Rank of w among the allowed words (w must be an allowed word).
+__device__ __forceinline__ uint64_t rank_word(uint64_t w, int n, const uint64_t* W) {
+  uint64_t r = 0;
+  int ones = 0;
+  while (w) {
+    int i = 63 - __builtin_clzll(w);
+    r += W[i * (n + 1) + ones];
+    ++ones;
+    w &= ~(1ull << i);
+  }
+  return r;
+}
+
+// Next larger word of the same popcount (Gosper); v != 0.
+__device__ __forceinline__ uint64_t next_same_popcount(uint64_t v) {
+  uint64_t t = v | (v - 1);
+  return (t + 1) | (((~t & (t + 1)) - 1) >> (__builtin_ctzll(v) + 1));
+}
+
+// ----------------------------------------------------------------------------
+// Lookup in a sorted key array behind a prefix table.
+//   bucket(key) = key >> shift; keys with that bucket live in
+//   [prefix[bucket], prefix[bucket+1]).  Inside, a branch-free binary search.
+// Returns the 0-based index or -1.
+// ----------------------------------------------------------------------------
+struct IndexDev {
+  const uint64_t* keys;
+  const uint32_t* prefix;
+  int64_t dim;
+  uint64_t n_buckets;
+  int shift;
+  int rank_mode;  // 1: the space is a full popcount sector; index == rank
+  int n_bits;
+  int pad_;
+  uint64_t pop_allowed;  // bit p set iff popcount p is allowed (rank mode)
+  const uint64_t* W;     // completion counts (rank mode)
+};
+
+__device__ __forceinline__ int64_t search_sorted(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ prefix,
+                                                 uint64_t n_buckets, int shift, uint64_t key) {
+  uint64_t bucket = key >> shift;
+  if (bucket >= n_buckets) return -1;
+  int64_t base = prefix[bucket];
+  int64_t len = (int64_t)prefix[bucket + 1] - base;
+  if (len <= 0) return -1;
+  while (len > 1) {
+    int64_t half = len >> 1;
+    base = (keys[base + half] <= key) ? base + half : base;
+    len -= half;
+  }
+  return (keys[base] == key) ? base : -1;
+}
+
+__device__ __forceinline__ int64_t index_of(const IndexDev& ix, const uint64_t* W_lds, uint64_t key) {
+  if (ix.rank_mode) {
+    if ((key >> ix.n_bits) != 0) return -1;
+    int pc = __builtin_popcountll(key);
+    if (!((ix.pop_allowed >> pc) & 1ull)) return -1;
+    return (int64_t)rank_word(key, ix.n_bits, W_lds);
+  }
+  return search_sorted(ix.keys, ix.prefix, ix.n_buckets, ix.shift, key);
+}
+
+// ----------------------------------------------------------------------------
+// Symmetry group description on the device.
+// ----------------------------------------------------------------------------
+enum GroupMode : int { kGroupNone = 0, kGroupGeneric = 1, kGroupTorus = 2 };
+
+constexpr int kMaxLy = 64;
+
+struct GroupDev {
+  int mode;
+  int G;
+  int n_nib;  // generic: number of 4-bit chunks of the word
+  int Lx, Ly, nb;
+  int minrot_table;     // torus: 1 if the per-row minimal-rotation table is used
+  int pad_;
+  uint64_t row_mask;    // torus: (1 << Lx) - 1
+  uint64_t full_mask;   // (1 << nb) - 1
+  const uint64_t* nib;  // generic: G * n_nib * 16 images of 4-bit chunks
+  const uint32_t* minrot;  // torus: 2^Lx entries, low 16: min rotation value, high 16: mask of rotations
+  const uint64_t* rot_hi;  // torus: Lx entries, per-row bits kept by a left rotation by a
+  const uint64_t* rot_lo;  // torus: Lx entries, per-row bits wrapped around
+  const int32_t* g_of_ab;  // torus: Lx*Ly -> group index of T(a, b)
+  const int32_t* inv;      // G: index of the inverse element
+  const uint8_t* trivial;  // G: |chi(g) - 1| <= tol
+  const double* ph_re;     // G: phase[h] = conj(chi(inv(h)))
+  const double* ph_im;
+  const double* invsqrt;   // G+1: 1/sqrt(N)
+};
+
+// Pointers to the LDS copies of the group tables (same meaning as GroupDev).
+struct GroupLds {
+  const double* ph_re;
+  const double* ph_im;
+  const double* invsqrt;
+  const uint64_t* nib;
+  const uint32_t* minrot;
+  const uint64_t* rot_hi;
+  const uint64_t* rot_lo;
+  const int32_t* g_of_ab;
+  const int32_t* inv;
+  const uint8_t* trivial;
+};
+
+__device__ __forceinline__ uint64_t perm_apply_nib(const uint64_t* tab, int n_nib, uint64_t b) {
+  uint64_t out = 0;
+  for (int k = 0; k < n_nib; ++k) out |= tab[(k << 4) + (int)((b >> (4 * k)) & 15ull)];
+  return out;
+}
+
+__device__ __forceinline__ uint64_t rows_rotl(uint64_t w, int a, int Lx, const uint64_t* hi, const uint64_t* lo) {
+  if (a == 0) return w;
+  return ((w << a) & hi[a]) | ((w >> (Lx - a)) & lo[a]);
+}
+
+__device__ __forceinline__ uint64_t rows_shift(uint64_t w, int b, int Lx, int nb, uint64_t full) {
+  if (b == 0) return w;
+  int s = b * Lx;
+  return ((w << s) | (w >> (nb - s))) & full;
+}
+
+// minimal left rotation of an Lx-bit row value and the mask of rotations reaching it
+__device__ __forceinline__ void row_minrot(const GroupDev& g, const GroupLds& L, uint64_t v, uint64_t& mv, uint64_t& amask) {
+  if (g.minrot_table) {
+    uint32_t e = L.minrot[(int)v];
+    mv = e & 0xffffu;
+    amask = e >> 16;
+    return;
+  }
+  mv = v;
+  amask = 1;
+  for (int a = 1; a < g.Lx; ++a) {
+    uint64_t r = ((v << a) | (v >> (g.Lx - a))) & g.row_mask;
+    if (r < mv) {
+      mv = r;
+      amask = 1ull << a;
+    } else if (r == mv) {
+      amask |= 1ull << a;
+    }
+  }
+}
+
+// Canonical form s = min_h h(b), an element h with h(b) = s (among ties, the
+// one whose inverse has the largest index: the reference's last writer of the
+// orbit dictionary), and cnt = #{h : h(b) = s} = |Stab(s)|.
+__device__ __forceinline__ void canonicalize(const GroupDev& g, const GroupLds& L, uint64_t b, uint64_t& s, int& h,
+                                             int& cnt) {
+  if (g.mode == kGroupGeneric) {
+    s = b;
+    h = 0;
+    cnt = 1;
+    const int stride = g.n_nib << 4;
+    for (int e = 1; e < g.G; ++e) {
+      uint64_t img = perm_apply_nib(L.nib + e * stride, g.n_nib, b);
+      if (img < s) {
+        s = img;
+        h = e;
+        cnt = 1;
+      } else if (img == s) {
+        ++cnt;
+        if (L.inv[e] > L.inv[h]) h = e;
+      }
+    }
+    return;
+  }
+  if (g.mode == kGroupTorus) {
+    // pass 1: the smallest possible top row
+    uint64_t best_top = ~0ull;
+    for (int y = 0; y < g.Ly; ++y) {
+      uint64_t mv, am;
+      row_minrot(g, L, (b >> (y * g.Lx)) & g.row_mask, mv, am);
+      best_top = mv < best_top ? mv : best_top;
+    }
+    // pass 2: full images of the candidates reaching it
+    s = ~0ull;
+    h = 0;
+    cnt = 0;
+    for (int y = 0; y < g.Ly; ++y) {
+      uint64_t mv, am;
+      row_minrot(g, L, (b >> (y * g.Lx)) & g.row_mask, mv, am);
+      if (mv != best_top) continue;
+      int bb = g.Ly - 1 - y;  // this row lands on top under T(a, bb)
+      uint64_t shifted = rows_shift(b, bb, g.Lx, g.nb, g.full_mask);
+      while (am) {
+        int a = __builtin_ctzll(am);
+        am &= am - 1;
+        uint64_t img = rows_rotl(shifted, a, g.Lx, L.rot_hi, L.rot_lo);
+        int e = L.g_of_ab[a + g.Lx * bb];
+        if (img < s) {
+          s = img;
+          h = e;
+          cnt = 1;
+        } else if (img == s) {
+          ++cnt;
+          if (L.inv[e] > L.inv[h]) h = e;
+        }
+      }
+    }
+    return;
+  }
+  s = b;
+  h = 0;
+  cnt = 1;
+}
+
+// Representative test of symmetry_reduce (symmetry_reduce_generic.jl:56-69):
+// b is kept iff no image is smaller and every element fixing b has a trivial
+// character.  On success orbit = G / |Stab(b)|.
+__device__ __forceinline__ bool is_representative(const GroupDev& g, const GroupLds& L, uint64_t b, int& orbit) {
+  if (g.mode == kGroupGeneric) {
+    int stab = 1;
+    const int stride = g.n_nib << 4;
+    for (int e = 1; e < g.G; ++e) {
+      uint64_t img = perm_apply_nib(L.nib + e * stride, g.n_nib, b);
+      if (img < b) return false;
+      if (img == b) {
+        if (!L.trivial[e]) return false;
+        ++stab;
+      }
+    }
+    orbit = g.G / stab;
+    return true;
+  }
+  if (g.mode == kGroupTorus) {
+    uint64_t top = (b >> ((g.Ly - 1) * g.Lx)) & g.row_mask;
+    uint64_t best_top = ~0ull;
+    for (int y = 0; y < g.Ly; ++y) {
+      uint64_t mv, am;
+      row_minrot(g, L, (b >> (y * g.Lx)) & g.row_mask, mv, am);
+      best_top = mv < best_top ? mv : best_top;
+    }
+    if (best_top < top) return false;
+    int stab = 0;
+    for (int y = 0; y < g.Ly; ++y) {
+      uint64_t mv, am;
+      row_minrot(g, L, (b >> (y * g.Lx)) & g.row_mask, mv, am);
+      if (mv != best_top) continue;
+      int bb = g.Ly - 1 - y;
+      uint64_t shifted = rows_shift(b, bb, g.Lx, g.nb, g.full_mask);
+      while (am) {
+        int a = __builtin_ctzll(am);
+        am &= am - 1;
+        uint64_t img = rows_rotl(shifted, a, g.Lx, L.rot_hi, L.rot_lo);
+        if (img < b) return false;
+        if (img == b) {
+          if (!L.trivial[L.g_of_ab[a + g.Lx * bb]]) return false;
+          ++stab;
+        }
+      }
+    }
+    orbit = g.G / stab;
+    return true;
+  }
+  orbit = 1;
+  return true;
+}
+
+// ----------------------------------------------------------------------------
+// LDS staging of the group tables.  The host computes the byte offsets.
+// ----------------------------------------------------------------------------
+struct GroupLdsLayout {
+  uint32_t off_nib, off_minrot, off_rot_hi, off_rot_lo, off_g_of_ab, off_inv, off_trivial;
+  uint32_t off_ph_re, off_ph_im, off_invsqrt;
+  uint32_t bytes_nib, bytes_minrot, bytes_rot, bytes_g_of_ab, bytes_inv, bytes_trivial;
+  uint32_t bytes_ph, bytes_invsqrt;
+  uint32_t total;
+};
+
+__device__ __forceinline__ void lds_copy(unsigned char* dst, const void* src, uint32_t bytes) {
+  if (!src || bytes == 0) return;
+  const uint32_t n4 = bytes >> 2;  // all staged tables are multiples of 4 bytes
+  const uint32_t* s = (const uint32_t*)src;
+  uint32_t* d = (uint32_t*)dst;
+  for (uint32_t i = threadIdx.x; i < n4; i += blockDim.x) d[i] = s[i];
+}
+
+__device__ __forceinline__ GroupLds stage_group(const GroupDev& g, const GroupLdsLayout& lay, unsigned char* lds) {
+  GroupLds L;
+  lds_copy(lds + lay.off_nib, g.nib, lay.bytes_nib);
+  lds_copy(lds + lay.off_minrot, g.minrot, lay.bytes_minrot);
+  lds_copy(lds + lay.off_rot_hi, g.rot_hi, lay.bytes_rot);
+  lds_copy(lds + lay.off_rot_lo, g.rot_lo, lay.bytes_rot);
+  lds_copy(lds + lay.off_g_of_ab, g.g_of_ab, lay.bytes_g_of_ab);
+  lds_copy(lds + lay.off_inv, g.inv, lay.bytes_inv);
+  lds_copy(lds + lay.off_trivial, g.trivial, lay.bytes_trivial);
+  lds_copy(lds + lay.off_ph_re, g.ph_re, lay.bytes_ph);
+  lds_copy(lds + lay.off_ph_im, g.ph_im, lay.bytes_ph);
+  lds_copy(lds + lay.off_invsqrt, g.invsqrt, lay.bytes_invsqrt);
+  L.ph_re = (const double*)(lds + lay.off_ph_re);
+  L.ph_im = (const double*)(lds + lay.off_ph_im);
+  L.invsqrt = (const double*)(lds + lay.off_invsqrt);
+  L.nib = (const uint64_t*)(lds + lay.off_nib);
+  L.minrot = (const uint32_t*)(lds + lay.off_minrot);
+  L.rot_hi = (const uint64_t*)(lds + lay.off_rot_hi);
+  L.rot_lo = (const uint64_t*)(lds + lay.off_rot_lo);
+  L.g_of_ab = (const int32_t*)(lds + lay.off_g_of_ab);
+  L.inv = (const int32_t*)(lds + lay.off_inv);
+  L.trivial = (const uint8_t*)(lds + lay.off_trivial);
+  return L;
+}
+
+}  // namespace edhip

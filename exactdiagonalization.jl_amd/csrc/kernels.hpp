@@ -1,0 +1,119 @@
+// Host-side launchers of the edhip kernels (implemented in the .hip files).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "device.hpp"
+
+namespace edhip {
+
+// ---- space.hip -------------------------------------------------------------
+// words[i] = unrank(first + i), i < count
+hipError_t launch_enumerate_sector(uint64_t first, uint64_t count, int n_bits, const uint64_t* d_W, uint64_t* d_out,
+                                   hipStream_t st);
+// prefix[j] = lower_bound(keys, j << shift), j <= n_buckets
+hipError_t launch_build_prefix(const uint64_t* d_keys, int64_t dim, int shift, uint64_t n_buckets, uint32_t* d_prefix,
+                               hipStream_t st);
+
+struct ReduceSource {
+  // either a popcount sector (unranked on the fly) or an explicit device list
+  const uint64_t* list;  // nullptr for a sector
+  uint64_t n_words;
+  int n_bits;
+  int single_popcount;   // 1: Gosper stepping is valid inside a chunk
+  const uint64_t* W;     // completion counts (sector)
+};
+
+// pass 0: per-thread counts of representatives; pass 1: write them (ascending)
+hipError_t launch_reduce_count(const ReduceSource& src, uint64_t chunk, uint64_t n_threads, const GroupDev& g,
+                               const GroupLdsLayout& lay, const uint64_t* d_Wsrc, uint32_t* d_counts, hipStream_t st);
+hipError_t launch_reduce_write(const ReduceSource& src, uint64_t chunk, uint64_t n_threads, const GroupDev& g,
+                               const GroupLdsLayout& lay, const uint64_t* d_offsets, uint64_t* d_reps,
+                               uint16_t* d_orbit, hipStream_t st);
+
+// For every parent word p: mapping index (rep index or -1) and amplitude
+// U[p] = phase[h] / sqrt(N_s) (0 when excluded).
+hipError_t launch_export_mapping(const ReduceSource& src, const GroupDev& g, const GroupLdsLayout& lay,
+                                 const IndexDev& reduced_index, const uint16_t* d_orbit, int64_t* d_idx, double* d_re,
+                                 double* d_im, hipStream_t st);
+
+hipError_t launch_lookup(const IndexDev& ix, int64_t n, const uint64_t* d_keys, int64_t* d_idx, hipStream_t st);
+
+hipError_t exclusive_scan_u32_to_u64(const uint32_t* d_in, uint64_t* d_out, uint64_t n, void* d_tmp, size_t& tmp_bytes,
+                                     hipStream_t st);
+hipError_t exclusive_scan_i64(const int64_t* d_in, int64_t* d_out, uint64_t n, void* d_tmp, size_t& tmp_bytes,
+                              hipStream_t st);
+
+// ---- matvec.hip ------------------------------------------------------------
+// Off-diagonal term as the matvec sees it: the row matches iff
+// (b & mask) == match; its image is (b & ~mask) | image.
+struct OffTerm {
+  uint64_t mask, match, image;
+  double are, aim;
+};
+
+struct DiagTerm {
+  uint64_t mask, match;
+  double are, aim;
+};
+
+struct OpDev {
+  IndexDev ix;           // lookup of canonical images
+  GroupDev g;            // symmetry group
+  GroupLdsLayout lay;    // LDS layout of the group tables
+  const uint64_t* rows;  // basis words (the space's sorted keys)
+  const uint16_t* orbit; // orbit sizes (reduced) or nullptr
+  int64_t dim;
+  int n_off;             // number of off-diagonal terms (in the row or column sense)
+  int n_diag;
+  const OffTerm* off;    // row form: match = R, image = C ; column form: match = C, image = R
+  const DiagTerm* diag;
+  uint32_t off_lds;      // LDS offset of the staged off-diagonal terms
+  uint32_t w_lds;        // LDS offset of the staged rank table (rank mode)
+  uint32_t w_bytes;
+  uint32_t lds_total;
+};
+
+enum Prec : int { kF64 = 0, kC128 = 1, kF32 = 2, kC64 = 3 };
+
+// y = (accumulate ? y : 0) + A x   (transpose=0: row form; 1: column form, y = x A)
+hipError_t launch_matvec(const OpDev& op, int prec, int transpose, int accumulate, const void* x, void* y,
+                         hipStream_t st);
+
+// Raw iterator elements (index or -1, value) in term order for the given rows.
+hipError_t launch_iterate(const OpDev& op, const OffTerm* all_terms, int n_all, int transpose, int64_t n_rows,
+                          const int64_t* d_rows, int64_t* d_idx, double* d_re, double* d_im, hipStream_t st);
+
+// Sparse (CSC) build, column form, merge + chop + sort per column.
+hipError_t launch_sparse_columns(const OpDev& op, double tol, int slot, int64_t* d_scratch_idx, double* d_scratch_re,
+                                 double* d_scratch_im, int64_t* d_count, hipStream_t st);
+hipError_t launch_sparse_compact(int64_t dim, int slot, const int64_t* d_colptr, const int64_t* d_scratch_idx,
+                                 const double* d_scratch_re, const double* d_scratch_im, int64_t* d_rowval,
+                                 double* d_re, double* d_im, hipStream_t st);
+
+// Per-term bilinear forms g_t = sum_r conj(phi_r) sum_{images of t} w psi_s,
+// deterministic two-stage reduction.  n_terms = n_diag + n_off.
+hipError_t launch_term_expect(const OpDev& op, int prec, const void* phi, const void* psi, double* d_partial,
+                              int64_t n_blocks, double* d_out, hipStream_t st);
+int64_t term_expect_blocks(const OpDev& op);
+
+// ---- vecops.hip ------------------------------------------------------------
+// All reductions accumulate in double and are deterministic (fixed tree).
+hipError_t vec_dot(int prec, int64_t n, const void* x, const void* y, double* d_partial, double* d_out2,
+                   hipStream_t st);  // <x, y> (conjugated x), d_out2 = {re, im}
+hipError_t vec_multi_dot(int prec, int64_t n, int k, const void* V, int64_t ldv, const void* w, double* d_partial,
+                         double* d_out, hipStream_t st);  // d_out[2j..] = <V_j, w>
+hipError_t vec_multi_axpy(int prec, int64_t n, int k, const void* V, int64_t ldv, const double* d_coef, void* w,
+                          hipStream_t st);  // w -= sum_j coef_j V_j (complex coef)
+hipError_t vec_axpby(int prec, int64_t n, double ar, double ai, const void* x, double br, double bi, void* y,
+                     hipStream_t st);  // y = a x + b y
+hipError_t vec_convert(int prec_in, int prec_out, int64_t n, const void* x, void* y, hipStream_t st);
+hipError_t vec_gaussian(int prec, int64_t n, uint64_t seed, void* x, hipStream_t st);
+hipError_t vec_fill_zero(int prec, int64_t n, void* x, hipStream_t st);
+// out_l = sum_j coef[j*k + l] V_j for l < k (real coefficients, double accumulation)
+hipError_t vec_combine(int prec_v, int prec_out, int64_t n, int m, const void* V, int64_t ldv, int k,
+                       const double* d_coef, void* out, int64_t ldo, hipStream_t st);
+int64_t vec_partial_blocks(int64_t n);
+
+}  // namespace edhip

@@ -1,0 +1,507 @@
+// Matrix-free application of an operator on a (possibly symmetry-reduced)
+// basis, in gather form with no atomics: one lane owns one output entry.
+//
+// Row form (apply!(out, opr, state), abstract_operator_representation.jl:296-316):
+//   y[r] += sum_t [ (b_r & m_t) == R_t ] w * x[s],  image b' = (b_r & ~m_t) | C_t
+// Column form (apply!(out, state, opr), :327-347) swaps R_t and C_t and uses
+// the column element of reduced_operator_representation.jl:88-116.
+//
+// For a reduced basis, b' is canonicalised to its representative s = h(b'),
+// and the element is a * phase[h] * sqrt(N_r / N_s) (row form) with
+// phase[h] = conj(chi(h^-1)); this equals a * U[b'] / U[b_r] of
+// reduced_operator_representation.jl:57-85 with U[g(s)] = conj(chi(g))/sqrt(N_s).
+#include "kernels.hpp"
+
+namespace edhip {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+template <int MODE>
+struct Canon {
+  __device__ __forceinline__ static void run(const GroupDev& g, const GroupLds& L, uint64_t b, uint64_t& s, int& h,
+                                             int& cnt) {
+    if constexpr (MODE == kGroupNone) {
+      s = b;
+      h = 0;
+      cnt = 1;
+    } else {
+      canonicalize(g, L, b, s, h, cnt);
+    }
+  }
+};
+
+struct Staged {
+  GroupLds L;
+  const OffTerm* off;
+  const uint64_t* W;
+};
+
+template <int MODE>
+__device__ __forceinline__ Staged stage_op(const OpDev& op, unsigned char* lds) {
+  Staged S;
+  if constexpr (MODE != kGroupNone) {
+    S.L = stage_group(op.g, op.lay, lds);
+  }
+  lds_copy(lds + op.off_lds, op.off, (uint32_t)(op.n_off * sizeof(OffTerm)));
+  S.off = (const OffTerm*)(lds + op.off_lds);
+  lds_copy(lds + op.w_lds, op.ix.W, op.w_bytes);
+  S.W = (const uint64_t*)(lds + op.w_lds);
+  return S;
+}
+
+// Weight of an off-diagonal element: a * phase^(conj if TRANS) * sqrt(N_own) / sqrt(N_img)
+template <int MODE, bool TRANS>
+__device__ __forceinline__ void element_weight(const OpDev& op, const Staged& S, const OffTerm& t, int h, int cnt,
+                                               double sqrt_own, double& wr, double& wi) {
+  if constexpr (MODE == kGroupNone) {
+    wr = t.are;
+    wi = t.aim;
+  } else {
+    double pr = S.L.ph_re[h];
+    double pi = TRANS ? -S.L.ph_im[h] : S.L.ph_im[h];
+    double sc = sqrt_own * S.L.invsqrt[op.g.G / cnt];
+    wr = (t.are * pr - t.aim * pi) * sc;
+    wi = (t.are * pi + t.aim * pr) * sc;
+  }
+}
+
+template <typename V, int MODE, bool TRANS>
+__global__ __launch_bounds__(kBlock) void k_matvec(OpDev op, const V* __restrict__ x, V* __restrict__ y, int accumulate) {
+  extern __shared__ unsigned char lds[];
+  Staged S = stage_op<MODE>(op, lds);
+  __syncthreads();
+  const int64_t row = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (row >= op.dim) return;
+  using R = typename VT<V>::R;
+  const uint64_t b = op.rows[row];
+
+  // diagonal terms (C == R): the element is the summed amplitude (U[b]/U[b] = 1)
+  double dre = 0.0, dim_ = 0.0;
+  for (int t = 0; t < op.n_diag; ++t) {
+    const DiagTerm d = op.diag[t];
+    if ((b & d.mask) == d.match) {
+      dre += d.are;
+      dim_ += d.aim;
+    }
+  }
+  V acc = vzero<V>();
+  vfma(acc, (R)dre, (R)(TRANS ? dim_ : dim_), x[row]);
+
+  double sqrt_own = 1.0;
+  if constexpr (MODE != kGroupNone) sqrt_own = sqrt((double)op.orbit[row]);
+
+  for (int seg = 0; seg < op.n_off; seg += 64) {
+    const int nseg = (op.n_off - seg) < 64 ? (op.n_off - seg) : 64;
+    uint64_t bits = 0;
+    for (int j = 0; j < nseg; ++j) {
+      const OffTerm& t = S.off[seg + j];
+      bits |= (uint64_t)((b & t.mask) == t.match) << j;
+    }
+    while (bits) {
+      const int j = __builtin_ctzll(bits);
+      bits &= bits - 1;
+      const OffTerm t = S.off[seg + j];
+      const uint64_t img = (b & ~t.mask) | t.image;
+      uint64_t s;
+      int h, cnt;
+      Canon<MODE>::run(op.g, S.L, img, s, h, cnt);
+      const int64_t idx = index_of(op.ix, S.W, s);
+      if (idx >= 0) {
+        double wr, wi;
+        element_weight<MODE, TRANS>(op, S, t, h, cnt, sqrt_own, wr, wi);
+        vfma(acc, (R)wr, (R)wi, x[idx]);
+      }
+    }
+  }
+  y[row] = accumulate ? vadd(y[row], acc) : acc;
+}
+
+template <typename V, int MODE>
+hipError_t launch_mv_t(const OpDev& op, int transpose, int accumulate, const void* x, void* y, hipStream_t st) {
+  dim3 grid((unsigned)((op.dim + kBlock - 1) / kBlock));
+  if (transpose)
+    hipLaunchKernelGGL((k_matvec<V, MODE, true>), grid, dim3(kBlock), op.lds_total, st, op, (const V*)x, (V*)y,
+                       accumulate);
+  else
+    hipLaunchKernelGGL((k_matvec<V, MODE, false>), grid, dim3(kBlock), op.lds_total, st, op, (const V*)x, (V*)y,
+                       accumulate);
+  return hipGetLastError();
+}
+
+template <typename V>
+hipError_t launch_mv_v(const OpDev& op, int transpose, int accumulate, const void* x, void* y, hipStream_t st) {
+  switch (op.g.mode) {
+    case kGroupNone:
+      return launch_mv_t<V, kGroupNone>(op, transpose, accumulate, x, y, st);
+    case kGroupGeneric:
+      return launch_mv_t<V, kGroupGeneric>(op, transpose, accumulate, x, y, st);
+    default:
+      return launch_mv_t<V, kGroupTorus>(op, transpose, accumulate, x, y, st);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Raw iterators (index or -1, value) in term order, for a list of rows.
+// Reduced invalid images carry the bare amplitude, as at
+// reduced_operator_representation.jl:71-74.
+// ---------------------------------------------------------------------------
+template <int MODE, bool TRANS>
+__global__ __launch_bounds__(kBlock) void k_iterate(OpDev op, const OffTerm* __restrict__ terms, int n_terms,
+                                                    int64_t n_rows, const int64_t* __restrict__ rows,
+                                                    int64_t* __restrict__ idx_out, double* __restrict__ re_out,
+                                                    double* __restrict__ im_out) {
+  extern __shared__ unsigned char lds[];
+  Staged S = stage_op<MODE>(op, lds);
+  __syncthreads();
+  const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (k >= n_rows) return;
+  const int64_t row = rows[k];
+  const uint64_t b = op.rows[row];
+  double sqrt_own = 1.0;
+  if constexpr (MODE != kGroupNone) sqrt_own = sqrt((double)op.orbit[row]);
+  int64_t n = 0;
+  const int64_t base = k * (int64_t)n_terms;
+  for (int ti = 0; ti < n_terms; ++ti) {
+    const OffTerm t = terms[ti];
+    if ((b & t.mask) != t.match) continue;
+    const uint64_t img = (b & ~t.mask) | t.image;
+    uint64_t s;
+    int h, cnt;
+    Canon<MODE>::run(op.g, S.L, img, s, h, cnt);
+    const int64_t idx = index_of(op.ix, S.W, s);
+    double wr = t.are, wi = t.aim;
+    if (idx >= 0) element_weight<MODE, TRANS>(op, S, t, h, cnt, sqrt_own, wr, wi);
+    idx_out[base + n] = idx;
+    re_out[base + n] = wr;
+    im_out[base + n] = wi;
+    ++n;
+  }
+  for (int64_t j = n; j < n_terms; ++j) idx_out[base + j] = -2;  // end marker
+}
+
+// ---------------------------------------------------------------------------
+// Sparse export: per column, collect (row, value), merge duplicates, drop
+// |v| < tol, sort rows (sparse_serial, abstract_operator_representation.jl:145-169).
+// ---------------------------------------------------------------------------
+template <int MODE>
+__global__ __launch_bounds__(kBlock) void k_sparse_columns(OpDev op, double tol, int slot, int64_t* __restrict__ sidx,
+                                                           double* __restrict__ sre, double* __restrict__ sim,
+                                                           int64_t* __restrict__ count) {
+  extern __shared__ unsigned char lds[];
+  Staged S = stage_op<MODE>(op, lds);
+  __syncthreads();
+  const int64_t col = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (col >= op.dim) return;
+  const uint64_t b = op.rows[col];
+  int64_t* I = sidx + col * (int64_t)slot;
+  double* Re = sre + col * (int64_t)slot;
+  double* Im = sim + col * (int64_t)slot;
+  int n = 0;
+  double dre = 0.0, dim_ = 0.0;
+  bool any_diag = false;
+  for (int t = 0; t < op.n_diag; ++t) {
+    const DiagTerm d = op.diag[t];
+    if ((b & d.mask) == d.match) {
+      dre += d.are;
+      dim_ += d.aim;
+      any_diag = true;
+    }
+  }
+  if (any_diag) {
+    I[n] = col;
+    Re[n] = dre;
+    Im[n] = dim_;
+    ++n;
+  }
+  double sqrt_own = 1.0;
+  if constexpr (MODE != kGroupNone) sqrt_own = sqrt((double)op.orbit[col]);
+  for (int ti = 0; ti < op.n_off; ++ti) {
+    const OffTerm t = S.off[ti];
+    if ((b & t.mask) != t.match) continue;
+    const uint64_t img = (b & ~t.mask) | t.image;
+    uint64_t s;
+    int h, cnt;
+    Canon<MODE>::run(op.g, S.L, img, s, h, cnt);
+    const int64_t idx = index_of(op.ix, S.W, s);
+    if (idx < 0) continue;
+    double wr, wi;
+    element_weight<MODE, true>(op, S, t, h, cnt, sqrt_own, wr, wi);
+    I[n] = idx;
+    Re[n] = wr;
+    Im[n] = wi;
+    ++n;
+  }
+  // stable insertion sort by row index
+  for (int i = 1; i < n; ++i) {
+    int64_t ki = I[i];
+    double ri = Re[i], ii = Im[i];
+    int j = i - 1;
+    while (j >= 0 && I[j] > ki) {
+      I[j + 1] = I[j];
+      Re[j + 1] = Re[j];
+      Im[j + 1] = Im[j];
+      --j;
+    }
+    I[j + 1] = ki;
+    Re[j + 1] = ri;
+    Im[j + 1] = ii;
+  }
+  // merge duplicates, chop
+  int m = 0;
+  int i = 0;
+  while (i < n) {
+    int64_t r = I[i];
+    double sr = 0.0, si = 0.0;
+    while (i < n && I[i] == r) {
+      sr += Re[i];
+      si += Im[i];
+      ++i;
+    }
+    if (hypot(sr, si) < tol) continue;
+    I[m] = r;
+    Re[m] = sr;
+    Im[m] = si;
+    ++m;
+  }
+  count[col] = m;
+}
+
+__global__ __launch_bounds__(kBlock) void k_sparse_compact(int64_t dim, int slot, const int64_t* __restrict__ colptr,
+                                                           const int64_t* __restrict__ sidx, const double* __restrict__ sre,
+                                                           const double* __restrict__ sim, int64_t* __restrict__ rowval,
+                                                           double* __restrict__ re, double* __restrict__ im) {
+  const int64_t col = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (col >= dim) return;
+  const int64_t o = colptr[col];
+  const int64_t n = colptr[col + 1] - o;
+  for (int64_t k = 0; k < n; ++k) {
+    rowval[o + k] = sidx[col * (int64_t)slot + k];
+    re[o + k] = sre[col * (int64_t)slot + k];
+    im[o + k] = sim[col * (int64_t)slot + k];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Per-term bilinear forms (backward pass):
+//   g_t = sum_r conj(phi_r) * sum_{images of term t from b_r} w_unit * psi_s
+// where w_unit is the element weight with unit amplitude.  One uniform loop
+// over the terms; each term is reduced across the wavefront by xor-shuffles,
+// then across the block in LDS in a fixed order; the per-block partials are
+// summed in a fixed order by k_reduce_partials.  Output is interleaved
+// (re, im) per term, diagonal terms first, then off-diagonal ones.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+template <typename V>
+__device__ __forceinline__ void load_c(const V* p, int64_t i, double& re, double& im) {
+  if constexpr (VT<V>::cx) {
+    re = (double)p[i].re;
+    im = (double)p[i].im;
+  } else {
+    re = (double)p[i];
+    im = 0.0;
+  }
+}
+
+constexpr int kExpectBlocksMax = 2048;
+
+template <typename V, int MODE>
+__global__ __launch_bounds__(kBlock) void k_term_expect(OpDev op, const V* __restrict__ phi, const V* __restrict__ psi,
+                                                        double* __restrict__ partial) {
+  extern __shared__ unsigned char lds[];
+  Staged S = stage_op<MODE>(op, lds);
+  const int n_terms = op.n_diag + op.n_off;
+  double* acc = (double*)(lds + op.lds_total);  // [kBlock/64 waves][n_terms][2]
+  const int wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  for (int i = threadIdx.x; i < (kBlock / 64) * n_terms * 2; i += kBlock) acc[i] = 0.0;
+  __syncthreads();
+  for (int64_t base = (int64_t)blockIdx.x * kBlock; base < op.dim; base += (int64_t)gridDim.x * kBlock) {
+    const int64_t row = base + threadIdx.x;
+    const bool valid = row < op.dim;
+    const uint64_t b = valid ? op.rows[row] : 0ull;
+    double fr = 0.0, fi = 0.0, pr = 0.0, pi = 0.0;
+    if (valid) {
+      load_c(phi, row, fr, fi);
+      load_c(psi, row, pr, pi);
+    }
+    double sqrt_own = 1.0;
+    if constexpr (MODE != kGroupNone) sqrt_own = valid ? sqrt((double)op.orbit[row]) : 1.0;
+    // conj(phi_r) * psi_r for the diagonal terms
+    const double dr = fr * pr + fi * pi, di = fr * pi - fi * pr;
+    for (int t = 0; t < op.n_diag; ++t) {
+      const DiagTerm d = op.diag[t];
+      const bool m = valid && ((b & d.mask) == d.match);
+      double vr = wave_sum(m ? dr : 0.0);
+      double vi = wave_sum(m ? di : 0.0);
+      if (lane == 0) {
+        acc[(wave * n_terms + t) * 2] += vr;
+        acc[(wave * n_terms + t) * 2 + 1] += vi;
+      }
+    }
+    for (int t = 0; t < op.n_off; ++t) {
+      const OffTerm& ot = S.off[t];
+      double vr = 0.0, vi = 0.0;
+      if (valid && (b & ot.mask) == ot.match) {
+        const uint64_t img = (b & ~ot.mask) | ot.image;
+        uint64_t s;
+        int h, cnt;
+        Canon<MODE>::run(op.g, S.L, img, s, h, cnt);
+        const int64_t idx = index_of(op.ix, S.W, s);
+        if (idx >= 0) {
+          OffTerm unit = ot;
+          unit.are = 1.0;
+          unit.aim = 0.0;
+          double wr, wi;
+          element_weight<MODE, false>(op, S, unit, h, cnt, sqrt_own, wr, wi);
+          double sr, si;
+          load_c(psi, idx, sr, si);
+          const double qr = wr * sr - wi * si, qi = wr * si + wi * sr;  // w * psi_s
+          vr = fr * qr + fi * qi;                                       // conj(phi) * (w psi_s)
+          vi = fr * qi - fi * qr;
+        }
+      }
+      vr = wave_sum(vr);
+      vi = wave_sum(vi);
+      if (lane == 0) {
+        acc[(wave * n_terms + op.n_diag + t) * 2] += vr;
+        acc[(wave * n_terms + op.n_diag + t) * 2 + 1] += vi;
+      }
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < n_terms * 2; i += kBlock) {
+    double s = 0.0;
+    for (int w = 0; w < kBlock / 64; ++w) s += acc[w * n_terms * 2 + i];
+    partial[(int64_t)blockIdx.x * n_terms * 2 + i] = s;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_reduce_partials(const double* __restrict__ partial, int64_t n_blocks, int width,
+                                                            double* __restrict__ out) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= width) return;
+  double s = 0.0;
+  for (int64_t b = 0; b < n_blocks; ++b) s += partial[b * width + i];
+  out[i] = s;
+}
+
+template <typename V, int MODE>
+hipError_t launch_expect_t(const OpDev& op, const void* phi, const void* psi, double* d_partial, int64_t n_blocks,
+                           double* d_out, hipStream_t st) {
+  const int n_terms = op.n_diag + op.n_off;
+  uint32_t lds = op.lds_total + (uint32_t)((kBlock / 64) * n_terms * 2 * sizeof(double));
+  hipLaunchKernelGGL((k_term_expect<V, MODE>), dim3((unsigned)n_blocks), dim3(kBlock), lds, st, op, (const V*)phi,
+                     (const V*)psi, d_partial);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const int width = n_terms * 2;
+  hipLaunchKernelGGL(k_reduce_partials, dim3((unsigned)((width + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, d_partial,
+                     n_blocks, width, d_out);
+  return hipGetLastError();
+}
+
+template <typename V>
+hipError_t launch_expect_v(const OpDev& op, const void* phi, const void* psi, double* d_partial, int64_t n_blocks,
+                           double* d_out, hipStream_t st) {
+  switch (op.g.mode) {
+    case kGroupNone:
+      return launch_expect_t<V, kGroupNone>(op, phi, psi, d_partial, n_blocks, d_out, st);
+    case kGroupGeneric:
+      return launch_expect_t<V, kGroupGeneric>(op, phi, psi, d_partial, n_blocks, d_out, st);
+    default:
+      return launch_expect_t<V, kGroupTorus>(op, phi, psi, d_partial, n_blocks, d_out, st);
+  }
+}
+
+}  // namespace
+
+hipError_t launch_matvec(const OpDev& op, int prec, int transpose, int accumulate, const void* x, void* y,
+                         hipStream_t st) {
+  if (op.dim == 0) return hipSuccess;
+  switch (prec) {
+    case kF64:
+      return launch_mv_v<double>(op, transpose, accumulate, x, y, st);
+    case kC128:
+      return launch_mv_v<Cx<double>>(op, transpose, accumulate, x, y, st);
+    case kF32:
+      return launch_mv_v<float>(op, transpose, accumulate, x, y, st);
+    case kC64:
+      return launch_mv_v<Cx<float>>(op, transpose, accumulate, x, y, st);
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_iterate(const OpDev& op, const OffTerm* all_terms, int n_all, int transpose, int64_t n_rows,
+                          const int64_t* d_rows, int64_t* d_idx, double* d_re, double* d_im, hipStream_t st) {
+  if (n_rows == 0) return hipSuccess;
+  dim3 grid((unsigned)((n_rows + kBlock - 1) / kBlock));
+#define EDHIP_IT(M, T)                                                                                             \
+  hipLaunchKernelGGL((k_iterate<M, T>), grid, dim3(kBlock), op.lds_total, st, op, all_terms, n_all, n_rows, d_rows, \
+                     d_idx, d_re, d_im)
+  if (op.g.mode == kGroupNone) {
+    if (transpose) EDHIP_IT(kGroupNone, true); else EDHIP_IT(kGroupNone, false);
+  } else if (op.g.mode == kGroupGeneric) {
+    if (transpose) EDHIP_IT(kGroupGeneric, true); else EDHIP_IT(kGroupGeneric, false);
+  } else {
+    if (transpose) EDHIP_IT(kGroupTorus, true); else EDHIP_IT(kGroupTorus, false);
+  }
+#undef EDHIP_IT
+  return hipGetLastError();
+}
+
+hipError_t launch_sparse_columns(const OpDev& op, double tol, int slot, int64_t* d_scratch_idx, double* d_scratch_re,
+                                 double* d_scratch_im, int64_t* d_count, hipStream_t st) {
+  if (op.dim == 0) return hipSuccess;
+  dim3 grid((unsigned)((op.dim + kBlock - 1) / kBlock));
+  if (op.g.mode == kGroupNone)
+    hipLaunchKernelGGL((k_sparse_columns<kGroupNone>), grid, dim3(kBlock), op.lds_total, st, op, tol, slot,
+                       d_scratch_idx, d_scratch_re, d_scratch_im, d_count);
+  else if (op.g.mode == kGroupGeneric)
+    hipLaunchKernelGGL((k_sparse_columns<kGroupGeneric>), grid, dim3(kBlock), op.lds_total, st, op, tol, slot,
+                       d_scratch_idx, d_scratch_re, d_scratch_im, d_count);
+  else
+    hipLaunchKernelGGL((k_sparse_columns<kGroupTorus>), grid, dim3(kBlock), op.lds_total, st, op, tol, slot,
+                       d_scratch_idx, d_scratch_re, d_scratch_im, d_count);
+  return hipGetLastError();
+}
+
+hipError_t launch_sparse_compact(int64_t dim, int slot, const int64_t* d_colptr, const int64_t* d_scratch_idx,
+                                 const double* d_scratch_re, const double* d_scratch_im, int64_t* d_rowval,
+                                 double* d_re, double* d_im, hipStream_t st) {
+  if (dim == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_sparse_compact, dim3((unsigned)((dim + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, dim, slot,
+                     d_colptr, d_scratch_idx, d_scratch_re, d_scratch_im, d_rowval, d_re, d_im);
+  return hipGetLastError();
+}
+
+int64_t term_expect_blocks(const OpDev& op) {
+  int64_t nb = (op.dim + kBlock - 1) / kBlock;
+  if (nb > kExpectBlocksMax) nb = kExpectBlocksMax;
+  if (nb < 1) nb = 1;
+  return nb;
+}
+
+hipError_t launch_term_expect(const OpDev& op, int prec, const void* phi, const void* psi, double* d_partial,
+                              int64_t n_blocks, double* d_out, hipStream_t st) {
+  switch (prec) {
+    case kF64:
+      return launch_expect_v<double>(op, phi, psi, d_partial, n_blocks, d_out, st);
+    case kC128:
+      return launch_expect_v<Cx<double>>(op, phi, psi, d_partial, n_blocks, d_out, st);
+    case kF32:
+      return launch_expect_v<float>(op, phi, psi, d_partial, n_blocks, d_out, st);
+    case kC64:
+      return launch_expect_v<Cx<float>>(op, phi, psi, d_partial, n_blocks, d_out, st);
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+}  // namespace edhip

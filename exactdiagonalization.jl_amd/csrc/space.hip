@@ -1,0 +1,230 @@
+// Space kernels: sector enumeration, prefix table, symmetry reduction,
+// mapping export and batched lookup.
+#include <hipcub/hipcub.hpp>
+
+#include "kernels.hpp"
+
+namespace edhip {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+__global__ __launch_bounds__(kBlock) void k_enumerate_sector(uint64_t first, uint64_t count, int n_bits,
+                                                             const uint64_t* __restrict__ W, uint64_t* __restrict__ out) {
+  extern __shared__ unsigned char lds[];
+  uint64_t* Wl = (uint64_t*)lds;
+  const int nw = (n_bits + 1) * (n_bits + 1);
+  for (int i = threadIdx.x; i < nw; i += blockDim.x) Wl[i] = W[i];
+  __syncthreads();
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  out[i] = unrank_word(first + i, n_bits, Wl);
+}
+
+__global__ __launch_bounds__(kBlock) void k_build_prefix(const uint64_t* __restrict__ keys, int64_t dim, int shift,
+                                                         uint64_t n_buckets, uint32_t* __restrict__ prefix) {
+  uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j > n_buckets) return;
+  uint64_t key = j << shift;
+  int64_t lo = 0, hi = dim;
+  while (lo < hi) {
+    int64_t mid = lo + ((hi - lo) >> 1);
+    if (keys[mid] < key)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  prefix[j] = (uint32_t)lo;
+}
+
+// Walk the words of chunk [r0, r1) of the source in ascending order.
+template <typename F>
+__device__ __forceinline__ void for_chunk(const ReduceSource& src, const uint64_t* Wl, uint64_t r0, uint64_t r1, F&& f) {
+  if (r0 >= r1) return;
+  if (src.list) {
+    for (uint64_t r = r0; r < r1; ++r) f(src.list[r]);
+    return;
+  }
+  uint64_t w = unrank_word(r0, src.n_bits, Wl);
+  for (uint64_t r = r0; r < r1; ++r) {
+    if (r != r0) w = src.single_popcount ? next_same_popcount(w) : unrank_word(r, src.n_bits, Wl);
+    f(w);
+  }
+}
+
+__device__ __forceinline__ const uint64_t* stage_W(const ReduceSource& src, unsigned char* lds, uint32_t off) {
+  uint64_t* Wl = (uint64_t*)(lds + off);
+  if (!src.list) {
+    const int nw = (src.n_bits + 1) * (src.n_bits + 1);
+    for (int i = threadIdx.x; i < nw; i += blockDim.x) Wl[i] = src.W[i];
+  }
+  return Wl;
+}
+
+__global__ __launch_bounds__(kBlock) void k_reduce_count(ReduceSource src, uint64_t chunk, uint64_t n_threads, GroupDev g,
+                                                         GroupLdsLayout lay, uint32_t* __restrict__ counts) {
+  extern __shared__ unsigned char lds[];
+  GroupLds L = stage_group(g, lay, lds);
+  const uint64_t* Wl = stage_W(src, lds, lay.total);
+  __syncthreads();
+  uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n_threads) return;
+  uint64_t r0 = t * chunk;
+  uint64_t r1 = r0 + chunk < src.n_words ? r0 + chunk : src.n_words;
+  uint32_t c = 0;
+  for_chunk(src, Wl, r0, r1, [&](uint64_t w) {
+    int orbit;
+    if (is_representative(g, L, w, orbit)) ++c;
+  });
+  counts[t] = c;
+}
+
+__global__ __launch_bounds__(kBlock) void k_reduce_write(ReduceSource src, uint64_t chunk, uint64_t n_threads, GroupDev g,
+                                                         GroupLdsLayout lay, const uint64_t* __restrict__ offsets,
+                                                         uint64_t* __restrict__ reps, uint16_t* __restrict__ orbits) {
+  extern __shared__ unsigned char lds[];
+  GroupLds L = stage_group(g, lay, lds);
+  const uint64_t* Wl = stage_W(src, lds, lay.total);
+  __syncthreads();
+  uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n_threads) return;
+  uint64_t r0 = t * chunk;
+  uint64_t r1 = r0 + chunk < src.n_words ? r0 + chunk : src.n_words;
+  uint64_t pos = offsets[t];
+  for_chunk(src, Wl, r0, r1, [&](uint64_t w) {
+    int orbit;
+    if (is_representative(g, L, w, orbit)) {
+      reps[pos] = w;
+      orbits[pos] = (uint16_t)orbit;
+      ++pos;
+    }
+  });
+}
+
+__global__ __launch_bounds__(kBlock) void k_export_mapping(ReduceSource src, GroupDev g, GroupLdsLayout lay, IndexDev ix,
+                                                           const uint16_t* __restrict__ orbit,
+                                                           int64_t* __restrict__ idx_out, double* __restrict__ re_out,
+                                                           double* __restrict__ im_out) {
+  extern __shared__ unsigned char lds[];
+  GroupLds L = stage_group(g, lay, lds);
+  const uint64_t* Wl = stage_W(src, lds, lay.total);
+  __syncthreads();
+  uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= src.n_words) return;
+  uint64_t w = src.list ? src.list[p] : unrank_word(p, src.n_bits, Wl);
+  uint64_t s;
+  int h, cnt;
+  canonicalize(g, L, w, s, h, cnt);
+  int64_t r = search_sorted(ix.keys, ix.prefix, ix.n_buckets, ix.shift, s);
+  if (r < 0) {
+    idx_out[p] = -1;
+    re_out[p] = 0.0;
+    im_out[p] = 0.0;
+    return;
+  }
+  double scale = g.invsqrt[orbit[r]];
+  idx_out[p] = r;
+  re_out[p] = g.ph_re[h] * scale;
+  im_out[p] = g.ph_im[h] * scale;
+}
+
+__global__ __launch_bounds__(kBlock) void k_lookup(IndexDev ix, int64_t n, const uint64_t* __restrict__ keys,
+                                                   int64_t* __restrict__ out) {
+  extern __shared__ unsigned char lds[];
+  uint64_t* Wl = (uint64_t*)lds;
+  if (ix.rank_mode) {
+    const int nw = (ix.n_bits + 1) * (ix.n_bits + 1);
+    for (int i = threadIdx.x; i < nw; i += blockDim.x) Wl[i] = ix.W[i];
+  }
+  __syncthreads();
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  out[i] = index_of(ix, Wl, keys[i]);
+}
+
+__global__ void k_u32_to_u64(const uint32_t* __restrict__ in, uint64_t* __restrict__ out, uint64_t n) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = in[i];
+}
+
+inline unsigned grid_for(uint64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+inline uint32_t w_bytes(int n_bits) { return (uint32_t)((n_bits + 1) * (n_bits + 1) * sizeof(uint64_t)); }
+
+}  // namespace
+
+hipError_t launch_enumerate_sector(uint64_t first, uint64_t count, int n_bits, const uint64_t* d_W, uint64_t* d_out,
+                                   hipStream_t st) {
+  if (count == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_enumerate_sector, dim3(grid_for(count)), dim3(kBlock), w_bytes(n_bits), st, first, count, n_bits,
+                     d_W, d_out);
+  return hipGetLastError();
+}
+
+hipError_t launch_build_prefix(const uint64_t* d_keys, int64_t dim, int shift, uint64_t n_buckets, uint32_t* d_prefix,
+                               hipStream_t st) {
+  hipLaunchKernelGGL(k_build_prefix, dim3(grid_for(n_buckets + 1)), dim3(kBlock), 0, st, d_keys, dim, shift, n_buckets,
+                     d_prefix);
+  return hipGetLastError();
+}
+
+hipError_t launch_reduce_count(const ReduceSource& src, uint64_t chunk, uint64_t n_threads, const GroupDev& g,
+                               const GroupLdsLayout& lay, const uint64_t*, uint32_t* d_counts, hipStream_t st) {
+  if (n_threads == 0) return hipSuccess;
+  uint32_t lds = lay.total + (src.list ? 0u : w_bytes(src.n_bits));
+  hipLaunchKernelGGL(k_reduce_count, dim3(grid_for(n_threads)), dim3(kBlock), lds, st, src, chunk, n_threads, g, lay,
+                     d_counts);
+  return hipGetLastError();
+}
+
+hipError_t launch_reduce_write(const ReduceSource& src, uint64_t chunk, uint64_t n_threads, const GroupDev& g,
+                               const GroupLdsLayout& lay, const uint64_t* d_offsets, uint64_t* d_reps,
+                               uint16_t* d_orbit, hipStream_t st) {
+  if (n_threads == 0) return hipSuccess;
+  uint32_t lds = lay.total + (src.list ? 0u : w_bytes(src.n_bits));
+  hipLaunchKernelGGL(k_reduce_write, dim3(grid_for(n_threads)), dim3(kBlock), lds, st, src, chunk, n_threads, g, lay,
+                     d_offsets, d_reps, d_orbit);
+  return hipGetLastError();
+}
+
+hipError_t launch_export_mapping(const ReduceSource& src, const GroupDev& g, const GroupLdsLayout& lay,
+                                 const IndexDev& reduced_index, const uint16_t* d_orbit, int64_t* d_idx, double* d_re,
+                                 double* d_im, hipStream_t st) {
+  if (src.n_words == 0) return hipSuccess;
+  uint32_t lds = lay.total + (src.list ? 0u : w_bytes(src.n_bits));
+  hipLaunchKernelGGL(k_export_mapping, dim3(grid_for(src.n_words)), dim3(kBlock), lds, st, src, g, lay, reduced_index,
+                     d_orbit, d_idx, d_re, d_im);
+  return hipGetLastError();
+}
+
+hipError_t launch_lookup(const IndexDev& ix, int64_t n, const uint64_t* d_keys, int64_t* d_idx, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  uint32_t lds = ix.rank_mode ? w_bytes(ix.n_bits) : 0u;
+  hipLaunchKernelGGL(k_lookup, dim3(grid_for((uint64_t)n)), dim3(kBlock), lds, st, ix, n, d_keys, d_idx);
+  return hipGetLastError();
+}
+
+hipError_t exclusive_scan_u32_to_u64(const uint32_t* d_in, uint64_t* d_out, uint64_t n, void* d_tmp, size_t& tmp_bytes,
+                                     hipStream_t st) {
+  // d_out doubles as the widened input; the scan runs in place.
+  if (d_tmp == nullptr) {
+    return hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, d_out, d_out, (int64_t)n, st);
+  }
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_u32_to_u64, dim3(grid_for(n)), dim3(kBlock), 0, st, d_in, d_out, n);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  return hipcub::DeviceScan::ExclusiveSum(d_tmp, tmp_bytes, d_out, d_out, (int64_t)n, st);
+}
+
+hipError_t exclusive_scan_i64(const int64_t* d_in, int64_t* d_out, uint64_t n, void* d_tmp, size_t& tmp_bytes,
+                              hipStream_t st) {
+  if (d_tmp == nullptr) {
+    return hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, d_in, d_out, (int64_t)n, st);
+  }
+  if (n == 0) return hipSuccess;
+  return hipcub::DeviceScan::ExclusiveSum(d_tmp, tmp_bytes, d_in, d_out, (int64_t)n, st);
+}
+
+}  // namespace edhip

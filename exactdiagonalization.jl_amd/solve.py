@@ -1,0 +1,134 @@
+"""Eigensolver and linear solver on the device.
+
+The reference has no eigensolver of its own: its callers run
+``Arpack.eigs(opr; nev, which=:SR)`` through ``mul!``
+(docs/src/examples/spinhalf.md:26).  ``eigs`` here is the thick-restart
+Lanczos of libedhip (FP64, or FP32 Krylov storage with an FP64 refinement),
+with the tridiagonal/arrowhead eigenproblem solved on the host.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from ._lib import ArgumentError
+
+
+@dataclass
+class EigsResult:
+    values: np.ndarray
+    vectors: object  # torch tensor (nev, dim) on the GPU, or None
+    iterations: int
+    restarts: int
+    matvecs: int
+    matvecs_fp64: int
+    converged: bool
+    residuals: np.ndarray
+    unrefined: np.ndarray
+    t_total: float
+    t_matvec: float
+    t_refine: float
+
+
+def eigs(opr, nev: int = 1, which: str = "SR", *, mode: str = "fp64", tol: float = 1e-10, max_basis: int = 120,
+         max_restarts: int = 50, seed: int = 20241202, return_vectors: bool = False, refine_basis: int = 8,
+         max_refine: int = 10, check_every: int = 5) -> EigsResult:
+    """Lowest ``nev`` eigenvalues of a Hermitian operator representation.
+
+    mode="fp64": FP64 (ComplexF64) vectors.  mode="mixed": FP32 (ComplexF32)
+    Krylov storage and products, FP64 reductions, then FP64 refinement cycles
+    started from the FP32 Ritz vectors.  Start vector: SplitMix64(seed) +
+    Box-Muller Gaussian.
+    """
+    if which != "SR":
+        raise ArgumentError("only which='SR' (smallest real) is supported")
+    o = _lib.LanczosOpts()
+    _lib.lib().edhip_lanczos_default_opts(ctypes.byref(o))
+    o.nev = int(nev)
+    o.max_basis = int(max_basis)
+    o.max_restarts = int(max_restarts)
+    o.mode = _lib.LANCZOS_MIXED if mode == "mixed" else _lib.LANCZOS_FP64
+    o.want_vectors = 1 if return_vectors else 0
+    o.refine_basis = int(refine_basis)
+    o.max_refine = int(max_refine)
+    o.check_every = int(check_every)
+    o.seed = int(seed)
+    o.tol = float(tol)
+    st = _lib.LanczosStats()
+    vals = np.zeros(nev)
+    vecs = None
+    vptr = ctypes.c_void_p(0)
+    if return_vectors:
+        import torch
+
+        dt = torch.complex128 if opr.is_complex else torch.float64
+        vecs = torch.empty((nev, opr.dimension), dtype=dt, device=f"cuda:{opr.ctx.device}")
+        vptr = ctypes.c_void_p(vecs.data_ptr())
+        torch.cuda.synchronize()
+    opr.ctx.check(
+        _lib.lib().edhip_lanczos(opr._handle, ctypes.byref(o), _lib.ptr(vals, ctypes.c_double), vptr, ctypes.byref(st))
+    )
+    return EigsResult(
+        values=vals,
+        vectors=vecs,
+        iterations=st.iterations,
+        restarts=st.restarts,
+        matvecs=st.matvecs,
+        matvecs_fp64=st.matvecs_fp64,
+        converged=bool(st.converged),
+        residuals=np.array(st.residual[:nev]),
+        unrefined=np.array(st.unrefined[:nev]),
+        t_total=st.t_total,
+        t_matvec=st.t_matvec,
+        t_refine=st.t_refine,
+    )
+
+
+def minres(opr, b, *, shift: float = 0.0, deflate=None, tol: float = 1e-10, max_iter: int = 2000):
+    """Solve P (H - shift) P x = P b with MINRES, P = 1 - sum d d^H over the
+    rows of ``deflate`` (orthonormal).  b, deflate: GPU tensors (float64 or
+    complex128).  Returns (x, iterations, residual)."""
+    import torch
+
+    if not (torch.is_tensor(b) and b.is_cuda):
+        raise ArgumentError("b must be a GPU tensor")
+    prec = {torch.float64: _lib.F64, torch.complex128: _lib.C128}.get(b.dtype)
+    if prec is None:
+        raise ArgumentError("MINRES runs in float64 or complex128")
+    x = torch.empty_like(b)
+    nd = 0
+    dptr = ctypes.c_void_p(0)
+    if deflate is not None:
+        d = deflate.reshape(-1, b.numel()).contiguous()
+        if d.dtype != b.dtype:
+            raise ArgumentError("deflation vectors must match b's dtype")
+        nd = d.shape[0]
+        dptr = ctypes.c_void_p(d.data_ptr())
+    torch.cuda.synchronize()
+    st = _lib.MinresStats()
+    opr.ctx.check(
+        _lib.lib().edhip_minres(opr._handle, prec, float(shift), nd, dptr, ctypes.c_void_p(b.data_ptr()),
+                                ctypes.c_void_p(x.data_ptr()), float(tol), int(max_iter), ctypes.byref(st))
+    )
+    return x, st.iterations, st.residual
+
+
+def term_expectations(opr, phi, psi) -> np.ndarray:
+    """Unit-amplitude bilinear forms <phi|T_t|psi> of every term t of the
+    operator, in term order (one fused pass on the device)."""
+    import torch
+
+    prec = {torch.float64: _lib.F64, torch.complex128: _lib.C128, torch.float32: _lib.F32,
+            torch.complex64: _lib.C64}.get(psi.dtype)
+    if prec is None or phi.dtype != psi.dtype:
+        raise ArgumentError("phi and psi must share a supported dtype")
+    out = np.zeros(2 * max(opr._n_terms, 1))
+    torch.cuda.synchronize()
+    opr.ctx.check(
+        _lib.lib().edhip_term_expect(opr._handle, prec, ctypes.c_void_p(phi.data_ptr()), ctypes.c_void_p(psi.data_ptr()),
+                                     _lib.ptr(out, ctypes.c_double))
+    )
+    return out[0::2][: opr._n_terms] + 1j * out[1::2][: opr._n_terms]

@@ -1,0 +1,236 @@
+/*
+ * edhip: MI355X-native (gfx950, HIP) Hamiltonian application on Sz-sector
+ * and symmetry-reduced bases — the C ABI.
+ *
+ * This is the drop-in boundary for the ExactDiagonalization.jl (v0.14.3)
+ * path  HilbertSpaceSector -> represent -> symmetry_reduce -> represent(rhsr, H)
+ * -> mul!/apply!/sparse -> eigensolver.  Each entry point names the reference
+ * interface it replaces (file:line, relative to the reference root).
+ *
+ * Conventions
+ *  - Every call returns an int status (EDHIP_OK = 0); edhip_last_error(ctx)
+ *    gives the message of the last failure on that context.  No C++
+ *    exception crosses the ABI.
+ *  - Indices are 0-based and 64-bit; -1 marks "not present", as the
+ *    reference's -1 (its 1-based wrapper adds one).
+ *  - Words (basis states) are uint64 bit patterns: site i (0-based) occupies
+ *    bits [bitoffset_i, bitoffset_i + bitwidth_i), as in HilbertSpace
+ *    (src/HilbertSpace/hilbert_space.jl:25-41).
+ *  - The library owns device memory behind opaque handles.  Pointers passed
+ *    with a *_host suffix are host memory borrowed for the call; all others
+ *    are device pointers, ordered on the context's stream.
+ *  - A reduced space keeps a pointer to its parent: the parent must outlive it.
+ *    A handle is not re-entrant; calls on one context are serialised by the
+ *    caller.
+ */
+#ifndef EDHIP_H
+#define EDHIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes (mapped by the host wrappers onto the reference's exceptions) */
+#define EDHIP_OK 0
+#define EDHIP_ERR_ARGUMENT 1  /* ArgumentError */
+#define EDHIP_ERR_DIMENSION 2 /* DimensionMismatch */
+#define EDHIP_ERR_MEMORY 3    /* device allocation failed */
+#define EDHIP_ERR_DEVICE 4    /* HIP runtime error / no device */
+#define EDHIP_ERR_KEY 5       /* KeyError */
+#define EDHIP_ERR_BOUNDS 6    /* BoundsError */
+#define EDHIP_ERR_NOCONV 7    /* iterative solver hit its iteration limit */
+
+/* vector element types */
+#define EDHIP_F64 0  /* Float64 */
+#define EDHIP_C128 1 /* ComplexF64 */
+#define EDHIP_F32 2  /* Float32 */
+#define EDHIP_C64 3  /* ComplexF32 */
+
+typedef struct edhip_ctx edhip_ctx;
+typedef struct edhip_space edhip_space;
+typedef struct edhip_op edhip_op;
+
+/* ---- context ------------------------------------------------------------ */
+int edhip_create(int device, edhip_ctx** out);
+void edhip_destroy(edhip_ctx* ctx);
+const char* edhip_last_error(const edhip_ctx* ctx);
+/* Use an external hipStream_t (NULL restores the context's own stream). */
+int edhip_set_stream(edhip_ctx* ctx, void* hip_stream);
+int edhip_synchronize(edhip_ctx* ctx);
+/* Device memory of the context's device (plain hipMalloc / hipFree). */
+int edhip_malloc(edhip_ctx* ctx, uint64_t bytes, void** out);
+int edhip_free(edhip_ctx* ctx, void* ptr);
+int edhip_memcpy(edhip_ctx* ctx, void* dst, const void* src, uint64_t bytes, int kind /* hipMemcpyKind */);
+
+/* ---- spaces ------------------------------------------------------------- */
+/* represent(HilbertSpaceSector(hs, qn)) for sites of bit width 1 whose
+ * sector fixes the number of set bits:
+ *   src/Representation/hilbert_space_representation.jl:215-230 (represent,
+ *   represent_array) over hs_get_basis_list(hss, BR) at :127-206.
+ * popcounts: the allowed numbers of set bits (ascending or not).  The basis is
+ * the ascending list of n_bits-bit words with an allowed popcount; it is
+ * enumerated on the device by unranking and materialised on first use. */
+int edhip_space_sector(edhip_ctx* ctx, int n_bits, int n_popcounts, const int32_t* popcounts_host,
+                       edhip_space** out);
+
+/* represent(hs, basis_list) / represent_array(hs, basis_list):
+ *   hilbert_space_representation.jl:239-256.  The list must be sorted and
+ * unique (FrozenSortedArrayIndex validation, src/frozensortedarray.jl:14-22);
+ * EDHIP_ERR_ARGUMENT otherwise. */
+int edhip_space_from_list(edhip_ctx* ctx, int n_bits, int64_t n, const uint64_t* basis_host, edhip_space** out);
+
+/* symmetry_reduce(hsr, symops_and_amplitudes; tol):
+ *   src/Symmetry/symmetry_reduce_generic.jl:7-106 (and the IrrepComponent
+ *   route, symmetry_reduce_translation.jl:10-106, after conj of the
+ *   characters).
+ * bit_perm: n_group x n_bits, bit_perm[g*n_bits + i] = image bit of bit i
+ *   (a SitePermutation expanded to bits, symmetry_apply.jl:82-92); element 0
+ *   must be the identity and the set must be closed under composition.
+ * chi: the amplitudes y_g, |y_g| = 1 (EDHIP_ERR_ARGUMENT otherwise, :27-29).
+ * A word is kept iff it is the smallest of its orbit and every element fixing
+ * it has |y_g - 1| <= tol.  The kept words are ascending; their orbit sizes
+ * are recorded.  Translation groups of an Lx x Ly torus are detected and use
+ * the pruned canonicaliser; every other group uses LDS permutation tables. */
+int edhip_space_reduce(edhip_ctx* ctx, const edhip_space* parent, int n_group, const int32_t* bit_perm_host,
+                       const double* chi_re_host, const double* chi_im_host, double tol, edhip_space** out);
+
+void edhip_space_destroy(edhip_space* space);
+/* dimension(hsr) / dimension(rhsr): hilbert_space_representation.jl:90,
+ * reduced_hilbert_space_representation.jl:38 */
+int edhip_space_dim(const edhip_space* space, int64_t* dim);
+/* hsr.basis_list / rhsr.basis_list (host copy) */
+int edhip_space_copy_basis(const edhip_space* space, uint64_t* out_host);
+/* device pointer to the sorted basis (materialises a sector basis) */
+int edhip_space_device_basis(const edhip_space* space, const uint64_t** out);
+/* orbit size |G|/|Stab| of every representative (1 for an unreduced space) */
+int edhip_space_copy_orbit(const edhip_space* space, uint16_t* out_host);
+/* get(basis_lookup, key, -1) for many keys: src/frozensortedarray.jl:29-48 */
+int edhip_space_lookup(const edhip_space* space, int64_t n, const uint64_t* keys_host, int64_t* idx_host);
+/* basis_mapping_index / basis_mapping_amplitude of a reduced space over its
+ * parent (reduced_hilbert_space_representation.jl:13-22): for every parent
+ * word p, the index of its representative (or -1) and U[p] (0 if excluded).
+ * Arrays have the parent's dimension.  For parity checks; O(parent). */
+int edhip_space_export_mapping(const edhip_space* reduced, int64_t* idx_host, double* amp_re_host,
+                               double* amp_im_host);
+/* 1 if the space is symmetry reduced; group order and canonicaliser kind
+ * (0 none, 1 generic LDS tables, 2 torus translations) */
+int edhip_space_info(const edhip_space* space, int* reduced, int* group_order, int* canon_kind);
+
+/* ---- operators ---------------------------------------------------------- */
+/* represent(hsr, op) / represent(rhsr, op):
+ *   src/Representation/operator_representation.jl:34-36,
+ *   src/Symmetry/reduced_operator_representation.jl:40-42.
+ * The terms are those of a simplified SumOperator (operator_simplify.jl:32-81)
+ * as PureOperator(bitmask, bitrow, bitcol, amplitude) (pure_operator.jl:25-52);
+ * bitrow/bitcol outside bitmask give EDHIP_ERR_ARGUMENT (pure_operator.jl:32-36). */
+int edhip_op_create(edhip_ctx* ctx, const edhip_space* space, int64_t n_terms, const uint64_t* mask_host,
+                    const uint64_t* row_host, const uint64_t* col_host, const double* amp_re_host,
+                    const double* amp_im_host, edhip_op** out);
+void edhip_op_destroy(edhip_op* op);
+/* 1 if the operator or the space's characters need complex arithmetic */
+int edhip_op_is_complex(const edhip_op* op, int* is_complex);
+
+/* apply! / mul!:
+ *   transpose = 0: out += opr * state  (apply!(out, opr, state),
+ *                  abstract_operator_representation.jl:260-267,296-316,358-378)
+ *   transpose = 1: out += state * opr  (apply!(out, state, opr), :278-285,327-347)
+ *   accumulate = 1 adds into y (apply!), 0 overwrites it (mul!, :110-118).
+ * x, y: device vectors of n elements of type prec (EDHIP_F64 ...).  n must
+ * equal the dimension (EDHIP_ERR_DIMENSION otherwise).  Real precisions
+ * require a real operator and real characters (EDHIP_ERR_ARGUMENT). */
+int edhip_apply(edhip_op* op, int prec, int transpose, int accumulate, int64_t n_x, const void* x, int64_t n_y,
+                void* y);
+/* same on host vectors (copied to and from the device) */
+int edhip_apply_host(edhip_op* op, int prec, int transpose, int accumulate, int64_t n_x, const void* x_host,
+                     int64_t n_y, void* y_host);
+
+/* get_row_iterator / get_column_iterator for a list of rows (columns):
+ *   operator_representation.jl:66-103, reduced_operator_representation.jl:57-116.
+ * For entry k and term slot j (< n_terms, in term order) writes the element
+ * index (or -1 for an invalid image) and value; unused slots get index -2. */
+int edhip_iterate(edhip_op* op, int transpose, int64_t n_rows, const int64_t* rows_host, int64_t* idx_host,
+                  double* re_host, double* im_host);
+
+/* sparse(opr; tol) (abstract_operator_representation.jl:136-204, choptol!
+ * src/util.jl:88-94): builds the CSC matrix on the device (per column:
+ * merge duplicates, drop |v| < tol, rows ascending) and returns nnz;
+ * edhip_sparse_export copies it (0-based colptr[dim+1], rowval[nnz]). */
+int edhip_sparse_build(edhip_op* op, double tol, int64_t* nnz);
+int edhip_sparse_export(edhip_op* op, int64_t* colptr_host, int64_t* rowval_host, double* re_host,
+                        double* im_host);
+
+/* ---- eigensolver (the reference's caller uses Arpack.eigs(opr; nev,
+ * which=:SR), docs/src/examples/spinhalf.md:26, through mul!) ------------- */
+#define EDHIP_LANCZOS_FP64 0  /* FP64 (or ComplexF64) vectors throughout */
+#define EDHIP_LANCZOS_MIXED 1 /* FP32 (ComplexF32) Krylov storage + FP64 refinement */
+
+typedef struct {
+  int nev;           /* lowest eigenvalues wanted */
+  int max_basis;     /* Krylov vectors kept before a thick restart */
+  int max_restarts;
+  int mode;          /* EDHIP_LANCZOS_FP64 or EDHIP_LANCZOS_MIXED */
+  int want_vectors;  /* 1: write the eigenvectors (nev x dim, FP64/C128) */
+  int refine_basis;  /* mixed mode: Krylov size of one FP64 refinement cycle */
+  int max_refine;    /* mixed mode: refinement cycles per eigenvalue */
+  int check_every;   /* Ritz-value convergence check period (iterations) */
+  uint64_t seed;     /* start vector: SplitMix64 + Box-Muller Gaussian */
+  double tol;        /* residual ||H y - theta y|| <= tol * |theta| */
+} edhip_lanczos_opts;
+
+typedef struct {
+  int iterations;     /* Lanczos steps (storage precision) */
+  int restarts;
+  int matvecs;        /* matvecs in storage precision */
+  int matvecs_fp64;   /* FP64 matvecs of the refinement */
+  int converged;
+  int pad_;
+  double residual[16];     /* final ||H y - theta y|| / max(1,|theta|) (FP64) */
+  double unrefined[16];    /* mixed mode: Ritz values before refinement */
+  double t_total;          /* seconds */
+  double t_matvec;         /* seconds in storage-precision matvecs */
+  double t_refine;         /* seconds in the FP64 refinement */
+} edhip_lanczos_stats;
+
+void edhip_lanczos_default_opts(edhip_lanczos_opts* opts);
+/* evals_host: nev doubles, ascending.  evecs: device pointer to nev*dim
+ * FP64 (real op) or C128 (complex op) elements, or NULL. */
+int edhip_lanczos(edhip_op* op, const edhip_lanczos_opts* opts, double* evals_host, void* evecs,
+                  edhip_lanczos_stats* stats);
+
+/* ---- differentiation (no counterpart in the reference; used by the
+ * rrule of the ground-state energy and of a correlator) ------------------ */
+/* Per-term bilinear forms with unit amplitude in one fused basis pass:
+ *   g_t = sum_r conj(phi_r) sum_{images b' of term t from b_r} w_t * psi_s
+ * (w_t the representation weight without the term amplitude).  out_host:
+ * 2*n_terms doubles (re, im), in the order of the terms given to
+ * edhip_op_create.  phi, psi: device vectors of type prec. */
+int edhip_term_expect(edhip_op* op, int prec, const void* phi, const void* psi, double* out_host);
+
+typedef struct {
+  int iterations;
+  int converged;
+  double residual; /* final ||b - A x|| / ||b|| estimate */
+} edhip_minres_stats;
+
+/* MINRES (Paige & Saunders) on A = P (H - shift) P, P = 1 - sum_i d_i d_i^H
+ * over n_deflate orthonormal device vectors d (may be 0):
+ * solves A x = P b.  FP64/C128 vectors (prec EDHIP_F64 or EDHIP_C128). */
+int edhip_minres(edhip_op* op, int prec, double shift, int n_deflate, const void* deflate, const void* b, void* x,
+                 double tol, int max_iter, edhip_minres_stats* stats);
+
+/* ---- vector helpers on the device (FP64-accumulated, deterministic) ----- */
+int edhip_vec_dot(edhip_ctx* ctx, int prec, int64_t n, const void* x, const void* y, double* out_re_im_host);
+int edhip_vec_gaussian(edhip_ctx* ctx, int prec, int64_t n, uint64_t seed, void* x);
+int edhip_vec_convert(edhip_ctx* ctx, int prec_in, int prec_out, int64_t n, const void* x, void* y);
+
+/* Timing on the context's stream (hipEvent pairs). */
+int edhip_event_record(edhip_ctx* ctx, int slot);           /* slot 0..15 */
+int edhip_event_elapsed(edhip_ctx* ctx, int slot0, int slot1, double* ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* EDHIP_H */
