@@ -75,6 +75,8 @@ class LanczosOpts(ctypes.Structure):
         ("refine_basis", ctypes.c_int),
         ("max_refine", ctypes.c_int),
         ("check_every", ctypes.c_int),
+        ("criterion", ctypes.c_int),
+        ("pad_", ctypes.c_int),
         ("seed", ctypes.c_uint64),
         ("tol", ctypes.c_double),
     ]
@@ -129,6 +131,7 @@ SIGNATURES = {
     "edhip_op_is_complex": (_i32, [_vp, _P(_i32)]),
     "edhip_apply": (_i32, [_vp, _i32, _i32, _i32, _i64, _vp, _i64, _vp]),
     "edhip_apply_host": (_i32, [_vp, _i32, _i32, _i32, _i64, _vp, _i64, _vp]),
+    "edhip_op_count_images": (_i32, [_vp, _P(_i64)]),
     "edhip_iterate": (_i32, [_vp, _i32, _i64, _P(_i64), _P(_i64), _P(_dbl), _P(_dbl)]),
     "edhip_sparse_build": (_i32, [_vp, _dbl, _P(_i64)]),
     "edhip_sparse_export": (_i32, [_vp, _P(_i64), _P(_i64), _P(_dbl), _P(_dbl)]),
@@ -152,6 +155,13 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise ImportError(f"{LIB_PATH} is missing: build it with __graft_entry__.build()")
+        # One HIP runtime per process: when PyTorch is present, load its
+        # libamdhip64 first so that libedhip binds to the same runtime (both
+        # carry the soname libamdhip64.so.7) and device pointers are shared.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             f = getattr(L, name)

@@ -757,20 +757,31 @@ struct TRLanczos {
 
   void scale(void* x, double s) { HIP_OR_THROW(vec_axpby(prec, n, s, 0.0, x, 0.0, 0.0, x, op->ctx->stream)); }
 
+  // Convergence rule of a Ritz pair (theta, residual r):
+  //   criterion 0: r <= tol * max(1, |theta|)
+  //   criterion 1: as 0, or (r^2 / gap <= tol * max(1, |theta|) and theta moved
+  //                by at most tol * max(1, |theta|) since the previous check),
+  //                gap = distance to the next Ritz value (or gap_hint): the
+  //                Hermitian bound |theta - lambda| <= r^2 / gap on the value.
+  int criterion = 0;
+  double gap_hint = 0.0;
+
   // Runs the iteration from the unit vector in vec(0).  Returns the lowest
-  // nev Ritz values; Ritz vectors (if out != nullptr) are written in
-  // precision out_prec with stride n.  res: residual norms.
+  // n_ret >= nev Ritz values (convergence is required of the lowest nev);
+  // Ritz vectors of the lowest nev (if out != nullptr) are written in
+  // precision out_prec with stride n.  res: residual norms of the lowest nev.
   bool run(int nev, double tol, int max_restarts, int check_every, std::vector<double>& theta, std::vector<double>& res,
-           void* out, int out_prec) {
+           void* out, int out_prec, int n_ret = 0) {
     edhip_ctx* ctx = op->ctx;
     const int64_t dim = n;
     int mm = (int)std::min<int64_t>(m, dim - n_locked);
     if (mm < 1) mm = 1;
     nev = std::min(nev, mm);
+    n_ret = std::max(nev, std::min(n_ret, mm));
     std::vector<double> T((size_t)mm * mm, 0.0);
     int k = 0;  // kept Ritz vectors after a restart
     bool converged = false;
-    std::vector<double> w, Z;
+    std::vector<double> w, Z, prev;
     int j = 0;
     double beta_last = 0.0;
     int size = 0;  // current basis size used for the final Ritz step
@@ -801,8 +812,17 @@ struct TRLanczos {
           res.assign(nev, 0.0);
           for (int i = 0; i < nev && i <= j; ++i) {
             res[i] = std::fabs(beta * Z[(size_t)j * (j + 1) + i]);
-            if (res[i] > tol * std::max(1.0, std::fabs(w[i]))) ok = false;
+            const double scale = tol * std::max(1.0, std::fabs(w[i]));
+            bool ok_i = res[i] <= scale;
+            if (!ok_i && criterion == 1 && prev.size() > (size_t)i) {
+              double gap = (i + 1 <= j) ? (w[i + 1] - w[i]) : gap_hint;
+              if (gap_hint > 0.0 && (i + 1 > j || gap > gap_hint)) gap = gap_hint;
+              gap = std::max(gap, 1e-12);
+              ok_i = (res[i] * res[i] / gap <= scale) && (std::fabs(w[i] - prev[i]) <= scale);
+            }
+            if (!ok_i) ok = false;
           }
+          prev.assign(w.begin(), w.begin() + std::min<size_t>(w.size(), (size_t)nev));
           if (ok || tiny) {
             converged = ok || tiny;
             break;
@@ -821,7 +841,7 @@ struct TRLanczos {
       for (int a = 0; a < mm; ++a)
         for (int b = 0; b < mm; ++b) Tm[(size_t)a * mm + b] = T[(size_t)a * mm + b];
       sym_eig(mm, Tm, !arrow, w, Z);
-      int keep = std::min(mm - 1, std::max(nev + 2, std::min(mm / 3, nev + 16)));
+      int keep = std::min(mm - 1, std::max(n_ret + 2, std::min(mm / 3, n_ret + 16)));
       // new basis: V'_i = sum_j Z[j][i] V_j (i < keep), V'_keep = normalised residual
       std::vector<double> coef((size_t)mm * keep);
       for (int a = 0; a < mm; ++a)
@@ -845,7 +865,7 @@ struct TRLanczos {
       ctx->sync();
     }
     // Ritz values / vectors from the current (size x size) block
-    theta.assign(w.begin(), w.begin() + std::min<size_t>(nev, w.size()));
+    theta.assign(w.begin(), w.begin() + std::min<size_t>(n_ret, w.size()));
     if (out) {
       int sz = size;
       std::vector<double> coef((size_t)sz * nev);
@@ -1251,6 +1271,20 @@ int edhip_apply_host(edhip_op* op, int prec, int transpose, int accumulate, int6
   });
 }
 
+int edhip_op_count_images(edhip_op* op, int64_t* n_images) {
+  if (!op || !n_images) return EDHIP_ERR_ARGUMENT;
+  return guarded(op->ctx, [&] {
+    DevBuf t;
+    t.alloc(8);
+    HIP_OR_THROW(hipMemsetAsync(t.p, 0, 8, op->ctx->stream));
+    HIP_OR_THROW(launch_count_images(op->dev_row, t.as<unsigned long long>(), op->ctx->stream));
+    unsigned long long h = 0;
+    HIP_OR_THROW(hipMemcpyAsync(&h, t.p, 8, hipMemcpyDeviceToHost, op->ctx->stream));
+    op->ctx->sync();
+    *n_images = (int64_t)h;
+  });
+}
+
 int edhip_iterate(edhip_op* op, int transpose, int64_t n_rows, const int64_t* rows, int64_t* idx, double* re,
                   double* im) {
   if (!op) return EDHIP_ERR_ARGUMENT;
@@ -1339,9 +1373,10 @@ void edhip_lanczos_default_opts(edhip_lanczos_opts* o) {
   o->max_restarts = 50;
   o->mode = EDHIP_LANCZOS_FP64;
   o->want_vectors = 0;
-  o->refine_basis = 8;
-  o->max_refine = 10;
+  o->refine_basis = 12;
+  o->max_refine = 20;
   o->check_every = 5;
+  o->criterion = 0;
   o->seed = 20241202ull;
   o->tol = 1e-10;
 }
@@ -1375,8 +1410,12 @@ int edhip_lanczos(edhip_op* op, const edhip_lanczos_opts* opts, double* evals, v
     // Ritz vectors of the main run, FP64 (or C128)
     DevBuf ritz;
     ritz.alloc((size_t)nev * n * elem_size(prec64));
-    bool conv = L.run(nev, opts->mode == EDHIP_LANCZOS_MIXED ? std::max(opts->tol, 1e-6) : opts->tol,
-                      opts->max_restarts, opts->check_every, theta, res, ritz.p, prec64);
+    const bool mixed = opts->mode == EDHIP_LANCZOS_MIXED;
+    // The FP32 phase stops on the eigenvalue criterion (its residual cannot go
+    // below single-precision rounding); the FP64 phase uses the caller's rule.
+    L.criterion = mixed ? 1 : opts->criterion;
+    const double tol_main = mixed ? std::max(opts->tol, 1e-8) : opts->tol;
+    bool conv = L.run(nev, tol_main, opts->max_restarts, opts->check_every, theta, res, ritz.p, prec64, nev + 1);
     st.iterations = L.iterations;
     st.restarts = L.restarts;
     st.matvecs = L.matvecs;
@@ -1385,18 +1424,21 @@ int edhip_lanczos(edhip_op* op, const edhip_lanczos_opts* opts, double* evals, v
     L.V.release();
     L.tmp.release();
 
-    if (opts->mode == EDHIP_LANCZOS_MIXED) {
+    if (mixed) {
       Timer tr;
       conv = true;
       // FP64 refinement with locking: eigenpair i is refined by short FP64
       // thick-restart cycles started from its mixed-precision Ritz vector,
       // orthogonal to the already refined ones.
+      const std::vector<double> theta32 = theta;
       for (int i = 0; i < nev; ++i) {
         TRLanczos R;
         R.op = op;
         R.prec = prec64;
         R.n = n;
         R.m = (int)std::min<int64_t>(std::max(opts->refine_basis, 2), n - i);
+        R.criterion = opts->criterion;
+        R.gap_hint = (theta32.size() > (size_t)i + 1) ? (theta32[i + 1] - theta32[i]) : 0.0;
         R.V.alloc((size_t)(R.m + 1) * n * elem_size(prec64));
         R.locked = ritz.p;
         R.n_locked = i;
@@ -1407,7 +1449,7 @@ int edhip_lanczos(edhip_op* op, const edhip_lanczos_opts* opts, double* evals, v
         double nr = std::sqrt(R.dot(R.vec(0), R.vec(0)).real());
         R.scale(R.vec(0), 1.0 / nr);
         std::vector<double> th1, rs1;
-        bool ok = R.run(1, opts->tol, std::max(opts->max_refine - 1, 0), std::max(1, R.m), th1, rs1, start, prec64);
+        bool ok = R.run(1, opts->tol, std::max(opts->max_refine - 1, 0), 2, th1, rs1, start, prec64, 2);
         theta[i] = th1[0];
         res[i] = rs1.empty() ? 0.0 : rs1[0];
         st.matvecs_fp64 += R.matvecs;

@@ -81,6 +81,9 @@ enum Prec : int { kF64 = 0, kC128 = 1, kF32 = 2, kC64 = 3 };
 hipError_t launch_matvec(const OpDev& op, int prec, int transpose, int accumulate, const void* x, void* y,
                          hipStream_t st);
 
+// Total number of off-diagonal images the row form generates (atomic u64 add).
+hipError_t launch_count_images(const OpDev& op, unsigned long long* d_total, hipStream_t st);
+
 // Raw iterator elements (index or -1, value) in term order for the given rows.
 hipError_t launch_iterate(const OpDev& op, const OffTerm* all_terms, int n_all, int transpose, int64_t n_rows,
                           const int64_t* d_rows, int64_t* d_idx, double* d_re, double* d_im, hipStream_t st);

@@ -419,7 +419,30 @@ hipError_t launch_expect_v(const OpDev& op, const void* phi, const void* psi, do
   }
 }
 
+// Number of off-diagonal images generated over all rows (matched terms).
+__global__ __launch_bounds__(kBlock) void k_count_images(OpDev op, unsigned long long* __restrict__ total) {
+  extern __shared__ unsigned char lds[];
+  const OffTerm* off = (const OffTerm*)(lds + op.off_lds);
+  lds_copy(lds + op.off_lds, op.off, (uint32_t)(op.n_off * sizeof(OffTerm)));
+  __syncthreads();
+  const int64_t row = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  unsigned long long c = 0;
+  if (row < op.dim) {
+    const uint64_t b = op.rows[row];
+    for (int t = 0; t < op.n_off; ++t) c += ((b & off[t].mask) == off[t].match) ? 1ull : 0ull;
+  }
+  for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+  if ((threadIdx.x & 63) == 0 && c) atomicAdd(total, c);
+}
+
 }  // namespace
+
+hipError_t launch_count_images(const OpDev& op, unsigned long long* d_total, hipStream_t st) {
+  if (op.dim == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_count_images, dim3((unsigned)((op.dim + kBlock - 1) / kBlock)), dim3(kBlock), op.lds_total, st, op,
+                     d_total);
+  return hipGetLastError();
+}
 
 hipError_t launch_matvec(const OpDev& op, int prec, int transpose, int accumulate, const void* x, void* y,
                          hipStream_t st) {

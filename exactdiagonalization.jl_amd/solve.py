@@ -34,14 +34,18 @@ class EigsResult:
 
 
 def eigs(opr, nev: int = 1, which: str = "SR", *, mode: str = "fp64", tol: float = 1e-10, max_basis: int = 120,
-         max_restarts: int = 50, seed: int = 20241202, return_vectors: bool = False, refine_basis: int = 8,
-         max_refine: int = 10, check_every: int = 5) -> EigsResult:
+         max_restarts: int = 50, seed: int = 20241202, return_vectors: bool = False, refine_basis: int = 12,
+         max_refine: int = 20, check_every: int = 5, criterion: str = "residual") -> EigsResult:
     """Lowest ``nev`` eigenvalues of a Hermitian operator representation.
 
     mode="fp64": FP64 (ComplexF64) vectors.  mode="mixed": FP32 (ComplexF32)
     Krylov storage and products, FP64 reductions, then FP64 refinement cycles
     started from the FP32 Ritz vectors.  Start vector: SplitMix64(seed) +
     Box-Muller Gaussian.
+
+    criterion="residual": converged when ||H y - theta y|| <= tol*max(1,|theta|).
+    criterion="eigenvalue": also when r^2/gap <= tol*max(1,|theta|) with theta
+    stationary between checks (a bound on the eigenvalue error).
     """
     if which != "SR":
         raise ArgumentError("only which='SR' (smallest real) is supported")
@@ -55,6 +59,7 @@ def eigs(opr, nev: int = 1, which: str = "SR", *, mode: str = "fp64", tol: float
     o.refine_basis = int(refine_basis)
     o.max_refine = int(max_refine)
     o.check_every = int(check_every)
+    o.criterion = 1 if criterion == "eigenvalue" else 0
     o.seed = int(seed)
     o.tol = float(tol)
     st = _lib.LanczosStats()

@@ -146,6 +146,11 @@ int edhip_apply(edhip_op* op, int prec, int transpose, int accumulate, int64_t n
 int edhip_apply_host(edhip_op* op, int prec, int transpose, int accumulate, int64_t n_x, const void* x_host,
                      int64_t n_y, void* y_host);
 
+/* Number of off-diagonal images the row iterators generate over all rows
+ * (terms whose bitrow matches), i.e. the off-diagonal elements one apply!
+ * visits; used for the element-throughput metric. */
+int edhip_op_count_images(edhip_op* op, int64_t* n_images);
+
 /* get_row_iterator / get_column_iterator for a list of rows (columns):
  *   operator_representation.jl:66-103, reduced_operator_representation.jl:57-116.
  * For entry k and term slot j (< n_terms, in term order) writes the element
@@ -175,8 +180,12 @@ typedef struct {
   int refine_basis;  /* mixed mode: Krylov size of one FP64 refinement cycle */
   int max_refine;    /* mixed mode: refinement cycles per eigenvalue */
   int check_every;   /* Ritz-value convergence check period (iterations) */
+  int criterion;     /* 0: residual ||H y - theta y|| <= tol*max(1,|theta|);
+                        1: also accept r^2/gap <= tol*max(1,|theta|) with theta
+                        stationary between checks (eigenvalue accuracy) */
+  int pad_;
   uint64_t seed;     /* start vector: SplitMix64 + Box-Muller Gaussian */
-  double tol;        /* residual ||H y - theta y|| <= tol * |theta| */
+  double tol;        /* see criterion */
 } edhip_lanczos_opts;
 
 typedef struct {

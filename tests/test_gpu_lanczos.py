@@ -1,0 +1,87 @@
+"""Lanczos eigensolver on the device (GPU).
+
+FP64 mode: eigenvalues within 1e-12 relative of the ARPACK (scipy eigsh)
+spectrum of the oracle's sparse matrix.  Mixed mode (FP32 Krylov storage +
+FP64 refinement): within 1e-10 relative of the FP64 path."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _heis(ed, hs, bonds, w=None):
+    return ed.simplify(sum((ed.heisenberg_bond(hs, i, j, 1.0 if w is None else w[k]) for k, (i, j) in enumerate(bonds)), ed.NullOperator()))
+
+
+def test_chain16_unreduced_known(ed, golden):
+    L = 16
+    hs, _ = ed.spin_half_system(L)
+    H = _heis(ed, hs, [(i, (i + 1) % L) for i in range(L)])
+    opr = ed.represent(ed.represent(ed.HilbertSpaceSector(hs, 0)), H)
+    ref = np.array(golden["eigenvalues_brute_force"]["chain16"])
+    r = ed.eigs(opr, 4, tol=1e-12, max_basis=200)
+    assert r.converged
+    assert np.all(np.abs(r.values - ref) < 1e-11 * np.abs(ref))
+    assert abs(r.values[0] - (-7.14229636)) < 1e-8
+    rm = ed.eigs(opr, 4, tol=1e-11, mode="mixed", max_basis=200)
+    assert np.all(np.abs(rm.values - ref) < 1e-10 * np.abs(ref))
+    assert rm.matvecs_fp64 > 0
+
+
+def test_thick_restart(ed, golden):
+    """A basis of 24 vectors forces thick restarts; same spectrum."""
+    L = 16
+    hs, _ = ed.spin_half_system(L)
+    H = _heis(ed, hs, [(i, (i + 1) % L) for i in range(L)])
+    opr = ed.represent(ed.represent(ed.HilbertSpaceSector(hs, 0)), H)
+    ref = np.array(golden["eigenvalues_brute_force"]["chain16"])[:2]
+    r = ed.eigs(opr, 2, tol=1e-11, max_basis=24, max_restarts=400)
+    assert r.restarts > 0 and r.converged
+    assert np.all(np.abs(r.values - ref) < 1e-10 * np.abs(ref))
+
+
+def test_square4x4_over_k(ed, golden):
+    hs, _ = ed.spin_half_system(16)
+    j1, _ = ed.square_bonds(4, 4)
+    H = _heis(ed, hs, j1)
+    hsr = ed.represent(ed.HilbertSpaceSector(hs, 0))
+    e = min(ed.eigs(ed.represent(ed.symmetry_reduce(hsr, ed.TranslationIrrep((4, 4), (kx, ky))), H), 1, tol=1e-12).values[0]
+            for kx in range(4) for ky in range(4))
+    ref = golden["eigenvalues_brute_force"]["square4x4"][0]
+    assert abs(e - ref) < 1e-11 * abs(ref)
+
+
+def test_eigenvectors(ed):
+    L = 14
+    hs, _ = ed.spin_half_system(L)
+    H = _heis(ed, hs, [(i, (i + 1) % L) for i in range(L)])
+    opr = ed.represent(ed.symmetry_reduce(ed.represent(ed.HilbertSpaceSector(hs, 0)), ed.TranslationIrrep(L, 3)), H)
+    r = ed.eigs(opr, 2, tol=1e-12, return_vectors=True)
+    V = r.vectors.cpu().numpy()
+    for i in range(2):
+        v = V[i]
+        assert abs(np.linalg.norm(v) - 1) < 1e-10
+        assert np.linalg.norm(opr @ v - r.values[i] * v) < 1e-9
+    assert abs(np.vdot(V[0], V[1])) < 1e-9
+
+
+@pytest.mark.parametrize("name,shape,k,nev", [("chain24_k3", (24, 1), (3, 0), 4), ("torus4x6_pi0", (4, 6), (2, 0), 1)])
+def test_full_size_configs_vs_arpack(ed, oracle, name, shape, k, nev):
+    """The 24-site configurations: FP64 Lanczos vs ARPACK on the oracle's
+    sparse matrix; mixed precision vs FP64 at 1e-10 relative."""
+    n = shape[0] * shape[1]
+    hs, _ = ed.spin_half_system(n)
+    if shape[1] == 1:
+        H = _heis(ed, hs, [(i, (i + 1) % n) for i in range(n)])
+    else:
+        j1, j2 = ed.square_bonds(*shape)
+        H = _heis(ed, hs, j1 + j2, [1.0] * len(j1) + [0.5] * len(j2))
+    irrep = ed.TranslationIrrep(shape, k)
+    opr = ed.represent(ed.symmetry_reduce(ed.represent(ed.HilbertSpaceSector(hs, 0)), irrep), H)
+    perms = np.stack([ed.bit_permutation(hs, p) for p, _ in irrep.symops_and_amplitudes()])
+    red = oracle.symmetry_reduce(oracle.sector_basis(n, 0), n, perms, [a for _, a in irrep.symops_and_amplitudes()])
+    ref = oracle.lowest_eigenvalues(oracle.Opr(red, ed.terms_arrays(H)), nev)
+    r64 = ed.eigs(opr, nev, tol=1e-12, max_basis=200)
+    assert np.all(np.abs(r64.values - ref) < 1e-12 * np.abs(ref))
+    rmx = ed.eigs(opr, nev, tol=1e-11, mode="mixed", max_basis=200, criterion="eigenvalue")
+    assert np.all(np.abs(rmx.values - r64.values) < 1e-10 * np.abs(r64.values))

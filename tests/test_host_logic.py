@@ -1,0 +1,179 @@
+"""Host-side logic of the product (no GPU): the operator term format and its
+algebra, Hilbert-space sectors, symmetry groups, and the C-ABI library."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, dense_from_terms
+
+
+def _dense_of(ed, n, op):
+    words = np.arange(2 ** n, dtype=np.uint64)
+    return dense_from_terms(n, words, ed.terms_arrays(op))
+
+
+def _kron_site(n, i, m):
+    out = np.array([[1.0 + 0j]])
+    for j in reversed(range(n)):
+        out = np.kron(out, m if j == i else np.eye(2))
+    return out
+
+
+SX = np.array([[0, 1], [1, 0]], complex)
+SY = np.array([[0, -1j], [1j, 0]], complex)
+SZ = np.array([[1, 0], [0, -1]], complex)
+SP = np.array([[0, 1], [0, 0]], complex)
+SM = np.array([[0, 0], [1, 0]], complex)
+
+
+@pytest.mark.parametrize("j,m", [("x", SX), ("y", SY), ("z", SZ), ("+", SP), ("-", SM)])
+def test_pauli_matrix_vs_kron(ed, j, m):
+    """pauli_matrix (spin_half.jl:28-42) with Up = state 0 = bit 0 clear."""
+    hs, _ = ed.spin_half_system(3)
+    for i in range(3):
+        assert np.allclose(_dense_of(ed, 3, ed.pauli_matrix(hs, i, j)), _kron_site(3, i, m))
+
+
+def test_products_and_sums_vs_kron(ed):
+    hs, s = ed.spin_half_system(3)
+    op = s(0, "x") * s(1, "y") + 2.5 * s(2, "z") - s(0, "+") * s(0, "-")
+    ref = _kron_site(3, 0, SX) @ _kron_site(3, 1, SY) + 2.5 * _kron_site(3, 2, SZ) - _kron_site(3, 0, SP @ SM)
+    assert np.allclose(_dense_of(ed, 3, op), ref)
+    assert np.allclose(_dense_of(ed, 3, ed.simplify(op)), ref)
+
+
+def test_pure_operator_validation(ed):
+    with pytest.raises(ed.ArgumentError):
+        ed.PureOperator(0b01, 0b10, 0b00, 1.0)
+    with pytest.raises(ed.ArgumentError):
+        ed.PureOperator(0b01, 0b00, 0b10, 1.0)
+    # product rule: pure_operator.jl:136-154
+    a = ed.PureOperator(0b011, 0b001, 0b010, 2.0)
+    b = ed.PureOperator(0b110, 0b010, 0b100, 3.0)
+    assert a * b == ed.PureOperator(0b111, 0b001, 0b100, 6.0)
+    c = ed.PureOperator(0b010, 0b000, 0b000, 1.0)
+    assert isinstance(a * c, ed.NullOperator)
+
+
+def test_simplify_heisenberg_known(ed):
+    """test_operator_simplify.jl:68-85: sum of sigma.sigma simplifies to the
+    term set of 2 s+s- + 2 s-s+ + szsz."""
+    n = 4
+    hs, s = ed.spin_half_system(n)
+    h1 = sum((2 * s(i, "+") * s((i + 1) % n, "-") for i in range(n)), ed.NullOperator())
+    h1 = h1 + sum((2 * s(i, "-") * s((i + 1) % n, "+") for i in range(n)), ed.NullOperator())
+    h1 = h1 + sum((s(i, "z") * s((i + 1) % n, "z") for i in range(n)), ed.NullOperator())
+    h2 = sum((s(i, j) * s((i + 1) % n, j) for i in range(n) for j in "xyz"), ed.NullOperator())
+    h3, h4 = ed.simplify(h1), ed.simplify(h2)
+    assert len(h1.terms) < len(h2.terms)
+    assert {(t.bitmask, t.bitrow, t.bitcol, complex(t.amplitude)) for t in h3.terms} == {
+        (t.bitmask, t.bitrow, t.bitcol, complex(t.amplitude)) for t in h4.terms
+    }
+    assert all(not isinstance(t.amplitude, complex) for t in h4.terms)  # demoted to real
+    assert len(h4.terms) == 6 * n  # the |00><11| parts cancel
+    keys = [(t.bitmask, t.bitrow, t.bitcol) for t in h4.terms]
+    assert keys == sorted(keys)
+    assert isinstance(ed.simplify(h3 - h3), ed.NullOperator)
+
+
+def test_heisenberg_bond_units(ed):
+    hs, _ = ed.spin_half_system(2)
+    bond = ed.heisenberg_bond(hs, 0, 1, J=1.0, Delta=0.5)
+    ref = 0.25 * (_kron_site(2, 0, SX) @ _kron_site(2, 1, SX) + _kron_site(2, 0, SY) @ _kron_site(2, 1, SY))
+    ref += 0.25 * 0.5 * _kron_site(2, 0, SZ) @ _kron_site(2, 1, SZ)
+    assert np.allclose(_dense_of(ed, 2, bond), ref)
+
+
+def test_sectors_and_popcounts(ed):
+    from edhip.hilbert import popcount_sets
+
+    hs, _ = ed.spin_half_system(4)
+    assert popcount_sets(ed.HilbertSpaceSector(hs, 0)) == [2]
+    assert popcount_sets(ed.HilbertSpaceSector(hs, [4, -2])) == [0, 3]
+    assert popcount_sets(ed.HilbertSpaceSector(hs, 1)) == []  # unreachable
+    assert popcount_sets(hs) == [0, 1, 2, 3, 4]
+    n7, _ = ed.spin_half_system(7)
+    assert popcount_sets(ed.HilbertSpaceSector(n7, 5)) == [1]
+    assert popcount_sets(ed.HilbertSpaceSector(n7, -5)) == [6]
+    up, dn, em = ed.State("Up", (1, 1)), ed.State("Dn", (1, -1)), ed.State("Em", (0, 0))
+    tj = ed.HilbertSpace([ed.Site([em, up, dn]), ed.Site([up, dn]), ed.Site([up, dn])])
+    assert popcount_sets(tj) is None  # general sites: explicit basis lists
+    assert tj.bitwidths == [2, 1, 1] and tj.bitoffsets == [0, 2, 3, 4]
+
+
+def test_symmetry_apply_host_known(ed, golden):
+    ka = golden["known_answers"]["symmetry_apply"]
+    hs, s = ed.spin_half_system(4)
+    T = ed.SitePermutation(ka["transop"])
+    I = ed.SitePermutation(ka["invop"])
+    for b, img in ka["bits"]:
+        assert ed.symmetry_apply(hs, T, b) == (img, 1)
+    p1 = ed.PureOperator(*ka["pop1"])
+    assert ed.symmetry_apply(hs, T, p1) == ed.PureOperator(*ka["pop1_trans"])
+    assert ed.symmetry_apply(hs, I, p1) == ed.PureOperator(*ka["pop1_inv"])
+    j1 = ed.simplify(sum((s(i, j) * s((i + 1) % 4, j) for i in range(4) for j in "xyz"), ed.NullOperator()))
+    assert ed.isinvariant(hs, T, j1) and ed.isinvariant(hs, I, j1)
+    assert not ed.isinvariant(hs, T, p1)
+
+
+def test_translation_group_and_bit_permutation(ed):
+    g = ed.translation_group((4, 3))
+    assert len(g) == 12 and g[0].map == list(range(1, 13))
+    hs, _ = ed.spin_half_system(12)
+    bp = ed.bit_permutation(hs, ed.translation_perm((4, 3), 1, 0))
+    # (x, y) -> (x+1, y): site 3 (x=3, y=0) wraps to site 0; site 4 -> 5
+    assert bp[3] == 0 and bp[4] == 5
+    irrep = ed.TranslationIrrep(7, 1)
+    ops = irrep.symops_and_amplitudes()
+    assert abs(ops[1][1] - np.exp(-2j * np.pi / 7)) < 1e-15
+    # 2-bit sites move as blocks
+    s3 = ed.Site([ed.State("a", 0), ed.State("b", 0), ed.State("c", 0)])
+    hs3 = ed.HilbertSpace([s3, s3])
+    assert ed.bit_permutation(hs3, ed.SitePermutation([2, 1])).tolist() == [2, 3, 0, 1]
+
+
+# ---------------------------------------------------------------- C ABI
+def _header_functions():
+    with open(os.path.join(ROOT, "include", "edhip.h")) as f:
+        txt = f.read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(edhip_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_abi_exports_every_declared_symbol(ed):
+    from edhip import _lib
+
+    L = _lib.lib()
+    declared = _header_functions()
+    assert len(declared) >= 30
+    for name in declared:
+        assert hasattr(L, name), name
+    # the ctypes signature table covers exactly the header
+    assert sorted(_lib.SIGNATURES) == declared
+
+
+def test_abi_without_device_reports_device_error(ed):
+    try:
+        import torch
+
+        if torch.cuda.is_available():
+            pytest.skip("a GPU is present")
+    except ImportError:
+        pass
+    from edhip import _lib
+
+    h = ctypes.c_void_p()
+    assert _lib.lib().edhip_create(0, ctypes.byref(h)) == _lib.EDHIP_ERR_DEVICE
+    with pytest.raises(_lib.DeviceError):
+        _lib.Context(0)
+
+
+def test_lanczos_default_opts(ed):
+    from edhip import _lib
+
+    o = _lib.LanczosOpts()
+    _lib.lib().edhip_lanczos_default_opts(ctypes.byref(o))
+    assert o.nev == 1 and o.max_basis == 120 and o.tol == 1e-10 and o.seed == 20241202
