@@ -135,9 +135,11 @@ def cpu_baseline(ed):
 
 
 def lanczos_times(ed, opr):
+    """Time to E0 at 1e-10 relative (eigenvalue criterion r^2/gap) for the
+    FP32-storage mixed Lanczos (with FP64 refinement) and the FP64 Lanczos."""
     out = {}
     for mode in ("mixed", "fp64"):
-        r = ed.eigs(opr, 1, mode=mode, tol=1e-10, max_basis=100, max_restarts=20)
+        r = ed.eigs(opr, 1, mode=mode, tol=1e-10, max_basis=100, max_restarts=20, criterion="eigenvalue")
         out[mode] = {
             "E0": float(r.values[0]),
             "seconds": r.t_total,
@@ -160,7 +162,19 @@ def main():
     ap.add_argument("--config", default="6x6", choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--e0", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run one plain 1 GiB device copy (elementwise kernel) for HBM-counter calibration and exit")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+
+        x = torch.ones(256 * 1024 * 1024, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        torch.cuda.synchronize()
+        torch.mul(x, 1.0, out=y)  # reads 1 GiB, writes 1 GiB
+        torch.cuda.synchronize()
+        print(json.dumps({"calibration_copy_bytes_read": x.numel() * 4, "bytes_written": y.numel() * 4}))
+        return
     if args.gpus != 1:
         print(
             f"bench.py runs on one GPU: this build has no multi-GPU path (asked for {args.gpus}); "

@@ -173,7 +173,7 @@ struct GroupHost {
   std::vector<int32_t> inv;
   bool complex_phase = false;
   // device tables
-  DevBuf nib, minrot, rot_hi, rot_lo, g_of_ab, inv_d, trivial, ph_re, ph_im, invsqrt;
+  DevBuf nib, minrot, rot_hi, rot_lo, g_of_ab, inv_d, trivial, ph_re, ph_im, invsqrt, sqcnt;
   GroupDev dev{};
   GroupLdsLayout lay{};
 };
@@ -200,6 +200,30 @@ struct edhip_space {
   const edhip_space* parent = nullptr;
   std::unique_ptr<GroupHost> group;
   DevBuf orbit;
+  // open-addressing table of the basis words with an embedded vector slot
+  mutable DevBuf hash_slots, hash_slot_of;
+  mutable HashDev hash_dev{};
+
+  // Builds (or widens) the table for vector elements of elem_bytes bytes.
+  const HashDev& ensure_hash(size_t elem_bytes) const {
+    const int want = elem_bytes > 8 ? 32 : 16;
+    if (hash_dev.slots && hash_dev.slot_bytes >= want) return hash_dev;
+    materialize();
+    uint64_t n_slots = 16;
+    while (n_slots < (uint64_t)dim * 2) n_slots <<= 1;
+    if (n_slots > (1ull << 32)) fail(EDHIP_ERR_ARGUMENT, "basis too large for the hash lookup");
+    hash_slots.alloc((size_t)n_slots * want);
+    hash_slot_of.alloc((size_t)std::max<int64_t>(dim, 1) * 4);
+    HIP_OR_THROW(hipMemsetAsync(hash_slots.p, 0xFF, (size_t)n_slots * want, ctx->stream));
+    HashDev h{};
+    h.slots = hash_slots.as<unsigned char>();
+    h.mask = (uint32_t)(n_slots - 1);
+    h.slot_bytes = want;
+    h.xoff = want == 32 ? 16 : 8;
+    HIP_OR_THROW(launch_hash_build(keys.as<uint64_t>(), dim, h, hash_slot_of.as<uint32_t>(), ctx->stream));
+    hash_dev = h;
+    return hash_dev;
+  }
 
   void materialize() const {
     if (materialized) return;
@@ -236,8 +260,44 @@ struct edhip_space {
     index_built = true;
   }
 
-  // Index of canonical words in this space.
-  IndexDev index_dev() const {
+  // Bitmap rank index over [0, 2^bm_bits): built when the largest word has
+  // at most 36 bits and the bitmap is at most 16x the size of the key array.
+  mutable DevBuf bitmap, bm_counts;
+  mutable int bm_bits = 0;
+  mutable bool bm_tried = false;
+
+  bool ensure_bitmap() const {
+    if (bm_tried) return bm_bits > 0;
+    bm_tried = true;
+    build_index();
+    if (dim == 0) return false;
+    uint64_t max_key = 0;
+    HIP_OR_THROW(hipMemcpyAsync(&max_key, keys.as<uint64_t>() + (dim - 1), 8, hipMemcpyDeviceToHost, ctx->stream));
+    ctx->sync();
+    const int K = std::max(9, bit_length(max_key));
+    if (K > 36) return false;
+    const uint64_t bytes = (1ull << K) / 8;
+    if (bytes > std::max<uint64_t>(16ull * (uint64_t)dim * 8ull, 1ull << 20)) return false;
+    const uint64_t n_blocks = (1ull << K) / 512;
+    bitmap.alloc(bytes);
+    bm_counts.alloc(n_blocks * 4);
+    HIP_OR_THROW(hipMemsetAsync(bitmap.p, 0, bytes, ctx->stream));
+    size_t tb = 0;
+    HIP_OR_THROW(launch_bitmap_build(keys.as<uint64_t>(), dim, bitmap.as<uint64_t>(), n_blocks, bm_counts.as<uint32_t>(),
+                                     nullptr, tb, ctx->stream));
+    DevBuf tmp;
+    tmp.alloc(tb);
+    HIP_OR_THROW(launch_bitmap_build(keys.as<uint64_t>(), dim, bitmap.as<uint64_t>(), n_blocks, bm_counts.as<uint32_t>(),
+                                     tmp.p, tb, ctx->stream));
+    ctx->sync();
+    bm_bits = K;
+    return true;
+  }
+
+  // Index of canonical words in this space.  lookup: 0 auto (bitmap when
+  // available, else sorted search), 1 sorted search, 3 bitmap.  A full
+  // popcount sector always uses rank arithmetic.
+  IndexDev index_dev(int lookup = 0) const {
     IndexDev ix{};
     ix.dim = dim;
     ix.n_bits = n_bits;
@@ -254,6 +314,15 @@ struct edhip_space {
     ix.prefix = prefix.as<uint32_t>();
     ix.n_buckets = n_buckets;
     ix.shift = shift;
+    if (lookup == 0 || lookup == 3) {
+      if (ensure_bitmap()) {
+        ix.bitmap = bitmap.as<uint64_t>();
+        ix.bm_counts = bm_counts.as<uint32_t>();
+        ix.bm_bits = bm_bits;
+      } else if (lookup == 3) {
+        fail(EDHIP_ERR_ARGUMENT, "no bitmap index for this space (words wider than 36 bits or too sparse)");
+      }
+    }
     return ix;
   }
 
@@ -358,7 +427,7 @@ void build_group(GroupHost& gh, int nb, double tol, hipStream_t st) {
   if (G == 1) gh.mode = kGroupGeneric;
 
   // phases, triviality, 1/sqrt(N)
-  std::vector<double> pr(G), pi(G), isq(G + 1, 0.0);
+  std::vector<double> pr(G), pi(G), isq(G + 1, 0.0), sqc(G + 1, 0.0);
   std::vector<uint8_t> triv(align_up((uint32_t)G, 4), 0);
   gh.complex_phase = false;
   for (int h = 0; h < G; ++h) {
@@ -369,6 +438,7 @@ void build_group(GroupHost& gh, int nb, double tol, hipStream_t st) {
     triv[h] = (std::abs(gh.chi[h] - 1.0) <= tol) ? 1 : 0;
   }
   for (int n = 1; n <= G; ++n) isq[n] = 1.0 / std::sqrt((double)n);
+  for (int c = 1; c <= G; ++c) sqc[c] = (G % c == 0) ? isq[G / c] : 0.0;
 
   GroupDev& d = gh.dev;
   d = GroupDev{};
@@ -387,11 +457,13 @@ void build_group(GroupHost& gh, int nb, double tol, hipStream_t st) {
   upload(gh.ph_re, pr, st);
   upload(gh.ph_im, pi, st);
   upload(gh.invsqrt, isq, st);
+  upload(gh.sqcnt, sqc, st);
   upload(gh.inv_d, gh.inv, st);
   upload(gh.trivial, triv, st);
   d.ph_re = gh.ph_re.as<double>();
   d.ph_im = gh.ph_im.as<double>();
   d.invsqrt = gh.invsqrt.as<double>();
+  d.sqcnt = gh.sqcnt.as<double>();
   d.inv = gh.inv_d.as<int32_t>();
   d.trivial = gh.trivial.as<uint8_t>();
   L.bytes_ph = (uint32_t)(G * 8);
@@ -401,6 +473,7 @@ void build_group(GroupHost& gh, int nb, double tol, hipStream_t st) {
   place(L.off_ph_re, L.bytes_ph);
   place(L.off_ph_im, L.bytes_ph);
   place(L.off_invsqrt, L.bytes_invsqrt);
+  place(L.off_sqcnt, L.bytes_invsqrt);
   place(L.off_inv, L.bytes_inv);
   place(L.off_trivial, L.bytes_trivial);
 
@@ -504,6 +577,13 @@ struct edhip_op {
   DevBuf sp_colptr, sp_rowval, sp_re, sp_im;
   // expectation scratch
   DevBuf ex_partial, ex_out;
+  // fast path: bond-pair groups (row and column forms)
+  bool pairs_ok = false;
+  DevBuf pg_meta[2], pg_dre[2], pg_dim[2], pg_ore[2], pg_oim[2];
+  PairGroupsDev pg[2]{};
+  int strat_kernel = 0;  // 0 auto, 1 term list, 2 pair groups
+  int strat_lookup = 0;  // 0 auto, 1 sorted search (or rank), 2 hash table
+  int used_kernel = 0, used_lookup = 0;
 
   bool is_complex() const { return complex_amp || (space->group && space->group->complex_phase); }
 };
@@ -552,6 +632,145 @@ void check_prec_for_op(const edhip_op* op, int prec) {
   if (elem_size(prec) == 0) fail(EDHIP_ERR_ARGUMENT, "unknown precision code");
   if (!prec_is_complex(prec) && op->is_complex())
     fail(EDHIP_ERR_ARGUMENT, "complex operator or characters need a complex vector type");
+}
+
+}  // namespace
+
+namespace {
+
+// Groups the terms by bit mask when every mask has one or two bits and each
+// (mask, pattern) has at most one off-diagonal entry.  form 0: row form
+// (match bitrow, image bitcol); form 1: column form (roles swapped).
+bool build_pair_groups(edhip_op* op, int form, hipStream_t st) {
+  std::vector<uint64_t> masks;
+  std::map<uint64_t, int> gid;
+  double cre = 0.0, cim = 0.0;
+  for (int64_t t = 0; t < op->n_terms; ++t) {
+    const int pc = __builtin_popcountll(op->mask[t]);
+    if (pc == 0) continue;
+    if (pc > 2) return false;
+    if (!gid.count(op->mask[t])) {
+      gid[op->mask[t]] = (int)masks.size();
+      masks.push_back(op->mask[t]);
+    }
+  }
+  const int ng = (int)masks.size();
+  if (ng > kMaxPairGroups) return false;
+  std::vector<uint32_t> meta(ng, 0);
+  std::vector<double> dre(4 * (size_t)ng, 0.0), dim(4 * (size_t)ng, 0.0), ore(4 * (size_t)ng, 0.0),
+      oim(4 * (size_t)ng, 0.0);
+  std::vector<int> pq(2 * (size_t)ng);
+  for (int g = 0; g < ng; ++g) {
+    const int p = __builtin_ctzll(masks[g]);
+    const int q = 63 - __builtin_clzll(masks[g]);
+    pq[2 * g] = p;
+    pq[2 * g + 1] = q;
+    meta[g] = (uint32_t)p | ((uint32_t)q << 8);
+  }
+  auto code_of = [&](int g, uint64_t bits) -> uint32_t {
+    const int p = pq[2 * g], q = pq[2 * g + 1];
+    return (uint32_t)((bits >> p) & 1ull) | ((uint32_t)((bits >> q) & 1ull) << 1);
+  };
+  for (int64_t t = 0; t < op->n_terms; ++t) {
+    const uint64_t m = op->mask[t];
+    const uint64_t match = form == 0 ? op->row[t] : op->col[t];
+    const uint64_t image = form == 0 ? op->col[t] : op->row[t];
+    if (m == 0) {
+      cre += op->are[t];
+      cim += op->aim[t];
+      continue;
+    }
+    const int g = gid[m];
+    const uint32_t c = code_of(g, match), nc = code_of(g, image);
+    if (match == image) {
+      dre[4 * (size_t)g + c] += op->are[t];
+      dim[4 * (size_t)g + c] += op->aim[t];
+    } else {
+      if ((meta[g] >> (16 + c)) & 1u) return false;  // two off-diagonal entries on one pattern
+      meta[g] |= 1u << (16 + c);
+      meta[g] |= nc << (24 + 2 * c);
+      ore[4 * (size_t)g + c] = op->are[t];
+      oim[4 * (size_t)g + c] = op->aim[t];
+    }
+  }
+  upload(op->pg_meta[form], meta, st);
+  upload(op->pg_dre[form], dre, st);
+  upload(op->pg_dim[form], dim, st);
+  upload(op->pg_ore[form], ore, st);
+  upload(op->pg_oim[form], oim, st);
+  PairGroupsDev& d = op->pg[form];
+  d.n_groups = ng;
+  d.meta = op->pg_meta[form].as<uint32_t>();
+  d.dre = op->pg_dre[form].as<double>();
+  d.dim = op->pg_dim[form].as<double>();
+  d.ore = op->pg_ore[form].as<double>();
+  d.oim = op->pg_oim[form].as<double>();
+  d.dconst_re = cre;
+  d.dconst_im = cim;
+  return true;
+}
+
+// Whether the pair-group kernel covers this operator and space.
+bool fast_path_usable(const edhip_op* op) {
+  if (!op->pairs_ok) return false;
+  const GroupHost* gh = op->space->group.get();
+  if (!gh) return true;
+  return gh->mode == kGroupTorus && gh->dev.minrot_table && gh->Ly <= 8;
+}
+
+FastOpDev make_fast(edhip_op* op, int transpose, const HashDev* hash) {
+  FastOpDev F{};
+  F.base = transpose ? op->dev_col : op->dev_row;
+  F.pg = op->pg[transpose ? 1 : 0];
+  if (hash) F.hash = *hash;
+  uint32_t off = align_up(F.base.lay.total, 16);
+  F.pg_lds = off;
+  off += (uint32_t)(kMaxPairGroups * 4) + (uint32_t)(F.pg.n_groups * 32 * 4);
+  off = align_up(off, 16);
+  F.base.w_lds = off;
+  off += F.base.w_bytes;
+  F.lds_total = align_up(off, 16);
+  if (F.lds_total > 64 * 1024) fail(EDHIP_ERR_ARGUMENT, "pair tables exceed the 64 KiB LDS budget");
+  return F;
+}
+
+// y = (accumulate ? y : 0) + A x (or x A), choosing the kernel and lookup.
+void op_matvec(edhip_op* op, int prec, int transpose, int accumulate, const void* x, void* y) {
+  hipStream_t st = op->ctx->stream;
+  const edhip_space* sp = op->space;
+  // auto: the term-list kernel (measured faster on the 6x6 J1-J2 sector: its
+  // lanes walk the terms in lock-step, which keeps their lookups in the same
+  // cache lines); the pair-group kernel on request
+  bool fast = fast_path_usable(op) && op->strat_kernel == 2;
+  if (op->strat_kernel == 2 && !fast_path_usable(op))
+    fail(EDHIP_ERR_ARGUMENT, "the pair-group kernel does not cover this operator or symmetry group");
+  const bool rank_mode = sp->is_sector && !sp->group;
+  const int idx_lookup = (op->strat_lookup == 2) ? 0 : op->strat_lookup;  // the hash applies to the pair kernel only
+  if (!fast) {
+    OpDev d = transpose ? op->dev_col : op->dev_row;
+    d.ix = sp->index_dev(idx_lookup);
+    op->used_kernel = 1;
+    op->used_lookup = rank_mode ? 1 : (d.ix.bm_bits ? 3 : 1);
+    HIP_OR_THROW(launch_matvec(d, prec, transpose ? 1 : 0, accumulate ? 1 : 0, x, y, st));
+    return;
+  }
+  bool use_hash = false;
+  if (!rank_mode) {
+    use_hash = op->strat_lookup == 2;
+  }
+  op->used_kernel = 2;
+  if (use_hash) {
+    op->used_lookup = 2;
+    const HashDev& h = sp->ensure_hash(elem_size(prec));
+    HIP_OR_THROW(launch_hash_scatter(prec, sp->dim, x, sp->hash_slot_of.as<uint32_t>(), h, st));
+    FastOpDev F = make_fast(op, transpose, &h);
+    HIP_OR_THROW(launch_matvec_pairs(F, prec, transpose, accumulate, x, y, st));
+  } else {
+    FastOpDev F = make_fast(op, transpose, nullptr);
+    F.base.ix = sp->index_dev(idx_lookup);
+    op->used_lookup = rank_mode ? 1 : (F.base.ix.bm_bits ? 3 : 1);
+    HIP_OR_THROW(launch_matvec_pairs(F, prec, transpose, accumulate, x, y, st));
+  }
 }
 
 }  // namespace
@@ -715,7 +934,7 @@ struct TRLanczos {
   void matvec(const void* x, void* y) {
     edhip_ctx* ctx = op->ctx;
     Timer t;
-    HIP_OR_THROW(launch_matvec(op->dev_row, prec, 0, 0, x, y, ctx->stream));
+    op_matvec(op, prec, 0, 0, x, y);
     ++matvecs;
     if (n > (1 << 22)) {  // time only the large products (avoids a sync per small step)
       ctx->sync();
@@ -1214,6 +1433,7 @@ int edhip_op_create(edhip_ctx* ctx, const edhip_space* space, int64_t n_terms, c
     upload(op->all_col, allc, ctx->stream);
     op->dev_row = make_opdev(op.get(), false);
     op->dev_col = make_opdev(op.get(), true);
+    op->pairs_ok = build_pair_groups(op.get(), 0, ctx->stream) && build_pair_groups(op.get(), 1, ctx->stream);
     ctx->sync();
   });
   if (rc == EDHIP_OK) *out = op.release();
@@ -1242,8 +1462,7 @@ int edhip_apply(edhip_op* op, int prec, int transpose, int accumulate, int64_t n
     if (n_y != d) fail(EDHIP_ERR_DIMENSION, "out has length " + std::to_string(n_y) + " != dimension " + std::to_string(d));
     if (n_x != d)
       fail(EDHIP_ERR_DIMENSION, "state has length " + std::to_string(n_x) + " != dimension " + std::to_string(d));
-    HIP_OR_THROW(launch_matvec(transpose ? op->dev_col : op->dev_row, prec, transpose ? 1 : 0, accumulate ? 1 : 0, x, y,
-                               op->ctx->stream));
+    op_matvec(op, prec, transpose ? 1 : 0, accumulate ? 1 : 0, x, y);
   });
 }
 
@@ -1263,12 +1482,29 @@ int edhip_apply_host(edhip_op* op, int prec, int transpose, int accumulate, int6
     if (d) {
       HIP_OR_THROW(hipMemcpyAsync(op->hx.p, x, bytes, hipMemcpyHostToDevice, st));
       if (accumulate) HIP_OR_THROW(hipMemcpyAsync(op->hy.p, y, bytes, hipMemcpyHostToDevice, st));
-      HIP_OR_THROW(launch_matvec(transpose ? op->dev_col : op->dev_row, prec, transpose ? 1 : 0, accumulate ? 1 : 0,
-                                 op->hx.p, op->hy.p, st));
+      op_matvec(op, prec, transpose ? 1 : 0, accumulate ? 1 : 0, op->hx.p, op->hy.p);
       HIP_OR_THROW(hipMemcpyAsync(y, op->hy.p, bytes, hipMemcpyDeviceToHost, st));
     }
     op->ctx->sync();
   });
+}
+
+int edhip_op_set_strategy(edhip_op* op, int kernel, int lookup) {
+  if (!op || kernel < 0 || kernel > 2 || lookup < 0 || lookup > 3) return EDHIP_ERR_ARGUMENT;
+  return guarded(op->ctx, [&] {
+    if (kernel == 2 && !fast_path_usable(op))
+      fail(EDHIP_ERR_ARGUMENT, "the pair-group kernel does not cover this operator or symmetry group");
+    if (lookup == 3) (void)op->space->index_dev(3);  // raises when no bitmap index exists
+    op->strat_kernel = kernel;
+    op->strat_lookup = lookup;
+  });
+}
+
+int edhip_op_last_strategy(const edhip_op* op, int* kernel, int* lookup) {
+  if (!op || !kernel || !lookup) return EDHIP_ERR_ARGUMENT;
+  *kernel = op->used_kernel;
+  *lookup = op->used_lookup;
+  return EDHIP_OK;
 }
 
 int edhip_op_count_images(edhip_op* op, int64_t* n_images) {
@@ -1524,7 +1760,7 @@ int edhip_minres(edhip_op* op, int prec, double shift, int n_deflate, const void
       if (n_deflate) H.orthogonalize(deflate, n_deflate, z);
     };
     auto apply_A = [&](const void* in, void* out) {
-      HIP_OR_THROW(launch_matvec(op->dev_row, prec, 0, 0, in, out, s));
+      op_matvec(op, prec, 0, 0, in, out);
       if (shift != 0.0) HIP_OR_THROW(vec_axpby(prec, n, -shift, 0.0, in, 1.0, 0.0, out, s));
       project(out);
     };

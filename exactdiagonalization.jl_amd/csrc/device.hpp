@@ -130,7 +130,34 @@ struct IndexDev {
   int pad_;
   uint64_t pop_allowed;  // bit p set iff popcount p is allowed (rank mode)
   const uint64_t* W;     // completion counts (rank mode)
+  // bitmap rank index (bm_bits > 0): bit s of the bitmap is set iff word s is
+  // in the basis; bm_counts[j] = number of basis words below 512*j
+  const uint64_t* bitmap;
+  const uint32_t* bm_counts;
+  int bm_bits;
+  int pad2_;
 };
+
+// Index of key through the bitmap: one 64-byte block of the bitmap plus one
+// block count (membership and rank together).
+__device__ __forceinline__ int64_t bitmap_index(const IndexDev& ix, uint64_t key) {
+  if ((key >> ix.bm_bits) != 0) return -1;
+  const uint64_t* blk = ix.bitmap + ((key >> 9) << 3);
+  const int w = (int)((key >> 6) & 7u);
+  uint64_t words[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) words[i] = blk[i];
+  uint64_t word = 0;
+  int64_t r = ix.bm_counts[key >> 9];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    r += (i < w) ? __builtin_popcountll(words[i]) : 0;
+    word = (i == w) ? words[i] : word;
+  }
+  const int bit = (int)(key & 63u);
+  if (!((word >> bit) & 1ull)) return -1;
+  return r + __builtin_popcountll(word & ((1ull << bit) - 1ull));
+}
 
 __device__ __forceinline__ int64_t search_sorted(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ prefix,
                                                  uint64_t n_buckets, int shift, uint64_t key) {
@@ -148,6 +175,7 @@ __device__ __forceinline__ int64_t search_sorted(const uint64_t* __restrict__ ke
 }
 
 __device__ __forceinline__ int64_t index_of(const IndexDev& ix, const uint64_t* W_lds, uint64_t key) {
+  if (ix.bm_bits) return bitmap_index(ix, key);
   if (ix.rank_mode) {
     if ((key >> ix.n_bits) != 0) return -1;
     int pc = __builtin_popcountll(key);
@@ -183,6 +211,7 @@ struct GroupDev {
   const double* ph_re;     // G: phase[h] = conj(chi(inv(h)))
   const double* ph_im;
   const double* invsqrt;   // G+1: 1/sqrt(N)
+  const double* sqcnt;     // G+1: sqrt(cnt/G) = 1/sqrt(G/cnt) for a stabiliser of cnt elements
 };
 
 // Pointers to the LDS copies of the group tables (same meaning as GroupDev).
@@ -190,6 +219,7 @@ struct GroupLds {
   const double* ph_re;
   const double* ph_im;
   const double* invsqrt;
+  const double* sqcnt;
   const uint64_t* nib;
   const uint32_t* minrot;
   const uint64_t* rot_hi;
@@ -357,7 +387,7 @@ __device__ __forceinline__ bool is_representative(const GroupDev& g, const Group
 // ----------------------------------------------------------------------------
 struct GroupLdsLayout {
   uint32_t off_nib, off_minrot, off_rot_hi, off_rot_lo, off_g_of_ab, off_inv, off_trivial;
-  uint32_t off_ph_re, off_ph_im, off_invsqrt;
+  uint32_t off_ph_re, off_ph_im, off_invsqrt, off_sqcnt;
   uint32_t bytes_nib, bytes_minrot, bytes_rot, bytes_g_of_ab, bytes_inv, bytes_trivial;
   uint32_t bytes_ph, bytes_invsqrt;
   uint32_t total;
@@ -383,6 +413,8 @@ __device__ __forceinline__ GroupLds stage_group(const GroupDev& g, const GroupLd
   lds_copy(lds + lay.off_ph_re, g.ph_re, lay.bytes_ph);
   lds_copy(lds + lay.off_ph_im, g.ph_im, lay.bytes_ph);
   lds_copy(lds + lay.off_invsqrt, g.invsqrt, lay.bytes_invsqrt);
+  lds_copy(lds + lay.off_sqcnt, g.sqcnt, lay.bytes_invsqrt);
+  L.sqcnt = (const double*)(lds + lay.off_sqcnt);
   L.ph_re = (const double*)(lds + lay.off_ph_re);
   L.ph_im = (const double*)(lds + lay.off_ph_im);
   L.invsqrt = (const double*)(lds + lay.off_invsqrt);
@@ -395,5 +427,76 @@ __device__ __forceinline__ GroupLds stage_group(const GroupDev& g, const GroupLd
   L.trivial = (const uint8_t*)(lds + lay.off_trivial);
   return L;
 }
+
+
+// ----------------------------------------------------------------------------
+// Torus canonicaliser with the per-row table entries kept in registers
+// (Ly <= LYMAX, minimal-rotation table in LDS).  Same result as
+// canonicalize() for kGroupTorus with g.minrot_table set.
+// ----------------------------------------------------------------------------
+template <int LYMAX>
+__device__ __forceinline__ void canonicalize_torus_reg(const GroupDev& g, const GroupLds& L, uint64_t b, uint64_t& s,
+                                                       int& h, int& cnt) {
+  uint32_t e[LYMAX];
+  uint32_t best = 0xffffffffu;
+#pragma unroll
+  for (int y = 0; y < LYMAX; ++y) {
+    if (y < g.Ly) {
+      e[y] = L.minrot[(uint32_t)((b >> (y * g.Lx)) & g.row_mask)];
+      const uint32_t mv = e[y] & 0xffffu;
+      best = mv < best ? mv : best;
+    }
+  }
+  s = ~0ull;
+  h = 0;
+  cnt = 0;
+#pragma unroll
+  for (int y = 0; y < LYMAX; ++y) {
+    if (y < g.Ly && (e[y] & 0xffffu) == best) {
+      const int bb = g.Ly - 1 - y;
+      const uint64_t shifted = rows_shift(b, bb, g.Lx, g.nb, g.full_mask);
+      uint32_t am = e[y] >> 16;
+      while (am) {
+        const int a = __builtin_ctz(am);
+        am &= am - 1;
+        const uint64_t img = rows_rotl(shifted, a, g.Lx, L.rot_hi, L.rot_lo);
+        const int el = L.g_of_ab[a + g.Lx * bb];
+        if (img < s) {
+          s = img;
+          h = el;
+          cnt = 1;
+        } else if (img == s) {
+          ++cnt;
+          if (L.inv[el] > L.inv[h]) h = el;
+        }
+      }
+    }
+  }
+}
+
+// ----------------------------------------------------------------------------
+// Open-addressing table of the representatives (linear probing) whose slots
+// carry the current input vector next to each key: one cache line per image.
+// ----------------------------------------------------------------------------
+constexpr uint64_t kHashEmpty = ~0ull;
+
+__device__ __forceinline__ uint32_t hash_word(uint64_t s) {
+  uint32_t lo = (uint32_t)s, hi = (uint32_t)(s >> 32);
+  uint32_t h = lo * 0x9E3779B1u ^ (hi + 0x7F4A7C15u) * 0x85EBCA77u;
+  h ^= h >> 16;
+  h *= 0x7FEB352Du;
+  h ^= h >> 15;
+  h *= 0x846CA68Bu;
+  h ^= h >> 16;
+  return h;
+}
+
+struct HashDev {
+  unsigned char* slots;  // n_slots * slot_bytes; key at offset 0, vector element at offset xoff
+  uint32_t mask;         // n_slots - 1 (power of two)
+  int slot_bytes;        // 16 or 32
+  int xoff;              // 8 or 16
+  int pad_;
+};
 
 }  // namespace edhip

@@ -40,6 +40,10 @@ hipError_t launch_export_mapping(const ReduceSource& src, const GroupDev& g, con
 
 hipError_t launch_lookup(const IndexDev& ix, int64_t n, const uint64_t* d_keys, int64_t* d_idx, hipStream_t st);
 
+// Bitmap over the words [0, 2^bits) with per-512-bit-block exclusive counts
+// (two-phase: call with d_tmp == nullptr to size the scan's scratch).
+hipError_t launch_bitmap_build(const uint64_t* d_keys, int64_t dim, uint64_t* d_bitmap, uint64_t n_blocks,
+                               uint32_t* d_counts, void* d_tmp, size_t& tmp_bytes, hipStream_t st);
 hipError_t exclusive_scan_u32_to_u64(const uint32_t* d_in, uint64_t* d_out, uint64_t n, void* d_tmp, size_t& tmp_bytes,
                                      hipStream_t st);
 hipError_t exclusive_scan_i64(const int64_t* d_in, int64_t* d_out, uint64_t n, void* d_tmp, size_t& tmp_bytes,
@@ -100,6 +104,39 @@ hipError_t launch_sparse_compact(int64_t dim, int slot, const int64_t* d_colptr,
 hipError_t launch_term_expect(const OpDev& op, int prec, const void* phi, const void* psi, double* d_partial,
                               int64_t n_blocks, double* d_out, hipStream_t st);
 int64_t term_expect_blocks(const OpDev& op);
+
+// ---- fast path: bond-pair term groups + register canonicaliser + hash lookup
+// Terms grouped by bit mask (1 or 2 bits).  For a row b the group's pattern
+// code = b[p] | b[q] << 1 (q = p for one-bit masks) selects the summed
+// diagonal amplitude and at most one off-diagonal entry (new pattern code +
+// amplitude).  meta[g] = p | q << 8 | offmask << 16 | newcode(4 x 2 bits) << 24.
+struct PairGroupsDev {
+  int n_groups;
+  int pad_;
+  const uint32_t* meta;
+  const double* dre;  // 4 per group
+  const double* dim;
+  const double* ore;  // 4 per group
+  const double* oim;
+  double dconst_re, dconst_im;  // identity terms (empty mask)
+};
+
+struct FastOpDev {
+  OpDev base;  // index, group, layout, rows, orbit, dim, rank table offsets (off terms unused)
+  PairGroupsDev pg;
+  HashDev hash;      // slots == nullptr: lookup through base.ix, x gathered separately
+  uint32_t pg_lds;   // LDS offset of the staged groups
+  uint32_t lds_total;
+};
+
+constexpr int kMaxPairGroups = 256;
+
+hipError_t launch_matvec_pairs(const FastOpDev& op, int prec, int transpose, int accumulate, const void* x, void* y,
+                               hipStream_t st);
+hipError_t launch_hash_build(const uint64_t* d_keys, int64_t dim, const HashDev& h, uint32_t* d_slot_of,
+                             hipStream_t st);
+hipError_t launch_hash_scatter(int prec, int64_t dim, const void* x, const uint32_t* d_slot_of, const HashDev& h,
+                               hipStream_t st);
 
 // ---- vecops.hip ------------------------------------------------------------
 // All reductions accumulate in double and are deterministic (fixed tree).

@@ -61,7 +61,7 @@ __device__ __forceinline__ void element_weight(const OpDev& op, const Staged& S,
   } else {
     double pr = S.L.ph_re[h];
     double pi = TRANS ? -S.L.ph_im[h] : S.L.ph_im[h];
-    double sc = sqrt_own * S.L.invsqrt[op.g.G / cnt];
+    double sc = sqrt_own * S.L.sqcnt[cnt];
     wr = (t.are * pr - t.aim * pi) * sc;
     wi = (t.are * pi + t.aim * pr) * sc;
   }
@@ -525,6 +525,250 @@ hipError_t launch_term_expect(const OpDev& op, int prec, const void* phi, const 
     default:
       return hipErrorInvalidValue;
   }
+}
+
+}  // namespace edhip
+
+// ============================================================================
+// Fast path: bond-pair groups, register torus canonicaliser, hash lookup with
+// the input vector embedded in the table slots.
+// ============================================================================
+namespace edhip {
+
+namespace {
+
+constexpr int kLyMax = 8;
+
+template <typename V>
+__device__ __forceinline__ bool hash_fetch(const HashDev& H, uint64_t s, V& xv) {
+  uint32_t h = hash_word(s) & H.mask;
+  for (;;) {
+    const unsigned char* sp = H.slots + (size_t)h * (size_t)H.slot_bytes;
+    const uint4 a = *(const uint4*)sp;
+    const uint64_t k = (uint64_t)a.x | ((uint64_t)a.y << 32);
+    if (k == s) {
+      // element at byte 8 of a 16-byte slot, or byte 16 of a 32-byte slot
+      const uint4 c = (H.xoff == 16) ? *(const uint4*)(sp + 16) : make_uint4(a.z, a.w, 0u, 0u);
+      if constexpr (VT<V>::cx && sizeof(typename VT<V>::R) == 8) {
+        xv.re = __hiloint2double((int)c.y, (int)c.x);
+        xv.im = __hiloint2double((int)c.w, (int)c.z);
+      } else if constexpr (VT<V>::cx) {
+        xv.re = __uint_as_float(c.x);
+        xv.im = __uint_as_float(c.y);
+      } else if constexpr (sizeof(V) == 8) {
+        xv = __hiloint2double((int)c.y, (int)c.x);
+      } else {
+        xv = __uint_as_float(c.x);
+      }
+      return true;
+    }
+    if (k == kHashEmpty) return false;
+    h = (h + 1) & H.mask;
+  }
+}
+
+template <typename V, int MODE, bool HASH, bool TRANS>
+__global__ __launch_bounds__(kBlock) void k_mv_pairs(FastOpDev F, const V* __restrict__ x, V* __restrict__ y,
+                                                     int accumulate) {
+  extern __shared__ unsigned char lds[];
+  const OpDev& op = F.base;
+  GroupLds L;
+  if constexpr (MODE != kGroupNone) L = stage_group(op.g, op.lay, lds);
+  const int ng = F.pg.n_groups;
+  uint32_t* meta = (uint32_t*)(lds + F.pg_lds);
+  double* dre = (double*)(lds + F.pg_lds + (uint32_t)(kMaxPairGroups * 4));
+  double* dim_ = dre + 4 * ng;
+  double* ore = dim_ + 4 * ng;
+  double* oim = ore + 4 * ng;
+  lds_copy((unsigned char*)meta, F.pg.meta, (uint32_t)(ng * 4));
+  lds_copy((unsigned char*)dre, F.pg.dre, (uint32_t)(ng * 32));
+  lds_copy((unsigned char*)dim_, F.pg.dim, (uint32_t)(ng * 32));
+  lds_copy((unsigned char*)ore, F.pg.ore, (uint32_t)(ng * 32));
+  lds_copy((unsigned char*)oim, F.pg.oim, (uint32_t)(ng * 32));
+  lds_copy(lds + op.w_lds, op.ix.W, op.w_bytes);
+  const uint64_t* Wl = (const uint64_t*)(lds + op.w_lds);
+  __syncthreads();
+  const int64_t row = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (row >= op.dim) return;
+  using R = typename VT<V>::R;
+  const uint64_t b = op.rows[row];
+
+  // one pass over the groups: summed diagonal and the matched groups
+  double dr = F.pg.dconst_re, di = F.pg.dconst_im;
+  uint64_t bw[kMaxPairGroups / 64];
+#pragma unroll
+  for (int w = 0; w < kMaxPairGroups / 64; ++w) {
+    uint64_t bits = 0;
+    const int g0 = w * 64;
+    const int gn = (ng - g0) < 64 ? (ng - g0) : 64;
+    for (int j = 0; j < gn; ++j) {
+      const uint32_t m = meta[g0 + j];
+      const uint32_t code = (uint32_t)((b >> (m & 63u)) & 1ull) | ((uint32_t)((b >> ((m >> 8) & 63u)) & 1ull) << 1);
+      dr += dre[(g0 + j) * 4 + code];
+      di += dim_[(g0 + j) * 4 + code];
+      bits |= (uint64_t)((m >> (16 + code)) & 1u) << j;
+    }
+    bw[w] = bits;
+  }
+  V acc = vzero<V>();
+  vfma(acc, (R)dr, (R)di, x[row]);
+  double sqrt_own = 1.0;
+  if constexpr (MODE != kGroupNone) sqrt_own = sqrt((double)op.orbit[row]);
+
+  for (;;) {
+    int g;
+    if (bw[0]) {
+      g = __builtin_ctzll(bw[0]);
+      bw[0] &= bw[0] - 1;
+    } else if (bw[1]) {
+      g = 64 + __builtin_ctzll(bw[1]);
+      bw[1] &= bw[1] - 1;
+    } else if (bw[2]) {
+      g = 128 + __builtin_ctzll(bw[2]);
+      bw[2] &= bw[2] - 1;
+    } else if (bw[3]) {
+      g = 192 + __builtin_ctzll(bw[3]);
+      bw[3] &= bw[3] - 1;
+    } else {
+      break;
+    }
+    const uint32_t m = meta[g];
+    const uint32_t p = m & 63u, q = (m >> 8) & 63u;
+    const uint32_t code = (uint32_t)((b >> p) & 1ull) | ((uint32_t)((b >> q) & 1ull) << 1);
+    const uint32_t d = code ^ ((m >> (24 + 2 * code)) & 3u);
+    const uint64_t img = b ^ (((uint64_t)(d & 1u) << p) | ((uint64_t)(d >> 1) << q));
+    uint64_t s;
+    int h, cnt;
+    if constexpr (MODE == kGroupTorus) {
+      canonicalize_torus_reg<kLyMax>(op.g, L, img, s, h, cnt);
+    } else {
+      s = img;
+      h = 0;
+      cnt = 1;
+    }
+    V xv;
+    bool found;
+    if constexpr (HASH) {
+      found = hash_fetch(F.hash, s, xv);
+    } else {
+      const int64_t idx = index_of(op.ix, Wl, s);
+      found = idx >= 0;
+      if (found) xv = x[idx];
+    }
+    if (found) {
+      const double ar = ore[g * 4 + code], ai = oim[g * 4 + code];
+      double wr = ar, wi = ai;
+      if constexpr (MODE != kGroupNone) {
+        const double pr = L.ph_re[h];
+        const double pi = TRANS ? -L.ph_im[h] : L.ph_im[h];
+        const double sc = sqrt_own * L.sqcnt[cnt];
+        wr = (ar * pr - ai * pi) * sc;
+        wi = (ar * pi + ai * pr) * sc;
+      }
+      vfma(acc, (R)wr, (R)wi, xv);
+    }
+  }
+  y[row] = accumulate ? vadd(y[row], acc) : acc;
+}
+
+template <typename V, int MODE, bool HASH>
+hipError_t launch_pairs_t(const FastOpDev& op, int transpose, int accumulate, const void* x, void* y, hipStream_t st) {
+  dim3 grid((unsigned)((op.base.dim + kBlock - 1) / kBlock));
+  if (transpose)
+    hipLaunchKernelGGL((k_mv_pairs<V, MODE, HASH, true>), grid, dim3(kBlock), op.lds_total, st, op, (const V*)x, (V*)y,
+                       accumulate);
+  else
+    hipLaunchKernelGGL((k_mv_pairs<V, MODE, HASH, false>), grid, dim3(kBlock), op.lds_total, st, op, (const V*)x,
+                       (V*)y, accumulate);
+  return hipGetLastError();
+}
+
+template <typename V>
+hipError_t launch_pairs_v(const FastOpDev& op, int transpose, int accumulate, const void* x, void* y, hipStream_t st) {
+  const bool hash = op.hash.slots != nullptr;
+  if (op.base.g.mode == kGroupTorus)
+    return hash ? launch_pairs_t<V, kGroupTorus, true>(op, transpose, accumulate, x, y, st)
+                : launch_pairs_t<V, kGroupTorus, false>(op, transpose, accumulate, x, y, st);
+  return hash ? launch_pairs_t<V, kGroupNone, true>(op, transpose, accumulate, x, y, st)
+              : launch_pairs_t<V, kGroupNone, false>(op, transpose, accumulate, x, y, st);
+}
+
+__global__ __launch_bounds__(kBlock) void k_hash_build(const uint64_t* __restrict__ keys, int64_t dim, HashDev H,
+                                                       uint32_t* __restrict__ slot_of) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= dim) return;
+  const uint64_t s = keys[i];
+  uint32_t h = hash_word(s) & H.mask;
+  for (;;) {
+    unsigned long long* kp = (unsigned long long*)(H.slots + (size_t)h * (size_t)H.slot_bytes);
+    const unsigned long long prev = atomicCAS(kp, (unsigned long long)kHashEmpty, (unsigned long long)s);
+    if (prev == (unsigned long long)kHashEmpty || prev == (unsigned long long)s) {
+      slot_of[i] = h;
+      return;
+    }
+    h = (h + 1) & H.mask;
+  }
+}
+
+template <typename V>
+__global__ __launch_bounds__(kBlock) void k_hash_scatter(int64_t dim, const V* __restrict__ x,
+                                                         const uint32_t* __restrict__ slot_of, HashDev H) {
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < dim; i += (int64_t)gridDim.x * kBlock) {
+    *(V*)(H.slots + (size_t)slot_of[i] * (size_t)H.slot_bytes + H.xoff) = x[i];
+  }
+}
+
+}  // namespace
+
+hipError_t launch_matvec_pairs(const FastOpDev& op, int prec, int transpose, int accumulate, const void* x, void* y,
+                               hipStream_t st) {
+  if (op.base.dim == 0) return hipSuccess;
+  switch (prec) {
+    case kF64:
+      return launch_pairs_v<double>(op, transpose, accumulate, x, y, st);
+    case kC128:
+      return launch_pairs_v<Cx<double>>(op, transpose, accumulate, x, y, st);
+    case kF32:
+      return launch_pairs_v<float>(op, transpose, accumulate, x, y, st);
+    case kC64:
+      return launch_pairs_v<Cx<float>>(op, transpose, accumulate, x, y, st);
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_hash_build(const uint64_t* d_keys, int64_t dim, const HashDev& h, uint32_t* d_slot_of,
+                             hipStream_t st) {
+  if (dim == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_hash_build, dim3((unsigned)((dim + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, d_keys, dim, h,
+                     d_slot_of);
+  return hipGetLastError();
+}
+
+hipError_t launch_hash_scatter(int prec, int64_t dim, const void* x, const uint32_t* d_slot_of, const HashDev& h,
+                               hipStream_t st) {
+  if (dim == 0) return hipSuccess;
+  int64_t nb = (dim + kBlock - 1) / kBlock;
+  if (nb > 16384) nb = 16384;
+  switch (prec) {
+    case kF64:
+      hipLaunchKernelGGL(k_hash_scatter<double>, dim3((unsigned)nb), dim3(kBlock), 0, st, dim, (const double*)x, d_slot_of, h);
+      break;
+    case kC128:
+      hipLaunchKernelGGL(k_hash_scatter<Cx<double>>, dim3((unsigned)nb), dim3(kBlock), 0, st, dim, (const Cx<double>*)x,
+                         d_slot_of, h);
+      break;
+    case kF32:
+      hipLaunchKernelGGL(k_hash_scatter<float>, dim3((unsigned)nb), dim3(kBlock), 0, st, dim, (const float*)x, d_slot_of, h);
+      break;
+    case kC64:
+      hipLaunchKernelGGL(k_hash_scatter<Cx<float>>, dim3((unsigned)nb), dim3(kBlock), 0, st, dim, (const Cx<float>*)x,
+                         d_slot_of, h);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
 }
 
 }  // namespace edhip

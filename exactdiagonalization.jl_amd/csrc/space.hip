@@ -143,6 +143,23 @@ __global__ __launch_bounds__(kBlock) void k_lookup(IndexDev ix, int64_t n, const
   out[i] = index_of(ix, Wl, keys[i]);
 }
 
+__global__ __launch_bounds__(kBlock) void k_bitmap_set(const uint64_t* __restrict__ keys, int64_t dim,
+                                                       unsigned long long* __restrict__ bitmap) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= dim) return;
+  const uint64_t k = keys[i];
+  atomicOr(bitmap + (k >> 6), 1ull << (k & 63u));
+}
+
+__global__ __launch_bounds__(kBlock) void k_bitmap_block_counts(const uint64_t* __restrict__ bitmap, uint64_t n_blocks,
+                                                                uint32_t* __restrict__ counts) {
+  uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n_blocks) return;
+  uint32_t c = 0;
+  for (int i = 0; i < 8; ++i) c += (uint32_t)__builtin_popcountll(bitmap[j * 8 + i]);
+  counts[j] = c;
+}
+
 __global__ void k_u32_to_u64(const uint32_t* __restrict__ in, uint64_t* __restrict__ out, uint64_t n) {
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) out[i] = in[i];
@@ -203,6 +220,23 @@ hipError_t launch_lookup(const IndexDev& ix, int64_t n, const uint64_t* d_keys, 
   uint32_t lds = ix.rank_mode ? w_bytes(ix.n_bits) : 0u;
   hipLaunchKernelGGL(k_lookup, dim3(grid_for((uint64_t)n)), dim3(kBlock), lds, st, ix, n, d_keys, d_idx);
   return hipGetLastError();
+}
+
+hipError_t launch_bitmap_build(const uint64_t* d_keys, int64_t dim, uint64_t* d_bitmap, uint64_t n_blocks,
+                               uint32_t* d_counts, void* d_tmp, size_t& tmp_bytes, hipStream_t st) {
+  if (d_tmp == nullptr) {
+    return hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, d_counts, d_counts, (int64_t)n_blocks, st);
+  }
+  if (dim > 0) {
+    hipLaunchKernelGGL(k_bitmap_set, dim3(grid_for((uint64_t)dim)), dim3(kBlock), 0, st, d_keys, dim,
+                       (unsigned long long*)d_bitmap);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(k_bitmap_block_counts, dim3(grid_for(n_blocks)), dim3(kBlock), 0, st, d_bitmap, n_blocks, d_counts);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  return hipcub::DeviceScan::ExclusiveSum(d_tmp, tmp_bytes, d_counts, d_counts, (int64_t)n_blocks, st);
 }
 
 hipError_t exclusive_scan_u32_to_u64(const uint32_t* d_in, uint64_t* d_out, uint64_t n, void* d_tmp, size_t& tmp_bytes,

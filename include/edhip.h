@@ -146,6 +146,21 @@ int edhip_apply(edhip_op* op, int prec, int transpose, int accumulate, int64_t n
 int edhip_apply_host(edhip_op* op, int prec, int transpose, int accumulate, int64_t n_x, const void* x_host,
                      int64_t n_y, void* y_host);
 
+/* Kernel and lookup used by apply/mul/eigensolver calls (performance knob;
+ * every choice gives the same matrix):
+ *   kernel: 0 auto (term list), 1 term-list kernel, 2 bond-pair kernel
+ *           (operators whose terms act on one or two bits, plain or
+ *           torus-translation spaces)
+ *   lookup: 0 auto (bitmap when available, else sorted search),
+ *           1 sorted search behind the prefix table,
+ *           2 open-addressing table of the basis with the input vector
+ *             embedded (bond-pair kernel only; built on first use),
+ *           3 bitmap rank index (words of at most 36 bits)
+ *   a full popcount sector always uses rank arithmetic.
+ * edhip_op_last_strategy reports what the last product used. */
+int edhip_op_set_strategy(edhip_op* op, int kernel, int lookup);
+int edhip_op_last_strategy(const edhip_op* op, int* kernel, int* lookup);
+
 /* Number of off-diagonal images the row iterators generate over all rows
  * (terms whose bitrow matches), i.e. the off-diagonal elements one apply!
  * visits; used for the element-throughput metric. */
