@@ -125,6 +125,8 @@ SIGNATURES = {
     "edhip_space_copy_orbit": (_i32, [_vp, _P(ctypes.c_uint16)]),
     "edhip_space_lookup": (_i32, [_vp, _i64, _P(_u64), _P(_i64)]),
     "edhip_space_export_mapping": (_i32, [_vp, _P(_i64), _P(_dbl), _P(_dbl)]),
+    "edhip_space_vector_reduce": (_i32, [_vp, _i32, _i64, _vp, _i64, _vp]),
+    "edhip_space_vector_unreduce": (_i32, [_vp, _i32, _i64, _vp, _i64, _vp]),
     "edhip_space_info": (_i32, [_vp, _P(_i32), _P(_i32), _P(_i32)]),
     "edhip_op_create": (_i32, [_vp, _vp, _i64, _P(_u64), _P(_u64), _P(_u64), _P(_dbl), _P(_dbl), _P(_vp)]),
     "edhip_op_destroy": (None, [_vp]),

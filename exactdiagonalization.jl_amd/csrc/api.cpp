@@ -1381,6 +1381,44 @@ int edhip_space_export_mapping(const edhip_space* s, int64_t* idx, double* are, 
   });
 }
 
+namespace {
+void check_vector_map(const edhip_space* s, int prec, int64_t n_large, const void* large, int64_t n_small,
+                      const void* small) {
+  if (!s->group || !s->parent) fail(EDHIP_ERR_ARGUMENT, "space is not symmetry reduced");
+  if (elem_size(prec) == 0) fail(EDHIP_ERR_ARGUMENT, "unknown precision code");
+  if (!prec_is_complex(prec) && s->group->complex_phase)
+    fail(EDHIP_ERR_ARGUMENT, "complex characters need a complex vector type");
+  if (n_large != s->parent->dim)
+    fail(EDHIP_ERR_DIMENSION, "Dimension of the input vector should match the larger representation");
+  if (n_small != s->dim)
+    fail(EDHIP_ERR_DIMENSION, "Dimension of the input vector should match the reduced representation");
+  if ((n_large > 0 && !large) || (n_small > 0 && !small)) fail(EDHIP_ERR_ARGUMENT, "null vector");
+}
+}  // namespace
+
+int edhip_space_vector_reduce(const edhip_space* s, int prec, int64_t n_large, const void* large, int64_t n_small,
+                              void* small) {
+  if (!s) return EDHIP_ERR_ARGUMENT;
+  return guarded(s->ctx, [&] {
+    check_vector_map(s, prec, n_large, large, n_small, small);
+    if (s->dim == 0) return;
+    s->materialize();
+    HIP_OR_THROW(launch_vec_reduce(s->keys.as<uint64_t>(), s->orbit.as<uint16_t>(), s->dim, s->group->dev,
+                                   s->group->lay, s->parent->index_dev(), prec, large, small, s->ctx->stream));
+  });
+}
+
+int edhip_space_vector_unreduce(const edhip_space* s, int prec, int64_t n_small, const void* small, int64_t n_large,
+                                void* large) {
+  if (!s) return EDHIP_ERR_ARGUMENT;
+  return guarded(s->ctx, [&] {
+    check_vector_map(s, prec, n_large, large, n_small, small);
+    if (n_large == 0) return;
+    HIP_OR_THROW(launch_vec_unreduce(s->parent->source(), s->group->dev, s->group->lay, s->index_dev(1),
+                                     s->orbit.as<uint16_t>(), prec, small, large, s->ctx->stream));
+  });
+}
+
 int edhip_space_info(const edhip_space* s, int* reduced, int* group_order, int* canon_kind) {
   if (!s) return EDHIP_ERR_ARGUMENT;
   if (reduced) *reduced = s->group ? 1 : 0;

@@ -40,6 +40,16 @@ hipError_t launch_export_mapping(const ReduceSource& src, const GroupDev& g, con
 
 hipError_t launch_lookup(const IndexDev& ix, int64_t n, const uint64_t* d_keys, int64_t* d_idx, hipStream_t st);
 
+// Vector maps between a parent space and its reduction, in precision prec:
+// large[p] = U[p] small[r(p)] (unreduce), small[r] = sum conj(U[p]) large[p]
+// over the orbit of representative r (reduce).
+hipError_t launch_vec_unreduce(const ReduceSource& src, const GroupDev& g, const GroupLdsLayout& lay,
+                               const IndexDev& reduced_index, const uint16_t* d_orbit, int prec, const void* d_small,
+                               void* d_large, hipStream_t st);
+hipError_t launch_vec_reduce(const uint64_t* d_reps, const uint16_t* d_orbit, int64_t dim, const GroupDev& g,
+                             const GroupLdsLayout& lay, const IndexDev& parent_index, int prec, const void* d_large,
+                             void* d_small, hipStream_t st);
+
 // Bitmap over the words [0, 2^bits) with per-512-bit-block exclusive counts
 // (two-phase: call with d_tmp == nullptr to size the scan's scratch).
 hipError_t launch_bitmap_build(const uint64_t* d_keys, int64_t dim, uint64_t* d_bitmap, uint64_t n_blocks,

@@ -129,6 +129,77 @@ __global__ __launch_bounds__(kBlock) void k_export_mapping(ReduceSource src, Gro
   im_out[p] = g.ph_im[h] * scale;
 }
 
+// symmetry_unreduce (symmetry_reduce.jl:208-225): large[p] = U[p] * small[r(p)]
+// for every parent word p, 0 where p lies outside the reduced space.  The
+// mapping (r, U) is recomputed as in k_export_mapping; one lane per parent
+// entry, coalesced writes.
+template <typename V>
+__global__ __launch_bounds__(kBlock) void k_vec_unreduce(ReduceSource src, GroupDev g, GroupLdsLayout lay, IndexDev ix,
+                                                         const uint16_t* __restrict__ orbit, const V* __restrict__ small,
+                                                         V* __restrict__ large) {
+  extern __shared__ unsigned char lds[];
+  GroupLds L = stage_group(g, lay, lds);
+  const uint64_t* Wl = stage_W(src, lds, lay.total);
+  __syncthreads();
+  uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= src.n_words) return;
+  uint64_t w = src.list ? src.list[p] : unrank_word(p, src.n_bits, Wl);
+  uint64_t s;
+  int h, cnt;
+  canonicalize(g, L, w, s, h, cnt);
+  int64_t r = search_sorted(ix.keys, ix.prefix, ix.n_buckets, ix.shift, s);
+  V out = vzero<V>();
+  if (r >= 0) {
+    using R = typename VT<V>::R;
+    const double scale = g.invsqrt[orbit[r]];
+    vfma(out, (R)(L.ph_re[h] * scale), (R)(L.ph_im[h] * scale), small[r]);
+  }
+  large[p] = out;
+}
+
+// symmetry_reduce of a vector (symmetry_reduce.jl:38-56):
+//   small[r] = sum_{p in orbit(r)} conj(U[p]) * large[p].
+// Gather form, one lane per representative s: every element g gives the
+// orbit member g(s) with conj(U[g(s)]) = chi(g) / sqrt(N_s); a member reached
+// by |Stab(s)| elements (all with the same character, since s is compatible)
+// is counted once by dividing by |Stab(s)| = G / N_s.
+template <typename V>
+__global__ __launch_bounds__(kBlock) void k_vec_reduce(const uint64_t* __restrict__ reps, const uint16_t* __restrict__ orbit,
+                                                       int64_t dim, GroupDev g, GroupLdsLayout lay, IndexDev pix,
+                                                       const V* __restrict__ large, V* __restrict__ small) {
+  extern __shared__ unsigned char lds[];
+  GroupLds L = stage_group(g, lay, lds);
+  uint64_t* Wl = (uint64_t*)(lds + lay.total);
+  if (pix.rank_mode) {
+    const int nw = (pix.n_bits + 1) * (pix.n_bits + 1);
+    for (int i = threadIdx.x; i < nw; i += blockDim.x) Wl[i] = pix.W[i];
+  }
+  __syncthreads();
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= dim) return;
+  using R = typename VT<V>::R;
+  const uint64_t s = reps[r];
+  const int n_orb = orbit[r];
+  const double scale = g.invsqrt[n_orb] * (double)n_orb / (double)g.G;
+  V acc = vzero<V>();
+  auto add = [&](int e, uint64_t img) {
+    const int64_t p = index_of(pix, Wl, img);
+    if (p < 0) return;
+    const int ie = L.inv[e];  // chi(e) = conj(phase[e^-1])
+    vfma(acc, (R)(L.ph_re[ie] * scale), (R)(-L.ph_im[ie] * scale), large[p]);
+  };
+  if (g.mode == kGroupTorus) {
+    for (int bb = 0; bb < g.Ly; ++bb) {
+      const uint64_t shifted = rows_shift(s, bb, g.Lx, g.nb, g.full_mask);
+      for (int a = 0; a < g.Lx; ++a) add(L.g_of_ab[a + g.Lx * bb], rows_rotl(shifted, a, g.Lx, L.rot_hi, L.rot_lo));
+    }
+  } else {
+    const int stride = g.n_nib << 4;
+    for (int e = 0; e < g.G; ++e) add(e, perm_apply_nib(L.nib + e * stride, g.n_nib, s));
+  }
+  small[r] = acc;
+}
+
 __global__ __launch_bounds__(kBlock) void k_lookup(IndexDev ix, int64_t n, const uint64_t* __restrict__ keys,
                                                    int64_t* __restrict__ out) {
   extern __shared__ unsigned char lds[];
@@ -259,6 +330,62 @@ hipError_t exclusive_scan_i64(const int64_t* d_in, int64_t* d_out, uint64_t n, v
   }
   if (n == 0) return hipSuccess;
   return hipcub::DeviceScan::ExclusiveSum(d_tmp, tmp_bytes, d_in, d_out, (int64_t)n, st);
+}
+
+namespace {
+template <typename V>
+hipError_t vec_unreduce_t(const ReduceSource& src, const GroupDev& g, const GroupLdsLayout& lay, const IndexDev& ix,
+                          const uint16_t* d_orbit, const void* small, void* large, hipStream_t st) {
+  uint32_t lds = lay.total + (src.list ? 0u : w_bytes(src.n_bits));
+  hipLaunchKernelGGL(k_vec_unreduce<V>, dim3(grid_for(src.n_words)), dim3(kBlock), lds, st, src, g, lay, ix, d_orbit,
+                     (const V*)small, (V*)large);
+  return hipGetLastError();
+}
+
+template <typename V>
+hipError_t vec_reduce_t(const uint64_t* d_reps, const uint16_t* d_orbit, int64_t dim, const GroupDev& g,
+                        const GroupLdsLayout& lay, const IndexDev& pix, const void* large, void* small, hipStream_t st) {
+  uint32_t lds = lay.total + (pix.rank_mode ? w_bytes(pix.n_bits) : 0u);
+  hipLaunchKernelGGL(k_vec_reduce<V>, dim3(grid_for((uint64_t)dim)), dim3(kBlock), lds, st, d_reps, d_orbit, dim, g, lay,
+                     pix, (const V*)large, (V*)small);
+  return hipGetLastError();
+}
+}  // namespace
+
+hipError_t launch_vec_unreduce(const ReduceSource& src, const GroupDev& g, const GroupLdsLayout& lay,
+                               const IndexDev& reduced_index, const uint16_t* d_orbit, int prec, const void* d_small,
+                               void* d_large, hipStream_t st) {
+  if (src.n_words == 0) return hipSuccess;
+  switch (prec) {
+    case kF64:
+      return vec_unreduce_t<double>(src, g, lay, reduced_index, d_orbit, d_small, d_large, st);
+    case kC128:
+      return vec_unreduce_t<Cx<double>>(src, g, lay, reduced_index, d_orbit, d_small, d_large, st);
+    case kF32:
+      return vec_unreduce_t<float>(src, g, lay, reduced_index, d_orbit, d_small, d_large, st);
+    case kC64:
+      return vec_unreduce_t<Cx<float>>(src, g, lay, reduced_index, d_orbit, d_small, d_large, st);
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_vec_reduce(const uint64_t* d_reps, const uint16_t* d_orbit, int64_t dim, const GroupDev& g,
+                             const GroupLdsLayout& lay, const IndexDev& parent_index, int prec, const void* d_large,
+                             void* d_small, hipStream_t st) {
+  if (dim == 0) return hipSuccess;
+  switch (prec) {
+    case kF64:
+      return vec_reduce_t<double>(d_reps, d_orbit, dim, g, lay, parent_index, d_large, d_small, st);
+    case kC128:
+      return vec_reduce_t<Cx<double>>(d_reps, d_orbit, dim, g, lay, parent_index, d_large, d_small, st);
+    case kF32:
+      return vec_reduce_t<float>(d_reps, d_orbit, dim, g, lay, parent_index, d_large, d_small, st);
+    case kC64:
+      return vec_reduce_t<Cx<float>>(d_reps, d_orbit, dim, g, lay, parent_index, d_large, d_small, st);
+    default:
+      return hipErrorInvalidValue;
+  }
 }
 
 }  // namespace edhip

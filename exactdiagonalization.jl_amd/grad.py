@@ -94,9 +94,7 @@ def correlator_and_grad(model: LinearCouplings, c, dc_dtheta, observable_op, *, 
     Opsi = torch.zeros_like(psi)
     from .representation import apply_
 
-    torch.cuda.synchronize()
     apply_(Opsi, O, psi)
-    O.ctx.synchronize()
     expO = torch.vdot(psi, Opsi).real.item()
     rhs = (Opsi - expO * psi).contiguous()
     lam, iters, res = solve.minres(H, rhs, shift=E0, deflate=psi, tol=minres_tol)
@@ -148,9 +146,7 @@ def _torch_function():
             from .representation import apply_
 
             Opsi = torch.zeros_like(psi)
-            torch.cuda.synchronize()
             apply_(Opsi, O, psi)
-            O.ctx.synchronize()
             val = torch.vdot(psi, Opsi).real.item()
             ctx.model, ctx.coupling_fn, ctx.obs, ctx.opts = model, coupling_fn, observable_op, opts
             ctx.save_for_backward(theta)

@@ -307,14 +307,50 @@ symmetry_reduce_serial = symmetry_reduce
 symmetry_reduce_parallel = symmetry_reduce
 
 
+def _vector_map(rhsr: ReducedHilbertSpaceRepresentation, vec, reduce: bool):
+    """Device vector map between rhsr.parent and rhsr.  The result type is
+    promote_type(C, eltype(vec)) widened to 64-bit components, with C complex
+    when the amplitudes are (symmetry_reduce.jl:43,215).  GPU tensors give a
+    GPU tensor, anything else a numpy array."""
+    import torch
+
+    if not isinstance(rhsr, ReducedHilbertSpaceRepresentation):
+        raise ArgumentError("expected a ReducedHilbertSpaceRepresentation")
+    amps = np.array([a for _, a in _symops(rhsr.symops_and_amplitudes)])
+    on_device = _is_torch(vec)
+    if on_device:
+        if not vec.is_cuda:
+            raise ArgumentError("a tensor argument must live on the GPU")
+        cx = np.iscomplexobj(amps) or vec.is_complex()
+        src = vec.reshape(-1)
+    else:
+        arr = np.asarray(vec)
+        cx = np.iscomplexobj(amps) or np.iscomplexobj(arr)
+        src = torch.from_numpy(np.ascontiguousarray(arr.reshape(-1)))
+    dt = torch.complex128 if cx else torch.float64
+    x = src.to(device=f"cuda:{rhsr.ctx.device}", dtype=dt).contiguous()
+    n_out = rhsr.dimension if reduce else rhsr.parent.dimension
+    y = torch.empty(max(n_out, 1), dtype=dt, device=x.device)
+    prec = _lib.C128 if cx else _lib.F64
+    fn = _lib.lib().edhip_space_vector_reduce if reduce else _lib.lib().edhip_space_vector_unreduce
+    _device_call(rhsr.ctx, fn, rhsr._handle, prec, x.numel(), ctypes.c_void_p(x.data_ptr()), n_out,
+                 ctypes.c_void_p(y.data_ptr()))
+    y = y[:n_out]
+    return y if on_device else y.cpu().numpy()
+
+
 def symmetry_reduce_vector(rhsr: ReducedHilbertSpaceRepresentation, large_vector):
-    """small[r] = sum_p conj(U[p]) large[p] (symmetry_reduce.jl:38-56)."""
-    raise NotImplementedError("vector reduce/unreduce maps are not built on the device yet")
+    """symmetry_reduce(rhsr, large_vector) (symmetry_reduce.jl:29-56):
+    small[r] = sum_p conj(U[p]) large[p] over the parent words p of the orbit
+    of r.  DimensionMismatch unless len(large_vector) == dimension(parent)."""
+    return _vector_map(rhsr, large_vector, True)
 
 
 def symmetry_unreduce(rhsr: ReducedHilbertSpaceRepresentation, small_vector):
-    """large[p] = U[p] small[r] (symmetry_reduce.jl:208-225)."""
-    raise NotImplementedError("vector reduce/unreduce maps are not built on the device yet")
+    """symmetry_unreduce(rhsr, small_vector) (symmetry_reduce.jl:208-225):
+    large[p] = U[p] small[r(p)], 0 outside the reduced space.
+    DimensionMismatch unless len(small_vector) == dimension(rhsr)."""
+    return _vector_map(rhsr, small_vector, False)
 
 
 # ---------------------------------------------------------------------------
@@ -324,6 +360,18 @@ def symmetry_unreduce(rhsr: ReducedHilbertSpaceRepresentation, small_vector):
 
 def _is_torch(x):
     return type(x).__module__.startswith("torch")
+
+
+def _device_call(ctx, fn, *args):
+    """Run a library call on GPU tensors.  The library works on its own
+    non-blocking stream: wait for the caller's current stream before the call
+    (its producers of the inputs) and for the library's stream after it (so
+    that the outputs are complete for whatever reads them next)."""
+    import torch
+
+    torch.cuda.current_stream(ctx.device).synchronize()
+    ctx.check(fn(*args))
+    ctx.synchronize()
 
 
 def _torch_prec(t):
@@ -422,10 +470,8 @@ class AbstractOperatorRepresentation:
                 raise ArgumentError("device vectors must share one of float64/complex128/float32/complex64")
             if not (out.is_contiguous() and state.is_contiguous()):
                 raise ArgumentError("device vectors must be contiguous")
-            self.ctx.check(
-                lib.edhip_apply(self._handle, prec, int(transpose), int(accumulate), state.numel(),
-                                ctypes.c_void_p(state.data_ptr()), out.numel(), ctypes.c_void_p(out.data_ptr()))
-            )
+            _device_call(self.ctx, lib.edhip_apply, self._handle, prec, int(transpose), int(accumulate), state.numel(),
+                         ctypes.c_void_p(state.data_ptr()), out.numel(), ctypes.c_void_p(out.data_ptr()))
             return out
         out_arr = out
         if not isinstance(out_arr, np.ndarray):

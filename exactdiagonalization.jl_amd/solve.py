@@ -15,6 +15,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import ArgumentError
+from .representation import _device_call
 
 
 @dataclass
@@ -72,10 +73,12 @@ def eigs(opr, nev: int = 1, which: str = "SR", *, mode: str = "fp64", tol: float
         dt = torch.complex128 if opr.is_complex else torch.float64
         vecs = torch.empty((nev, opr.dimension), dtype=dt, device=f"cuda:{opr.ctx.device}")
         vptr = ctypes.c_void_p(vecs.data_ptr())
-        torch.cuda.synchronize()
-    opr.ctx.check(
-        _lib.lib().edhip_lanczos(opr._handle, ctypes.byref(o), _lib.ptr(vals, ctypes.c_double), vptr, ctypes.byref(st))
-    )
+        _device_call(opr.ctx, _lib.lib().edhip_lanczos, opr._handle, ctypes.byref(o), _lib.ptr(vals, ctypes.c_double), vptr,
+                     ctypes.byref(st))
+    else:
+        opr.ctx.check(
+            _lib.lib().edhip_lanczos(opr._handle, ctypes.byref(o), _lib.ptr(vals, ctypes.c_double), vptr, ctypes.byref(st))
+        )
     return EigsResult(
         values=vals,
         vectors=vecs,
@@ -112,12 +115,9 @@ def minres(opr, b, *, shift: float = 0.0, deflate=None, tol: float = 1e-10, max_
             raise ArgumentError("deflation vectors must match b's dtype")
         nd = d.shape[0]
         dptr = ctypes.c_void_p(d.data_ptr())
-    torch.cuda.synchronize()
     st = _lib.MinresStats()
-    opr.ctx.check(
-        _lib.lib().edhip_minres(opr._handle, prec, float(shift), nd, dptr, ctypes.c_void_p(b.data_ptr()),
-                                ctypes.c_void_p(x.data_ptr()), float(tol), int(max_iter), ctypes.byref(st))
-    )
+    _device_call(opr.ctx, _lib.lib().edhip_minres, opr._handle, prec, float(shift), nd, dptr, ctypes.c_void_p(b.data_ptr()),
+                 ctypes.c_void_p(x.data_ptr()), float(tol), int(max_iter), ctypes.byref(st))
     return x, st.iterations, st.residual
 
 
@@ -131,9 +131,6 @@ def term_expectations(opr, phi, psi) -> np.ndarray:
     if prec is None or phi.dtype != psi.dtype:
         raise ArgumentError("phi and psi must share a supported dtype")
     out = np.zeros(2 * max(opr._n_terms, 1))
-    torch.cuda.synchronize()
-    opr.ctx.check(
-        _lib.lib().edhip_term_expect(opr._handle, prec, ctypes.c_void_p(phi.data_ptr()), ctypes.c_void_p(psi.data_ptr()),
-                                     _lib.ptr(out, ctypes.c_double))
-    )
+    _device_call(opr.ctx, _lib.lib().edhip_term_expect, opr._handle, prec, ctypes.c_void_p(phi.data_ptr()),
+                 ctypes.c_void_p(psi.data_ptr()), _lib.ptr(out, ctypes.c_double))
     return out[0::2][: opr._n_terms] + 1j * out[1::2][: opr._n_terms]

@@ -114,6 +114,18 @@ int edhip_space_lookup(const edhip_space* space, int64_t n, const uint64_t* keys
  * Arrays have the parent's dimension.  For parity checks; O(parent). */
 int edhip_space_export_mapping(const edhip_space* reduced, int64_t* idx_host, double* amp_re_host,
                                double* amp_im_host);
+/* symmetry_reduce(rhsr, large_vector) (src/Symmetry/symmetry_reduce.jl:38-56):
+ * small[r] = sum over the parent words p of the orbit of r of conj(U[p]) large[p].
+ * symmetry_unreduce(rhsr, small_vector) (symmetry_reduce.jl:208-225):
+ * large[p] = U[p] small[r(p)], 0 for parent words outside the reduced space.
+ * Device vectors of precision prec; n_large must be the parent's dimension and
+ * n_small the reduced one (EDHIP_ERR_DIMENSION otherwise).  Complex
+ * characters need a complex precision (the caller promotes, as promote_type
+ * does at :43 and :215). */
+int edhip_space_vector_reduce(const edhip_space* reduced, int prec, int64_t n_large, const void* large,
+                              int64_t n_small, void* small);
+int edhip_space_vector_unreduce(const edhip_space* reduced, int prec, int64_t n_small, const void* small,
+                                int64_t n_large, void* large);
 /* 1 if the space is symmetry reduced; group order and canonicaliser kind
  * (0 none, 1 generic LDS tables, 2 torus translations) */
 int edhip_space_info(const edhip_space* space, int* reduced, int* group_order, int* canon_kind);
