@@ -15,8 +15,13 @@ apply_serial! (the reference algorithm: parent binary search + mapping
 arrays, one core) on the largest J1-J2 instance that algorithm can build,
 the 4x6 lattice at k = (pi, 0).
 
+``grad`` times the backward pass on the 4x6 random-bond XXZ model.
+
+``time_to_E0`` times the FP32-mixed and the FP64 Lanczos to E0 on the
+workload's operator.
+
 Options: --steps K --warmup W --gpus 1 [--config 4x6|6x6] [--no-cpu]
-[--e0] (also times the FP32-mixed and FP64 Lanczos to E0).
+[--no-grad] [--no-e0].
 """
 from __future__ import annotations
 
@@ -154,6 +159,64 @@ def lanczos_times(ed, opr):
     return out
 
 
+def grad_times(ed):
+    """Backward pass on the 4x6 random-bond XXZ model (Sz = 0, 2,704,156
+    states, 48 bonds, theta = (J_b, Delta_b), J_b ~ U(0.5,1.5), Delta_b ~
+    U(0.8,1.2) from seed 2024): the ground state by FP64 Lanczos, the 96
+    Hellmann-Feynman derivatives from one fused term pass, and
+    d<Sz_0 Sz_12>/dtheta through the MINRES adjoint.  The fused pass is also
+    expressed in matvec equivalents (its time over one FP64 matvec of H)."""
+    import torch
+    from edhip.grad import correlator_and_grad, ground_state, xxz_model
+
+    Lx, Ly = 4, 6
+    n = Lx * Ly
+    hs, s = ed.spin_half_system(n)
+    j1, _ = ed.square_bonds(Lx, Ly)
+    hsr = ed.represent(ed.HilbertSpaceSector(hs, 0))
+    model, fn = xxz_model(hs, hsr, j1)
+    rng = np.random.default_rng(2024)
+    B = len(j1)
+    theta = np.concatenate([rng.uniform(0.5, 1.5, B), rng.uniform(0.8, 1.2, B)])
+    c, J = fn(theta)
+    t0 = time.perf_counter()
+    H, r = ground_state(model, c, tol=1e-12)
+    t_gs = time.perf_counter() - t0
+    psi = r.vectors[0]
+    model.expectations(psi, psi)  # warm-up (builds the structure operator's tables)
+    reps = 20
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ex = model.expectations(psi, psi)
+    t_bw = (time.perf_counter() - t0) / reps
+    g = J.T @ ex.real
+    y = torch.empty_like(psi)
+    ed.mul_(y, H, psi)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ed.mul_(y, H, psi)
+    t_mv = (time.perf_counter() - t0) / reps
+    O = ed.simplify(0.25 * s(0, "z") * s(12, "z"))
+    t0 = time.perf_counter()
+    val, gc, info = correlator_and_grad(model, c, J, O, tol=1e-12, minres_tol=1e-10)
+    t_corr = time.perf_counter() - t0
+    return {
+        "workload": "random-bond XXZ 4x6 torus, Sz=0, 2,704,156 states, 96 parameters (seed 2024)",
+        "E0": float(r.values[0]),
+        "ground_state_lanczos_s": t_gs,
+        "lanczos_matvecs": int(r.matvecs),
+        "energy_backward_ms": t_bw * 1e3,
+        "fp64_matvec_ms": t_mv * 1e3,
+        "energy_backward_matvec_equivalents": t_bw / t_mv,
+        "homogeneity_residual": float(abs(np.dot(theta[:B], g[:B]) - r.values[0])),
+        "correlator": "Sz_0 Sz_12",
+        "correlator_value": float(val),
+        "correlator_grad_total_s": t_corr,
+        "minres_iterations": int(info["minres_iterations"]),
+        "minres_residual": float(info["minres_residual"]),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -161,7 +224,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", default="6x6", choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--e0", action="store_true")
+    ap.add_argument("--no-e0", action="store_true", help="skip the time-to-E0 runs (about 40 s)")
+    ap.add_argument("--no-grad", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run one plain 1 GiB device copy (elementwise kernel) for HBM-counter calibration and exit")
     args = ap.parse_args()
@@ -243,8 +307,10 @@ def main():
         "checksum_f32": cs32,
         "checksum_f64": cs64,
     }
-    if args.e0:
+    if not args.no_e0:
         line["time_to_E0"] = lanczos_times(ed, opr)
+    if not args.no_grad:
+        line["grad"] = grad_times(ed)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(ed)
     print(json.dumps(line), flush=True)

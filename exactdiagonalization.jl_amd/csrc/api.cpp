@@ -173,7 +173,7 @@ struct GroupHost {
   std::vector<int32_t> inv;
   bool complex_phase = false;
   // device tables
-  DevBuf nib, minrot, rot_hi, rot_lo, g_of_ab, inv_d, trivial, ph_re, ph_im, invsqrt, sqcnt;
+  DevBuf nib, minrot, pair, rot_hi, rot_lo, g_of_ab, inv_d, trivial, ph_re, ph_im, invsqrt, sqcnt;
   GroupDev dev{};
   GroupLdsLayout lay{};
 };
@@ -524,6 +524,31 @@ void build_group(GroupHost& gh, int nb, double tol, hipStream_t st) {
       L.bytes_minrot = (uint32_t)(nv * 4);
       place(L.off_minrot, L.bytes_minrot);
     }
+    d.pair_table = (bestLx <= 6 && bestLy >= 2 && bestLy <= 8) ? 1 : 0;
+    if (d.pair_table) {
+      const int Lx = bestLx;
+      const uint32_t rm = (uint32_t)d.row_mask;
+      auto rot = [&](uint32_t v, int a) { return a == 0 ? v : (((v << a) | (v >> (Lx - a))) & rm); };
+      const int np = 1 << (2 * Lx);
+      std::vector<uint32_t> pt(np);
+      for (int v = 0; v < np; ++v) {
+        const uint32_t hi = (uint32_t)v >> Lx, lo = (uint32_t)v & rm;
+        uint32_t best = 0xffffffffu, am = 0;
+        for (int a = 0; a < Lx; ++a) {
+          const uint32_t val = (rot(hi, a) << Lx) | rot(lo, a);
+          if (val < best) {
+            best = val;
+            am = 1u << a;
+          } else if (val == best) {
+            am |= 1u << a;
+          }
+        }
+        pt[v] = (best << 8) | am;
+      }
+      upload(gh.pair, pt, st);
+      d.pair = gh.pair.as<uint32_t>();
+      L.bytes_pair = (uint32_t)(np * 4);
+    }
   } else {
     int n_nib = (nb + 3) / 4;
     d.n_nib = n_nib;
@@ -543,6 +568,9 @@ void build_group(GroupHost& gh, int nb, double tol, hipStream_t st) {
     L.bytes_nib = (uint32_t)(tab.size() * 8);
     place(L.off_nib, L.bytes_nib);
   }
+  // the row-pair table goes last, so that an operator whose terms leave no
+  // room for it can drop it (make_opdev) without moving the other tables
+  if (L.bytes_pair) place(L.off_pair, L.bytes_pair);
   L.total = align_up(off, 16);
 }
 
@@ -581,6 +609,9 @@ struct edhip_op {
   bool pairs_ok = false;
   DevBuf pg_meta[2], pg_dre[2], pg_dim[2], pg_ore[2], pg_oim[2];
   PairGroupsDev pg[2]{};
+  // per-row diagonal elements for the term-list kernel (built on first use)
+  bool diag_tried = false;
+  DevBuf diag_re, diag_im;
   int strat_kernel = 0;  // 0 auto, 1 term list, 2 pair groups
   int strat_lookup = 0;  // 0 auto, 1 sorted search (or rank), 2 hash table
   int used_kernel = 0, used_lookup = 0;
@@ -614,6 +645,14 @@ OpDev make_opdev(edhip_op* op, bool transpose) {
   d.n_diag = op->n_diag;
   d.off = transpose ? op->off_col.as<OffTerm>() : op->off_row.as<OffTerm>();
   d.diag = op->diag.as<DiagTerm>();
+  const uint32_t op_bytes = align_up((uint32_t)(op->n_off * sizeof(OffTerm)), 16) +
+                            (d.ix.rank_mode ? (uint32_t)((sp->n_bits + 1) * (sp->n_bits + 1) * 8) : 0u);
+  if (d.g.pair_table && align_up(d.lay.total, 16) + op_bytes > 64 * 1024) {
+    // not enough LDS next to this operator's terms: use the per-row table path
+    d.g.pair_table = 0;
+    d.lay.bytes_pair = 0;
+    d.lay.total = d.lay.off_pair;
+  }
   uint32_t off = align_up(d.lay.total, 16);
   d.off_lds = off;
   off += (uint32_t)(op->n_off * sizeof(OffTerm));
@@ -735,6 +774,39 @@ FastOpDev make_fast(edhip_op* op, int transpose, const HashDev* hash) {
 }
 
 // y = (accumulate ? y : 0) + A x (or x A), choosing the kernel and lookup.
+// Per-row diagonal elements, computed once per operator when the operator
+// has more than one diagonal term and the array (8 or 16 bytes per row) stays
+// under 16 GiB; the matvec then reads one number per row instead of looping
+// over the diagonal terms.  Returns false when not cached.
+bool ensure_diag(edhip_op* op) {
+  if (op->diag_tried) return op->diag_re.p != nullptr;
+  op->diag_tried = true;
+  const int64_t D = op->space->dim;
+  if (op->n_diag < 2 || D == 0) return false;
+  bool cx = false;
+  for (int64_t t : op->diag_ids) cx = cx || op->aim[t] != 0.0;
+  const uint64_t bytes = (uint64_t)D * 8ull * (cx ? 2ull : 1ull);
+  if (bytes > (16ull << 30)) return false;
+  void* pr = nullptr;
+  void* pi = nullptr;
+  if (hipMalloc(&pr, (size_t)D * 8) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  if (cx && hipMalloc(&pi, (size_t)D * 8) != hipSuccess) {
+    (void)hipGetLastError();
+    (void)hipFree(pr);
+    return false;
+  }
+  op->diag_re.p = pr;
+  op->diag_re.bytes = (size_t)D * 8;
+  op->diag_im.p = pi;
+  op->diag_im.bytes = pi ? (size_t)D * 8 : 0;
+  const OpDev& d = op->dev_row;
+  HIP_OR_THROW(launch_diag_values(d, op->diag_re.as<double>(), op->diag_im.as<double>(), op->ctx->stream));
+  return true;
+}
+
 void op_matvec(edhip_op* op, int prec, int transpose, int accumulate, const void* x, void* y) {
   hipStream_t st = op->ctx->stream;
   const edhip_space* sp = op->space;
@@ -749,6 +821,10 @@ void op_matvec(edhip_op* op, int prec, int transpose, int accumulate, const void
   if (!fast) {
     OpDev d = transpose ? op->dev_col : op->dev_row;
     d.ix = sp->index_dev(idx_lookup);
+    if (ensure_diag(op)) {
+      d.diag_re = op->diag_re.as<double>();
+      d.diag_im = op->diag_im.as<double>();
+    }
     op->used_kernel = 1;
     op->used_lookup = rank_mode ? 1 : (d.ix.bm_bits ? 3 : 1);
     HIP_OR_THROW(launch_matvec(d, prec, transpose ? 1 : 0, accumulate ? 1 : 0, x, y, st));

@@ -189,6 +189,8 @@ __device__ __forceinline__ int64_t index_of(const IndexDev& ix, const uint64_t* 
 // Symmetry group description on the device.
 // ----------------------------------------------------------------------------
 enum GroupMode : int { kGroupNone = 0, kGroupGeneric = 1, kGroupTorus = 2 };
+// kernel specialisation of kGroupTorus for groups with the row-pair table
+constexpr int kGroupTorusPairs = 3;
 
 constexpr int kMaxLy = 64;
 
@@ -198,11 +200,14 @@ struct GroupDev {
   int n_nib;  // generic: number of 4-bit chunks of the word
   int Lx, Ly, nb;
   int minrot_table;     // torus: 1 if the per-row minimal-rotation table is used
-  int pad_;
+  int pair_table;       // torus: 1 if the adjacent-row-pair table is used (Lx <= 6, 2 <= Ly <= 8)
   uint64_t row_mask;    // torus: (1 << Lx) - 1
   uint64_t full_mask;   // (1 << nb) - 1
   const uint64_t* nib;  // generic: G * n_nib * 16 images of 4-bit chunks
   const uint32_t* minrot;  // torus: 2^Lx entries, low 16: min rotation value, high 16: mask of rotations
+  // torus: 2^(2 Lx) entries indexed by (hi << Lx) | lo for two adjacent rows
+  // (hi above lo): min_a (rot_a(hi) << Lx | rot_a(lo)) << 8 | mask of the a reaching it
+  const uint32_t* pair;
   const uint64_t* rot_hi;  // torus: Lx entries, per-row bits kept by a left rotation by a
   const uint64_t* rot_lo;  // torus: Lx entries, per-row bits wrapped around
   const int32_t* g_of_ab;  // torus: Lx*Ly -> group index of T(a, b)
@@ -222,6 +227,7 @@ struct GroupLds {
   const double* sqcnt;
   const uint64_t* nib;
   const uint32_t* minrot;
+  const uint32_t* pair;
   const uint64_t* rot_hi;
   const uint64_t* rot_lo;
   const int32_t* g_of_ab;
@@ -267,6 +273,61 @@ __device__ __forceinline__ void row_minrot(const GroupDev& g, const GroupLds& L,
   }
 }
 
+// Torus canonicaliser over the adjacent-row-pair table (g.pair_table set).
+// The smallest image has the smallest (top row, second row) pair, so only the
+// (row y, rotation a) whose pair (rot_a(row_y), rot_a(row_{y-1})) reaches
+// that minimum can give it; they are collected in one mask (8 bits per row:
+// bit 8y + a) and walked in a single loop, so lanes of a wave stay together
+// unless one of them has several such candidates.  Same s, h and cnt as the
+// generic torus path below.
+__device__ __forceinline__ void canonicalize_torus_pairs(const GroupDev& g, const GroupLds& L, uint64_t b, uint64_t& s,
+                                                         int& h, int& cnt) {
+  const int Lx = g.Lx, Ly = g.Ly;
+  const uint32_t m2 = (1u << (2 * Lx)) - 1u;
+  // row Ly-1 copied below row 0: the window at y*Lx holds (row_y << Lx) | row_{y-1}
+  const uint64_t w = (b << Lx) | (b >> (g.nb - Lx));
+  uint32_t e[8];
+  uint32_t best = 0xffffffffu;
+#pragma unroll
+  for (int y = 0; y < 8; ++y) {
+    e[y] = 0xffffffffu;
+    if (y < Ly) {
+      e[y] = L.pair[(uint32_t)(w >> (y * Lx)) & m2];
+      best = min(best, e[y] | 0xffu);
+    }
+  }
+  uint32_t cand_lo = 0, cand_hi = 0;
+#pragma unroll
+  for (int y = 0; y < 4; ++y)
+    cand_lo |= ((e[y] | 0xffu) == best ? (e[y] & 0xffu) : 0u) << (8 * y);
+#pragma unroll
+  for (int y = 4; y < 8; ++y)
+    cand_hi |= ((e[y] | 0xffu) == best ? (e[y] & 0xffu) : 0u) << (8 * (y - 4));
+  uint64_t cand = ((uint64_t)cand_hi << 32) | cand_lo;
+  s = ~0ull;
+  cnt = 0;
+  int win = 0;  // bit 8y + a of the candidate giving s
+  while (cand) {
+    const int bit = __builtin_ctzll(cand);
+    cand &= cand - 1;
+    const int a = bit & 7;
+    const int sh = (Ly - 1 - (bit >> 3)) * Lx;  // row y moves to the top
+    const uint64_t t = ((b << sh) | (b >> (g.nb - sh))) & g.full_mask;
+    const uint64_t img = ((t << a) & L.rot_hi[a]) | ((t >> (Lx - a)) & L.rot_lo[a]);
+    if (img < s) {
+      s = img;
+      win = bit;
+      cnt = 1;
+    } else if (img == s) {
+      ++cnt;
+      const int e_new = L.g_of_ab[a + Lx * (Ly - 1 - (bit >> 3))];
+      const int e_cur = L.g_of_ab[(win & 7) + Lx * (Ly - 1 - (win >> 3))];
+      if (L.inv[e_new] > L.inv[e_cur]) win = bit;
+    }
+  }
+  h = L.g_of_ab[(win & 7) + Lx * (Ly - 1 - (win >> 3))];
+}
+
 // Canonical form s = min_h h(b), an element h with h(b) = s (among ties, the
 // one whose inverse has the largest index: the reference's last writer of the
 // orbit dictionary), and cnt = #{h : h(b) = s} = |Stab(s)|.
@@ -291,6 +352,10 @@ __device__ __forceinline__ void canonicalize(const GroupDev& g, const GroupLds& 
     return;
   }
   if (g.mode == kGroupTorus) {
+    if (g.pair_table) {
+      canonicalize_torus_pairs(g, L, b, s, h, cnt);
+      return;
+    }
     // pass 1: the smallest possible top row
     uint64_t best_top = ~0ull;
     for (int y = 0; y < g.Ly; ++y) {
@@ -386,9 +451,9 @@ __device__ __forceinline__ bool is_representative(const GroupDev& g, const Group
 // LDS staging of the group tables.  The host computes the byte offsets.
 // ----------------------------------------------------------------------------
 struct GroupLdsLayout {
-  uint32_t off_nib, off_minrot, off_rot_hi, off_rot_lo, off_g_of_ab, off_inv, off_trivial;
+  uint32_t off_nib, off_minrot, off_pair, off_rot_hi, off_rot_lo, off_g_of_ab, off_inv, off_trivial;
   uint32_t off_ph_re, off_ph_im, off_invsqrt, off_sqcnt;
-  uint32_t bytes_nib, bytes_minrot, bytes_rot, bytes_g_of_ab, bytes_inv, bytes_trivial;
+  uint32_t bytes_nib, bytes_minrot, bytes_pair, bytes_rot, bytes_g_of_ab, bytes_inv, bytes_trivial;
   uint32_t bytes_ph, bytes_invsqrt;
   uint32_t total;
 };
@@ -405,6 +470,7 @@ __device__ __forceinline__ GroupLds stage_group(const GroupDev& g, const GroupLd
   GroupLds L;
   lds_copy(lds + lay.off_nib, g.nib, lay.bytes_nib);
   lds_copy(lds + lay.off_minrot, g.minrot, lay.bytes_minrot);
+  lds_copy(lds + lay.off_pair, g.pair, lay.bytes_pair);
   lds_copy(lds + lay.off_rot_hi, g.rot_hi, lay.bytes_rot);
   lds_copy(lds + lay.off_rot_lo, g.rot_lo, lay.bytes_rot);
   lds_copy(lds + lay.off_g_of_ab, g.g_of_ab, lay.bytes_g_of_ab);
@@ -420,6 +486,7 @@ __device__ __forceinline__ GroupLds stage_group(const GroupDev& g, const GroupLd
   L.invsqrt = (const double*)(lds + lay.off_invsqrt);
   L.nib = (const uint64_t*)(lds + lay.off_nib);
   L.minrot = (const uint32_t*)(lds + lay.off_minrot);
+  L.pair = (const uint32_t*)(lds + lay.off_pair);
   L.rot_hi = (const uint64_t*)(lds + lay.off_rot_hi);
   L.rot_lo = (const uint64_t*)(lds + lay.off_rot_lo);
   L.g_of_ab = (const int32_t*)(lds + lay.off_g_of_ab);

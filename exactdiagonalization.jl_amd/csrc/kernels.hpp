@@ -83,6 +83,8 @@ struct OpDev {
   int n_diag;
   const OffTerm* off;    // row form: match = R, image = C ; column form: match = C, image = R
   const DiagTerm* diag;
+  const double* diag_re; // per-row diagonal element (summed diagonal terms), or nullptr
+  const double* diag_im; // its imaginary part, or nullptr when every diagonal amplitude is real
   uint32_t off_lds;      // LDS offset of the staged off-diagonal terms
   uint32_t w_lds;        // LDS offset of the staged rank table (rank mode)
   uint32_t w_bytes;
@@ -94,6 +96,10 @@ enum Prec : int { kF64 = 0, kC128 = 1, kF32 = 2, kC64 = 3 };
 // y = (accumulate ? y : 0) + A x   (transpose=0: row form; 1: column form, y = x A)
 hipError_t launch_matvec(const OpDev& op, int prec, int transpose, int accumulate, const void* x, void* y,
                          hipStream_t st);
+
+// Diagonal element of every row: re[r] (+ i im[r] when im != nullptr) = sum of
+// the amplitudes of the diagonal terms that match row r.
+hipError_t launch_diag_values(const OpDev& op, double* d_re, double* d_im, hipStream_t st);
 
 // Total number of off-diagonal images the row form generates (atomic u64 add).
 hipError_t launch_count_images(const OpDev& op, unsigned long long* d_total, hipStream_t st);

@@ -26,6 +26,8 @@ struct Canon {
       s = b;
       h = 0;
       cnt = 1;
+    } else if constexpr (MODE == kGroupTorusPairs) {
+      canonicalize_torus_pairs(g, L, b, s, h, cnt);
     } else {
       canonicalize(g, L, b, s, h, cnt);
     }
@@ -67,6 +69,25 @@ __device__ __forceinline__ void element_weight(const OpDev& op, const Staged& S,
   }
 }
 
+__device__ __forceinline__ void diag_sum(const OpDev& op, uint64_t b, double& dre, double& dim_) {
+  for (int t = 0; t < op.n_diag; ++t) {
+    const DiagTerm d = op.diag[t];
+    if ((b & d.mask) == d.match) {
+      dre += d.are;
+      dim_ += d.aim;
+    }
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_diag_values(OpDev op, double* __restrict__ re, double* __restrict__ im) {
+  const int64_t row = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (row >= op.dim) return;
+  double dre = 0.0, dim_ = 0.0;
+  diag_sum(op, op.rows[row], dre, dim_);
+  re[row] = dre;
+  if (im) im[row] = dim_;
+}
+
 template <typename V, int MODE, bool TRANS>
 __global__ __launch_bounds__(kBlock) void k_matvec(OpDev op, const V* __restrict__ x, V* __restrict__ y, int accumulate) {
   extern __shared__ unsigned char lds[];
@@ -79,12 +100,11 @@ __global__ __launch_bounds__(kBlock) void k_matvec(OpDev op, const V* __restrict
 
   // diagonal terms (C == R): the element is the summed amplitude (U[b]/U[b] = 1)
   double dre = 0.0, dim_ = 0.0;
-  for (int t = 0; t < op.n_diag; ++t) {
-    const DiagTerm d = op.diag[t];
-    if ((b & d.mask) == d.match) {
-      dre += d.are;
-      dim_ += d.aim;
-    }
+  if (op.diag_re) {
+    dre = op.diag_re[row];
+    if (op.diag_im) dim_ = op.diag_im[row];
+  } else {
+    diag_sum(op, b, dre, dim_);
   }
   V acc = vzero<V>();
   vfma(acc, (R)dre, (R)(TRANS ? dim_ : dim_), x[row]);
@@ -138,6 +158,7 @@ hipError_t launch_mv_v(const OpDev& op, int transpose, int accumulate, const voi
     case kGroupGeneric:
       return launch_mv_t<V, kGroupGeneric>(op, transpose, accumulate, x, y, st);
     default:
+      if (op.g.pair_table) return launch_mv_t<V, kGroupTorusPairs>(op, transpose, accumulate, x, y, st);
       return launch_mv_t<V, kGroupTorus>(op, transpose, accumulate, x, y, st);
   }
 }
@@ -444,6 +465,13 @@ hipError_t launch_count_images(const OpDev& op, unsigned long long* d_total, hip
   return hipGetLastError();
 }
 
+hipError_t launch_diag_values(const OpDev& op, double* d_re, double* d_im, hipStream_t st) {
+  if (op.dim == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_diag_values, dim3((unsigned)((op.dim + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, op, d_re,
+                     d_im);
+  return hipGetLastError();
+}
+
 hipError_t launch_matvec(const OpDev& op, int prec, int transpose, int accumulate, const void* x, void* y,
                          hipStream_t st) {
   if (op.dim == 0) return hipSuccess;
@@ -640,7 +668,10 @@ __global__ __launch_bounds__(kBlock) void k_mv_pairs(FastOpDev F, const V* __res
     uint64_t s;
     int h, cnt;
     if constexpr (MODE == kGroupTorus) {
-      canonicalize_torus_reg<kLyMax>(op.g, L, img, s, h, cnt);
+      if (op.g.pair_table)
+        canonicalize_torus_pairs(op.g, L, img, s, h, cnt);
+      else
+        canonicalize_torus_reg<kLyMax>(op.g, L, img, s, h, cnt);
     } else {
       s = img;
       h = 0;
