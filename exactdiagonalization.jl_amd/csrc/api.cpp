@@ -260,9 +260,9 @@ struct edhip_space {
     index_built = true;
   }
 
-  // Bitmap rank index over [0, 2^bm_bits): built when the largest word has
-  // at most 36 bits and the bitmap is at most 16x the size of the key array.
-  mutable DevBuf bitmap, bm_counts;
+  // Blocked rank index over [0, 2^bm_bits): built when the largest word has
+  // at most 36 bits and the index is at most 24x the size of the key array.
+  mutable DevBuf bitmap;
   mutable int bm_bits = 0;
   mutable bool bm_tried = false;
 
@@ -276,19 +276,20 @@ struct edhip_space {
     ctx->sync();
     const int K = std::max(9, bit_length(max_key));
     if (K > 36) return false;
-    const uint64_t bytes = (1ull << K) / 8;
-    if (bytes > std::max<uint64_t>(16ull * (uint64_t)dim * 8ull, 1ull << 20)) return false;
-    const uint64_t n_blocks = (1ull << K) / 512;
+    const uint64_t n_blocks = ((1ull << K) + kRankBlockWords - 1) / kRankBlockWords;
+    const uint64_t bytes = n_blocks * 64;
+    if (bytes > std::max<uint64_t>(24ull * (uint64_t)dim * 8ull, 1ull << 20)) return false;
     bitmap.alloc(bytes);
-    bm_counts.alloc(n_blocks * 4);
     HIP_OR_THROW(hipMemsetAsync(bitmap.p, 0, bytes, ctx->stream));
+    DevBuf totals, base, tmp;
+    totals.alloc(n_blocks * 4);
+    base.alloc(n_blocks * 8);
     size_t tb = 0;
-    HIP_OR_THROW(launch_bitmap_build(keys.as<uint64_t>(), dim, bitmap.as<uint64_t>(), n_blocks, bm_counts.as<uint32_t>(),
-                                     nullptr, tb, ctx->stream));
-    DevBuf tmp;
+    HIP_OR_THROW(launch_bitmap_build(keys.as<uint64_t>(), dim, bitmap.as<uint64_t>(), n_blocks, totals.as<uint32_t>(),
+                                     base.as<uint64_t>(), nullptr, tb, ctx->stream));
     tmp.alloc(tb);
-    HIP_OR_THROW(launch_bitmap_build(keys.as<uint64_t>(), dim, bitmap.as<uint64_t>(), n_blocks, bm_counts.as<uint32_t>(),
-                                     tmp.p, tb, ctx->stream));
+    HIP_OR_THROW(launch_bitmap_build(keys.as<uint64_t>(), dim, bitmap.as<uint64_t>(), n_blocks, totals.as<uint32_t>(),
+                                     base.as<uint64_t>(), tmp.p, tb, ctx->stream));
     ctx->sync();
     bm_bits = K;
     return true;
@@ -317,7 +318,7 @@ struct edhip_space {
     if (lookup == 0 || lookup == 3) {
       if (ensure_bitmap()) {
         ix.bitmap = bitmap.as<uint64_t>();
-        ix.bm_counts = bm_counts.as<uint32_t>();
+        ix.bm_counts = nullptr;
         ix.bm_bits = bm_bits;
       } else if (lookup == 3) {
         fail(EDHIP_ERR_ARGUMENT, "no bitmap index for this space (words wider than 36 bits or too sparse)");

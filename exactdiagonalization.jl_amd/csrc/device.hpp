@@ -130,33 +130,34 @@ struct IndexDev {
   int pad_;
   uint64_t pop_allowed;  // bit p set iff popcount p is allowed (rank mode)
   const uint64_t* W;     // completion counts (rank mode)
-  // bitmap rank index (bm_bits > 0): bit s of the bitmap is set iff word s is
-  // in the basis; bm_counts[j] = number of basis words below 512*j
+  // blocked rank index (bm_bits > 0) over the words [0, 2^bm_bits): block j
+  // (64 bytes) covers words [384 j, 384 j + 384): u64 0..5 hold their
+  // membership bits, u64 6 the number of basis words below 384 j, u64 7 the
+  // bit counts of u64 0..w-1 for w = 1..5 (9 bits each).  One cache line per
+  // lookup.
   const uint64_t* bitmap;
-  const uint32_t* bm_counts;
+  const uint32_t* bm_counts;  // unused
   int bm_bits;
   int pad2_;
 };
 
-// Index of key through the bitmap: one 64-byte block of the bitmap plus one
-// block count (membership and rank together).
+constexpr uint32_t kRankBlockWords = 384;  // membership bits per 64-byte block
+
+// Index of key through the blocked rank index (membership and rank from one
+// 64-byte block).
 __device__ __forceinline__ int64_t bitmap_index(const IndexDev& ix, uint64_t key) {
   if ((key >> ix.bm_bits) != 0) return -1;
-  const uint64_t* blk = ix.bitmap + ((key >> 9) << 3);
-  const int w = (int)((key >> 6) & 7u);
-  uint64_t words[8];
-#pragma unroll
-  for (int i = 0; i < 8; ++i) words[i] = blk[i];
-  uint64_t word = 0;
-  int64_t r = ix.bm_counts[key >> 9];
-#pragma unroll
-  for (int i = 0; i < 8; ++i) {
-    r += (i < w) ? __builtin_popcountll(words[i]) : 0;
-    word = (i == w) ? words[i] : word;
-  }
-  const int bit = (int)(key & 63u);
+  const uint32_t q = (uint32_t)(key >> 7);         // < 2^29
+  const uint32_t j = __umulhi(q, 0xAAAAAAABu) >> 1;  // q / 3 = key / 384
+  const uint32_t within = (uint32_t)(key - (uint64_t)j * kRankBlockWords);
+  const uint32_t w = within >> 6, bit = within & 63u;
+  const uint64_t* blk = ix.bitmap + (uint64_t)j * 8u;
+  const uint64_t word = blk[w];
   if (!((word >> bit) & 1ull)) return -1;
-  return r + __builtin_popcountll(word & ((1ull << bit) - 1ull));
+  const uint64_t base = blk[6];
+  const uint64_t packed = blk[7];
+  const uint32_t sub = w ? (uint32_t)(packed >> (9u * (w - 1u))) & 511u : 0u;
+  return (int64_t)(base + sub + (uint64_t)__builtin_popcountll(word & ((1ull << bit) - 1ull)));
 }
 
 __device__ __forceinline__ int64_t search_sorted(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ prefix,

@@ -50,10 +50,12 @@ hipError_t launch_vec_reduce(const uint64_t* d_reps, const uint16_t* d_orbit, in
                              const GroupLdsLayout& lay, const IndexDev& parent_index, int prec, const void* d_large,
                              void* d_small, hipStream_t st);
 
-// Bitmap over the words [0, 2^bits) with per-512-bit-block exclusive counts
-// (two-phase: call with d_tmp == nullptr to size the scan's scratch).
-hipError_t launch_bitmap_build(const uint64_t* d_keys, int64_t dim, uint64_t* d_bitmap, uint64_t n_blocks,
-                               uint32_t* d_counts, void* d_tmp, size_t& tmp_bytes, hipStream_t st);
+// Blocked rank index over the words [0, 384 n_blocks) (zeroed blocks in,
+// layout as in IndexDev); d_totals (u32) and d_base (u64) hold n_blocks
+// entries of scratch.  Two-phase: call with d_tmp == nullptr to size the
+// scan's scratch.
+hipError_t launch_bitmap_build(const uint64_t* d_keys, int64_t dim, uint64_t* d_blocks, uint64_t n_blocks,
+                               uint32_t* d_totals, uint64_t* d_base, void* d_tmp, size_t& tmp_bytes, hipStream_t st);
 hipError_t exclusive_scan_u32_to_u64(const uint32_t* d_in, uint64_t* d_out, uint64_t n, void* d_tmp, size_t& tmp_bytes,
                                      hipStream_t st);
 hipError_t exclusive_scan_i64(const int64_t* d_in, int64_t* d_out, uint64_t n, void* d_tmp, size_t& tmp_bytes,

@@ -215,20 +215,36 @@ __global__ __launch_bounds__(kBlock) void k_lookup(IndexDev ix, int64_t n, const
 }
 
 __global__ __launch_bounds__(kBlock) void k_bitmap_set(const uint64_t* __restrict__ keys, int64_t dim,
-                                                       unsigned long long* __restrict__ bitmap) {
+                                                       unsigned long long* __restrict__ blocks) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= dim) return;
   const uint64_t k = keys[i];
-  atomicOr(bitmap + (k >> 6), 1ull << (k & 63u));
+  const uint64_t j = k / kRankBlockWords;
+  const uint32_t within = (uint32_t)(k - j * kRankBlockWords);
+  atomicOr(blocks + j * 8 + (within >> 6), 1ull << (within & 63u));
 }
 
-__global__ __launch_bounds__(kBlock) void k_bitmap_block_counts(const uint64_t* __restrict__ bitmap, uint64_t n_blocks,
-                                                                uint32_t* __restrict__ counts) {
+// per block: total members (for the scan) and the packed counts of u64 0..w-1
+__global__ __launch_bounds__(kBlock) void k_bitmap_block_counts(uint64_t* __restrict__ blocks, uint64_t n_blocks,
+                                                                uint32_t* __restrict__ totals) {
   uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n_blocks) return;
+  uint64_t* blk = blocks + j * 8;
   uint32_t c = 0;
-  for (int i = 0; i < 8; ++i) c += (uint32_t)__builtin_popcountll(bitmap[j * 8 + i]);
-  counts[j] = c;
+  uint64_t packed = 0;
+  for (int w = 0; w < 6; ++w) {
+    if (w > 0) packed |= (uint64_t)c << (9 * (w - 1));
+    c += (uint32_t)__builtin_popcountll(blk[w]);
+  }
+  blk[7] = packed;
+  totals[j] = c;
+}
+
+__global__ __launch_bounds__(kBlock) void k_bitmap_set_base(uint64_t* __restrict__ blocks, uint64_t n_blocks,
+                                                            const uint64_t* __restrict__ base) {
+  uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n_blocks) return;
+  blocks[j * 8 + 6] = base[j];
 }
 
 __global__ void k_u32_to_u64(const uint32_t* __restrict__ in, uint64_t* __restrict__ out, uint64_t n) {
@@ -293,21 +309,24 @@ hipError_t launch_lookup(const IndexDev& ix, int64_t n, const uint64_t* d_keys, 
   return hipGetLastError();
 }
 
-hipError_t launch_bitmap_build(const uint64_t* d_keys, int64_t dim, uint64_t* d_bitmap, uint64_t n_blocks,
-                               uint32_t* d_counts, void* d_tmp, size_t& tmp_bytes, hipStream_t st) {
+hipError_t launch_bitmap_build(const uint64_t* d_keys, int64_t dim, uint64_t* d_blocks, uint64_t n_blocks,
+                               uint32_t* d_totals, uint64_t* d_base, void* d_tmp, size_t& tmp_bytes, hipStream_t st) {
   if (d_tmp == nullptr) {
-    return hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, d_counts, d_counts, (int64_t)n_blocks, st);
+    return hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, d_totals, d_base, (int64_t)n_blocks, st);
   }
   if (dim > 0) {
     hipLaunchKernelGGL(k_bitmap_set, dim3(grid_for((uint64_t)dim)), dim3(kBlock), 0, st, d_keys, dim,
-                       (unsigned long long*)d_bitmap);
+                       (unsigned long long*)d_blocks);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
-  hipLaunchKernelGGL(k_bitmap_block_counts, dim3(grid_for(n_blocks)), dim3(kBlock), 0, st, d_bitmap, n_blocks, d_counts);
+  hipLaunchKernelGGL(k_bitmap_block_counts, dim3(grid_for(n_blocks)), dim3(kBlock), 0, st, d_blocks, n_blocks, d_totals);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  return hipcub::DeviceScan::ExclusiveSum(d_tmp, tmp_bytes, d_counts, d_counts, (int64_t)n_blocks, st);
+  e = hipcub::DeviceScan::ExclusiveSum(d_tmp, tmp_bytes, d_totals, d_base, (int64_t)n_blocks, st);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_bitmap_set_base, dim3(grid_for(n_blocks)), dim3(kBlock), 0, st, d_blocks, n_blocks, d_base);
+  return hipGetLastError();
 }
 
 hipError_t exclusive_scan_u32_to_u64(const uint32_t* d_in, uint64_t* d_out, uint64_t n, void* d_tmp, size_t& tmp_bytes,
