@@ -315,16 +315,15 @@ __device__ __forceinline__ void canonicalize_torus_pairs(const GroupDev& g, cons
     const int sh = (Ly - 1 - (bit >> 3)) * Lx;  // row y moves to the top
     const uint64_t t = ((b << sh) | (b >> (g.nb - sh))) & g.full_mask;
     const uint64_t img = ((t << a) & L.rot_hi[a]) | ((t >> (Lx - a)) & L.rot_lo[a]);
-    if (img < s) {
-      s = img;
-      win = bit;
-      cnt = 1;
-    } else if (img == s) {
-      ++cnt;
+    const bool lt = img < s, eq = img == s;
+    if (eq) {  // a tie (s has a nontrivial stabiliser): keep the largest inverse index
       const int e_new = L.g_of_ab[a + Lx * (Ly - 1 - (bit >> 3))];
       const int e_cur = L.g_of_ab[(win & 7) + Lx * (Ly - 1 - (win >> 3))];
       if (L.inv[e_new] > L.inv[e_cur]) win = bit;
     }
+    cnt = lt ? 1 : cnt + (eq ? 1 : 0);
+    win = lt ? bit : win;
+    s = lt ? img : s;
   }
   h = L.g_of_ab[(win & 7) + Lx * (Ly - 1 - (win >> 3))];
 }

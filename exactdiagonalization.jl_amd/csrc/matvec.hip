@@ -88,12 +88,17 @@ __global__ __launch_bounds__(kBlock) void k_diag_values(OpDev op, double* __rest
   if (im) im[row] = dim_;
 }
 
+// The matvec runs 512-lane blocks, one row per lane, over a grid that covers
+// the rows in order (blocks that run together work on neighbouring rows, whose
+// images share cache lines of the index and of x).
+constexpr int kMvBlock = 512;
+
 template <typename V, int MODE, bool TRANS>
-__global__ __launch_bounds__(kBlock) void k_matvec(OpDev op, const V* __restrict__ x, V* __restrict__ y, int accumulate) {
+__global__ __launch_bounds__(kMvBlock) void k_matvec(OpDev op, const V* __restrict__ x, V* __restrict__ y, int accumulate) {
   extern __shared__ unsigned char lds[];
   Staged S = stage_op<MODE>(op, lds);
   __syncthreads();
-  const int64_t row = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const int64_t row = (int64_t)blockIdx.x * kMvBlock + threadIdx.x;
   if (row >= op.dim) return;
   using R = typename VT<V>::R;
   const uint64_t b = op.rows[row];
@@ -107,7 +112,7 @@ __global__ __launch_bounds__(kBlock) void k_matvec(OpDev op, const V* __restrict
     diag_sum(op, b, dre, dim_);
   }
   V acc = vzero<V>();
-  vfma(acc, (R)dre, (R)(TRANS ? dim_ : dim_), x[row]);
+  vfma(acc, (R)dre, (R)dim_, x[row]);
 
   double sqrt_own = 1.0;
   if constexpr (MODE != kGroupNone) sqrt_own = sqrt((double)op.orbit[row]);
@@ -140,12 +145,12 @@ __global__ __launch_bounds__(kBlock) void k_matvec(OpDev op, const V* __restrict
 
 template <typename V, int MODE>
 hipError_t launch_mv_t(const OpDev& op, int transpose, int accumulate, const void* x, void* y, hipStream_t st) {
-  dim3 grid((unsigned)((op.dim + kBlock - 1) / kBlock));
+  dim3 grid((unsigned)((op.dim + kMvBlock - 1) / kMvBlock));
   if (transpose)
-    hipLaunchKernelGGL((k_matvec<V, MODE, true>), grid, dim3(kBlock), op.lds_total, st, op, (const V*)x, (V*)y,
+    hipLaunchKernelGGL((k_matvec<V, MODE, true>), grid, dim3(kMvBlock), op.lds_total, st, op, (const V*)x, (V*)y,
                        accumulate);
   else
-    hipLaunchKernelGGL((k_matvec<V, MODE, false>), grid, dim3(kBlock), op.lds_total, st, op, (const V*)x, (V*)y,
+    hipLaunchKernelGGL((k_matvec<V, MODE, false>), grid, dim3(kMvBlock), op.lds_total, st, op, (const V*)x, (V*)y,
                        accumulate);
   return hipGetLastError();
 }
