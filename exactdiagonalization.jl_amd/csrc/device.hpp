@@ -152,12 +152,12 @@ __device__ __forceinline__ int64_t bitmap_index(const IndexDev& ix, uint64_t key
   const uint32_t within = (uint32_t)(key - (uint64_t)j * kRankBlockWords);
   const uint32_t w = within >> 6, bit = within & 63u;
   const uint64_t* blk = ix.bitmap + (uint64_t)j * 8u;
+  // the three reads hit one cache line; issue them together
   const uint64_t word = blk[w];
-  if (!((word >> bit) & 1ull)) return -1;
-  const uint64_t base = blk[6];
-  const uint64_t packed = blk[7];
-  const uint32_t sub = w ? (uint32_t)(packed >> (9u * (w - 1u))) & 511u : 0u;
-  return (int64_t)(base + sub + (uint64_t)__builtin_popcountll(word & ((1ull << bit) - 1ull)));
+  const ulonglong2 tail = *(const ulonglong2*)(blk + 6);  // base rank, packed sub-counts
+  const uint32_t sub = w ? (uint32_t)(tail.y >> (9u * (w - 1u))) & 511u : 0u;
+  const int64_t r = (int64_t)(tail.x + sub + (uint64_t)__builtin_popcountll(word & ((1ull << bit) - 1ull)));
+  return ((word >> bit) & 1ull) ? r : -1;
 }
 
 __device__ __forceinline__ int64_t search_sorted(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ prefix,
@@ -190,8 +190,13 @@ __device__ __forceinline__ int64_t index_of(const IndexDev& ix, const uint64_t* 
 // Symmetry group description on the device.
 // ----------------------------------------------------------------------------
 enum GroupMode : int { kGroupNone = 0, kGroupGeneric = 1, kGroupTorus = 2 };
-// kernel specialisation of kGroupTorus for groups with the row-pair table
+// kernel specialisations of kGroupTorus for groups with the row-pair table:
+// any shape, and the common shapes fixed at compile time
 constexpr int kGroupTorusPairs = 3;
+constexpr int kTorusPairs6x6 = 4;
+constexpr int kTorusPairs4x6 = 5;
+constexpr int kTorusPairs6x4 = 6;
+constexpr int kTorusPairs4x4 = 7;
 
 constexpr int kMaxLy = 64;
 
@@ -281,12 +286,17 @@ __device__ __forceinline__ void row_minrot(const GroupDev& g, const GroupLds& L,
 // bit 8y + a) and walked in a single loop, so lanes of a wave stay together
 // unless one of them has several such candidates.  Same s, h and cnt as the
 // generic torus path below.
+// LXC, LYC > 0 fix the torus shape at compile time; 0 reads it from g.
+template <int LXC = 0, int LYC = 0>
 __device__ __forceinline__ void canonicalize_torus_pairs(const GroupDev& g, const GroupLds& L, uint64_t b, uint64_t& s,
                                                          int& h, int& cnt) {
-  const int Lx = g.Lx, Ly = g.Ly;
+  const int Lx = LXC ? LXC : g.Lx;
+  const int Ly = LYC ? LYC : g.Ly;
+  const int NB = LXC ? LXC * LYC : g.nb;
+  const uint64_t FULL = LXC ? ((1ull << (LXC * LYC)) - 1ull) : g.full_mask;
   const uint32_t m2 = (1u << (2 * Lx)) - 1u;
   // row Ly-1 copied below row 0: the window at y*Lx holds (row_y << Lx) | row_{y-1}
-  const uint64_t w = (b << Lx) | (b >> (g.nb - Lx));
+  const uint64_t w = (b << Lx) | (b >> (NB - Lx));
   uint32_t e[8];
   uint32_t best = 0xffffffffu;
 #pragma unroll
@@ -313,7 +323,7 @@ __device__ __forceinline__ void canonicalize_torus_pairs(const GroupDev& g, cons
     cand &= cand - 1;
     const int a = bit & 7;
     const int sh = (Ly - 1 - (bit >> 3)) * Lx;  // row y moves to the top
-    const uint64_t t = ((b << sh) | (b >> (g.nb - sh))) & g.full_mask;
+    const uint64_t t = ((b << sh) | (b >> (NB - sh))) & FULL;
     const uint64_t img = ((t << a) & L.rot_hi[a]) | ((t >> (Lx - a)) & L.rot_lo[a]);
     const bool lt = img < s, eq = img == s;
     if (eq) {  // a tie (s has a nontrivial stabiliser): keep the largest inverse index

@@ -28,6 +28,14 @@ struct Canon {
       cnt = 1;
     } else if constexpr (MODE == kGroupTorusPairs) {
       canonicalize_torus_pairs(g, L, b, s, h, cnt);
+    } else if constexpr (MODE == kTorusPairs6x6) {
+      canonicalize_torus_pairs<6, 6>(g, L, b, s, h, cnt);
+    } else if constexpr (MODE == kTorusPairs4x6) {
+      canonicalize_torus_pairs<4, 6>(g, L, b, s, h, cnt);
+    } else if constexpr (MODE == kTorusPairs6x4) {
+      canonicalize_torus_pairs<6, 4>(g, L, b, s, h, cnt);
+    } else if constexpr (MODE == kTorusPairs4x4) {
+      canonicalize_torus_pairs<4, 4>(g, L, b, s, h, cnt);
     } else {
       canonicalize(g, L, b, s, h, cnt);
     }
@@ -163,7 +171,14 @@ hipError_t launch_mv_v(const OpDev& op, int transpose, int accumulate, const voi
     case kGroupGeneric:
       return launch_mv_t<V, kGroupGeneric>(op, transpose, accumulate, x, y, st);
     default:
-      if (op.g.pair_table) return launch_mv_t<V, kGroupTorusPairs>(op, transpose, accumulate, x, y, st);
+      if (op.g.pair_table) {
+        const int Lx = op.g.Lx, Ly = op.g.Ly;
+        if (Lx == 6 && Ly == 6) return launch_mv_t<V, kTorusPairs6x6>(op, transpose, accumulate, x, y, st);
+        if (Lx == 4 && Ly == 6) return launch_mv_t<V, kTorusPairs4x6>(op, transpose, accumulate, x, y, st);
+        if (Lx == 6 && Ly == 4) return launch_mv_t<V, kTorusPairs6x4>(op, transpose, accumulate, x, y, st);
+        if (Lx == 4 && Ly == 4) return launch_mv_t<V, kTorusPairs4x4>(op, transpose, accumulate, x, y, st);
+        return launch_mv_t<V, kGroupTorusPairs>(op, transpose, accumulate, x, y, st);
+      }
       return launch_mv_t<V, kGroupTorus>(op, transpose, accumulate, x, y, st);
   }
 }

@@ -34,7 +34,8 @@ def _torus_system(ed, oracle, Lx, Ly, k, j2=0.5, field=0.0):
 STRATS = [("terms", "search"), ("terms", "bitmap"), ("pairs", "search"), ("pairs", "bitmap"), ("pairs", "hash")]
 
 
-@pytest.mark.parametrize("Lx,Ly,k", [(4, 4, (2, 0)), (4, 4, (1, 2)), (4, 6, (2, 0)), (4, 6, (1, 3)), (3, 4, (0, 1))])
+@pytest.mark.parametrize("Lx,Ly,k", [(4, 4, (2, 0)), (4, 4, (1, 2)), (4, 6, (2, 0)), (4, 6, (1, 3)), (6, 4, (3, 0)),
+                                     (6, 4, (1, 2)), (3, 4, (0, 1)), (5, 4, (2, 1))])
 @pytest.mark.parametrize("dtype", [np.complex128, np.float64, np.float32, np.complex64])
 @pytest.mark.parametrize("transpose", [False, True])
 def test_strategies_vs_oracle(ed, oracle, Lx, Ly, k, dtype, transpose):
@@ -104,3 +105,42 @@ def test_hash_table_widening(ed, oracle):
     assert rel(y1, o.matvec(xr)) < 1e-13 and rel(y2, o.matvec(xc)) < 1e-13 and rel(y3, y1) < 1e-15
     y4 = (opr @ xr.astype(np.float32)).astype(np.float64)
     assert rel(y4, y1) < 1e-5
+
+
+def test_torus6x6_properties(ed):
+    """The 6x6 J1-J2 k=(0,0) sector (252,091,362 representatives), where no
+    oracle fits: the product is Hermitian, the row and column forms agree, the
+    bond-pair kernel agrees with the term-list kernel, and the FP32 product
+    agrees with the FP64 one to FP32 accuracy."""
+    import torch
+
+    Lx = Ly = 6
+    hs, _ = ed.spin_half_system(36)
+    j1, j2 = ed.square_bonds(Lx, Ly)
+    H = _heis(ed, hs, j1 + j2, [1.0] * len(j1) + [0.5] * len(j2))
+    rh = ed.symmetry_reduce(ed.represent(ed.HilbertSpaceSector(hs, 0)), ed.TranslationIrrep((Lx, Ly), (0, 0)))
+    opr = ed.represent(rh, H)
+    D = opr.dimension
+    assert D == 252091362
+    g = torch.Generator(device="cuda").manual_seed(11)
+    u = torch.randn(D, dtype=torch.float64, device="cuda", generator=g)
+    v = torch.randn(D, dtype=torch.float64, device="cuda", generator=g)
+    Hv = torch.empty_like(v)
+    Hu = torch.empty_like(u)
+    ed.mul_(Hv, opr, v)
+    ed.mul_(Hu, opr, u)
+    a, b = torch.dot(u, Hv).item(), torch.dot(Hu, v).item()
+    assert abs(a - b) <= 1e-11 * abs(a)
+    vH = torch.empty_like(v)
+    ed.mul_(vH, v, opr)
+    assert (torch.linalg.norm(vH - Hv) / torch.linalg.norm(Hv)).item() < 1e-13
+    opr.set_strategy("pairs", "bitmap")
+    Hv2 = torch.empty_like(v)
+    ed.mul_(Hv2, opr, v)
+    opr.set_strategy()
+    assert (torch.linalg.norm(Hv2 - Hv) / torch.linalg.norm(Hv)).item() < 1e-13
+    del Hu, vH, Hv2, u
+    v32 = v.float()
+    Hv32 = torch.empty_like(v32)
+    ed.mul_(Hv32, opr, v32)
+    assert (torch.linalg.norm(Hv32.double() - Hv) / torch.linalg.norm(Hv)).item() < 1e-5
