@@ -446,6 +446,9 @@ void build_group(GroupHost& gh, int nb, double tol, hipStream_t st) {
   d.mode = gh.mode;
   d.G = G;
   d.nb = nb;
+  d.unit_phases = 1;
+  for (int h = 0; h < G; ++h)
+    if (pr[h] != 1.0 || pi[h] != 0.0) d.unit_phases = 0;
   d.full_mask = (nb >= 64) ? ~0ull : ((1ull << nb) - 1);
   GroupLdsLayout& L = gh.lay;
   L = GroupLdsLayout{};

@@ -207,6 +207,8 @@ struct GroupDev {
   int Lx, Ly, nb;
   int minrot_table;     // torus: 1 if the per-row minimal-rotation table is used
   int pair_table;       // torus: 1 if the adjacent-row-pair table is used (Lx <= 6, 2 <= Ly <= 8)
+  int unit_phases;      // 1 if every phase is exactly 1 (all characters 1)
+  int pad3_;
   uint64_t row_mask;    // torus: (1 << Lx) - 1
   uint64_t full_mask;   // (1 << nb) - 1
   const uint64_t* nib;  // generic: G * n_nib * 16 images of 4-bit chunks

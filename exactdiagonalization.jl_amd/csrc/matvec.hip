@@ -61,7 +61,52 @@ __device__ __forceinline__ Staged stage_op(const OpDev& op, unsigned char* lds) 
   return S;
 }
 
+// The matvec's copy of the off-diagonal terms in LDS, split by use: (mask,
+// match) pairs read by every lane alike while matching a row, and (flip,
+// amplitude) pairs read per matched term, where flip = match ^ image gives the
+// image of a matching row b as b ^ flip; imaginary parts apart.
+struct MvTerms {
+  const uint64_t* mm;  // 2 per term: mask, match
+  const uint64_t* fa;  // 2 per term: flip, bits of Re(amplitude)
+  const double* ai;    // Im(amplitude)
+};
+
+__device__ __forceinline__ MvTerms stage_mv_terms(const OpDev& op, unsigned char* lds) {
+  uint64_t* mm = (uint64_t*)(lds + op.off_lds);  // same n_off * 40 bytes as the OffTerm list
+  uint64_t* fa = mm + 2 * op.n_off;
+  double* ai = (double*)(fa + 2 * op.n_off);
+  for (int t = threadIdx.x; t < op.n_off; t += blockDim.x) {
+    const OffTerm o = op.off[t];
+    mm[2 * t] = o.mask;
+    mm[2 * t + 1] = o.match;
+    fa[2 * t] = o.match ^ o.image;
+    fa[2 * t + 1] = (uint64_t)__double_as_longlong(o.are);
+    ai[t] = o.aim;
+  }
+  return MvTerms{mm, fa, ai};
+}
+
 // Weight of an off-diagonal element: a * phase^(conj if TRANS) * sqrt(N_own) / sqrt(N_img)
+template <int MODE, bool TRANS>
+__device__ __forceinline__ void element_weight_a(const OpDev& op, const GroupLds& L, double are, double aim, int h,
+                                                 int cnt, double sqrt_own, double& wr, double& wi) {
+  if constexpr (MODE == kGroupNone) {
+    wr = are;
+    wi = aim;
+  } else {
+    const double sc = sqrt_own * L.sqcnt[cnt];
+    if (op.g.unit_phases) {
+      wr = are * sc;
+      wi = aim * sc;
+    } else {
+      const double pr = L.ph_re[h];
+      const double pi = TRANS ? -L.ph_im[h] : L.ph_im[h];
+      wr = (are * pr - aim * pi) * sc;
+      wi = (are * pi + aim * pr) * sc;
+    }
+  }
+}
+
 template <int MODE, bool TRANS>
 __device__ __forceinline__ void element_weight(const OpDev& op, const Staged& S, const OffTerm& t, int h, int cnt,
                                                double sqrt_own, double& wr, double& wi) {
@@ -104,7 +149,11 @@ constexpr int kMvBlock = 512;
 template <typename V, int MODE, bool TRANS>
 __global__ __launch_bounds__(kMvBlock) void k_matvec(OpDev op, const V* __restrict__ x, V* __restrict__ y, int accumulate) {
   extern __shared__ unsigned char lds[];
-  Staged S = stage_op<MODE>(op, lds);
+  GroupLds L{};
+  if constexpr (MODE != kGroupNone) L = stage_group(op.g, op.lay, lds);
+  const MvTerms T = stage_mv_terms(op, lds);
+  lds_copy(lds + op.w_lds, op.ix.W, op.w_bytes);
+  const uint64_t* W = (const uint64_t*)(lds + op.w_lds);
   __syncthreads();
   const int64_t row = (int64_t)blockIdx.x * kMvBlock + threadIdx.x;
   if (row >= op.dim) return;
@@ -129,21 +178,23 @@ __global__ __launch_bounds__(kMvBlock) void k_matvec(OpDev op, const V* __restri
     const int nseg = (op.n_off - seg) < 64 ? (op.n_off - seg) : 64;
     uint64_t bits = 0;
     for (int j = 0; j < nseg; ++j) {
-      const OffTerm& t = S.off[seg + j];
-      bits |= (uint64_t)((b & t.mask) == t.match) << j;
+      const uint64_t* m = T.mm + 2 * (seg + j);
+      bits |= (uint64_t)((b & m[0]) == m[1]) << j;
     }
     while (bits) {
-      const int j = __builtin_ctzll(bits);
+      const int t = seg + __builtin_ctzll(bits);
       bits &= bits - 1;
-      const OffTerm t = S.off[seg + j];
-      const uint64_t img = (b & ~t.mask) | t.image;
+      const ulonglong2 fa = *(const ulonglong2*)(T.fa + 2 * t);
+      const uint64_t img = b ^ fa.x;  // (b & ~mask) | image for a matching row
+      const double are = __longlong_as_double((long long)fa.y);
+      const double aim = VT<V>::cx ? T.ai[t] : 0.0;
       uint64_t s;
       int h, cnt;
-      Canon<MODE>::run(op.g, S.L, img, s, h, cnt);
-      const int64_t idx = index_of(op.ix, S.W, s);
+      Canon<MODE>::run(op.g, L, img, s, h, cnt);
+      const int64_t idx = index_of(op.ix, W, s);
       if (idx >= 0) {
         double wr, wi;
-        element_weight<MODE, TRANS>(op, S, t, h, cnt, sqrt_own, wr, wi);
+        element_weight_a<MODE, TRANS>(op, L, are, aim, h, cnt, sqrt_own, wr, wi);
         vfma(acc, (R)wr, (R)wi, x[idx]);
       }
     }
