@@ -1834,10 +1834,10 @@ int edhip_term_expect(edhip_op* op, int prec, const void* phi, const void* psi, 
     const int nt = op->n_diag + op->n_off;
     if (nt == 0) return;
     int64_t nb = term_expect_blocks(op->dev_row);
-    op->ex_partial.ensure((size_t)nb * nt * 2 * 8);
+    op->ex_partial.ensure((size_t)term_expect_scratch_rows(op->dev_row) * nt * 2 * 8);
     op->ex_out.ensure((size_t)nt * 2 * 8);
-    uint32_t lds = op->dev_row.lds_total + (uint32_t)(4 * nt * 2 * 8);
-    if (lds > 64 * 1024) fail(EDHIP_ERR_ARGUMENT, "too many terms for the fused expectation kernel");
+    if (term_expect_window(op->dev_row, prec_is_complex(prec) ? 2 : 1) < 1)
+      fail(EDHIP_ERR_ARGUMENT, "operator and symmetry tables leave no LDS room for the fused expectation kernel");
     HIP_OR_THROW(launch_term_expect(op->dev_row, prec, phi, psi, op->ex_partial.as<double>(), nb,
                                     op->ex_out.as<double>(), op->ctx->stream));
     std::vector<double> h((size_t)nt * 2);

@@ -122,6 +122,11 @@ hipError_t launch_sparse_compact(int64_t dim, int slot, const int64_t* d_colptr,
 hipError_t launch_term_expect(const OpDev& op, int prec, const void* phi, const void* psi, double* d_partial,
                               int64_t n_blocks, double* d_out, hipStream_t st);
 int64_t term_expect_blocks(const OpDev& op);
+// rows of n_terms * 2 doubles the kernel needs in d_partial (per-block sums
+// and the reduction's chunk sums); terms per pass for vectors of nc real
+// components (0: the tables leave no LDS room)
+int64_t term_expect_scratch_rows(const OpDev& op);
+int term_expect_window(const OpDev& op, int nc);
 
 // ---- fast path: bond-pair term groups + register canonicaliser + hash lookup
 // Terms grouped by bit mask (1 or 2 bits).  For a row b the group's pattern

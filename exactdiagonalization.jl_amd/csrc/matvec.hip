@@ -378,16 +378,9 @@ __global__ __launch_bounds__(kBlock) void k_sparse_compact(int64_t dim, int slot
 // ---------------------------------------------------------------------------
 // Per-term bilinear forms (backward pass):
 //   g_t = sum_r conj(phi_r) * sum_{images of term t from b_r} w_unit * psi_s
-// where w_unit is the element weight with unit amplitude.  One uniform loop
-// over the terms; each term is reduced across the wavefront by xor-shuffles,
-// then across the block in LDS in a fixed order; the per-block partials are
-// summed in a fixed order by k_reduce_partials.  Output is interleaved
-// (re, im) per term, diagonal terms first, then off-diagonal ones.
+// where w_unit is the element weight with unit amplitude.  Output is
+// interleaved (re, im) per term, diagonal terms first, then off-diagonal ones.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ double wave_sum(double v) {
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
-}
 
 template <typename V>
 __device__ __forceinline__ void load_c(const V* p, int64_t i, double& re, double& im) {
@@ -401,77 +394,114 @@ __device__ __forceinline__ void load_c(const V* p, int64_t i, double& re, double
 }
 
 constexpr int kExpectBlocksMax = 2048;
+constexpr int kExpectWaves = kBlock / 64;
 
+// Per-term bilinear forms g_t = sum_r conj(phi_r) sum_{images of t} w psi_s
+// (unit amplitudes), for the terms [t0, t0 + nw) of the combined numbering
+// (diagonal terms first, then off-diagonal ones).  One lane per row, as in the
+// matvec: a lane walks its row's matched terms and adds each contribution to
+// its wave's bin of that term in LDS (float64 LDS atomics; a wave only adds
+// to its own bins, in program order).  Blocks walk the rows grid-stride; the
+// block sums its waves' bins in a fixed order into partial[block][t][2].
 template <typename V, int MODE>
 __global__ __launch_bounds__(kBlock) void k_term_expect(OpDev op, const V* __restrict__ phi, const V* __restrict__ psi,
-                                                        double* __restrict__ partial) {
+                                                        int t0, int nw, double* __restrict__ partial) {
+  constexpr bool CX = VT<V>::cx;  // real vectors come with real weights
+  constexpr int NC = CX ? 2 : 1;
   extern __shared__ unsigned char lds[];
-  Staged S = stage_op<MODE>(op, lds);
-  const int n_terms = op.n_diag + op.n_off;
-  double* acc = (double*)(lds + op.lds_total);  // [kBlock/64 waves][n_terms][2]
-  const int wave = threadIdx.x >> 6;
-  const int lane = threadIdx.x & 63;
-  for (int i = threadIdx.x; i < (kBlock / 64) * n_terms * 2; i += kBlock) acc[i] = 0.0;
+  GroupLds L{};
+  if constexpr (MODE != kGroupNone) L = stage_group(op.g, op.lay, lds);
+  const MvTerms T = stage_mv_terms(op, lds);
+  lds_copy(lds + op.w_lds, op.ix.W, op.w_bytes);
+  const uint64_t* W = (const uint64_t*)(lds + op.w_lds);
+  double* bins = (double*)(lds + op.lds_total);  // [wave][nw][NC]
+  for (int i = threadIdx.x; i < kExpectWaves * nw * NC; i += kBlock) bins[i] = 0.0;
   __syncthreads();
-  for (int64_t base = (int64_t)blockIdx.x * kBlock; base < op.dim; base += (int64_t)gridDim.x * kBlock) {
-    const int64_t row = base + threadIdx.x;
-    const bool valid = row < op.dim;
-    const uint64_t b = valid ? op.rows[row] : 0ull;
-    double fr = 0.0, fi = 0.0, pr = 0.0, pi = 0.0;
-    if (valid) {
-      load_c(phi, row, fr, fi);
-      load_c(psi, row, pr, pi);
-    }
-    double sqrt_own = 1.0;
-    if constexpr (MODE != kGroupNone) sqrt_own = valid ? sqrt((double)op.orbit[row]) : 1.0;
-    // conj(phi_r) * psi_r for the diagonal terms
-    const double dr = fr * pr + fi * pi, di = fr * pi - fi * pr;
-    for (int t = 0; t < op.n_diag; ++t) {
-      const DiagTerm d = op.diag[t];
-      const bool m = valid && ((b & d.mask) == d.match);
-      double vr = wave_sum(m ? dr : 0.0);
-      double vi = wave_sum(m ? di : 0.0);
-      if (lane == 0) {
-        acc[(wave * n_terms + t) * 2] += vr;
-        acc[(wave * n_terms + t) * 2 + 1] += vi;
+  double* mine = bins + (threadIdx.x >> 6) * nw * NC;
+  const int n_diag = op.n_diag, n_terms = op.n_diag + op.n_off;
+  const int d_hi = t0 + nw < n_diag ? t0 + nw : n_diag;              // diagonal terms [t0, d_hi)
+  const int o_lo = (t0 > n_diag ? t0 : n_diag) - n_diag;              // off-diagonal terms [o_lo, o_hi)
+  const int o_hi = (t0 + nw < n_terms ? t0 + nw : n_terms) - n_diag;
+  for (int64_t row = (int64_t)blockIdx.x * kBlock + threadIdx.x; row < op.dim; row += (int64_t)gridDim.x * kBlock) {
+    const uint64_t b = op.rows[row];
+    double fr, fi, pr, pi;
+    load_c(phi, row, fr, fi);
+    load_c(psi, row, pr, pi);
+    for (int seg = t0; seg < d_hi; seg += 64) {
+      const int n = d_hi - seg < 64 ? d_hi - seg : 64;
+      uint64_t bits = 0;
+      for (int j = 0; j < n; ++j) {
+        const DiagTerm d = op.diag[seg + j];
+        bits |= (uint64_t)((b & d.mask) == d.match) << j;
+      }
+      while (bits) {
+        const int t = seg + __builtin_ctzll(bits);
+        bits &= bits - 1;
+        atomicAdd(mine + (t - t0) * NC, fr * pr + fi * pi);  // conj(phi_r) psi_r
+        if constexpr (CX) atomicAdd(mine + (t - t0) * NC + 1, fr * pi - fi * pr);
       }
     }
-    for (int t = 0; t < op.n_off; ++t) {
-      const OffTerm& ot = S.off[t];
-      double vr = 0.0, vi = 0.0;
-      if (valid && (b & ot.mask) == ot.match) {
-        const uint64_t img = (b & ~ot.mask) | ot.image;
-        uint64_t s;
-        int h, cnt;
-        Canon<MODE>::run(op.g, S.L, img, s, h, cnt);
-        const int64_t idx = index_of(op.ix, S.W, s);
-        if (idx >= 0) {
-          OffTerm unit = ot;
-          unit.are = 1.0;
-          unit.aim = 0.0;
-          double wr, wi;
-          element_weight<MODE, false>(op, S, unit, h, cnt, sqrt_own, wr, wi);
+    if (o_lo < o_hi) {
+      double sqrt_own = 1.0;
+      if constexpr (MODE != kGroupNone) sqrt_own = sqrt((double)op.orbit[row]);
+      for (int seg = o_lo; seg < o_hi; seg += 64) {
+        const int n = o_hi - seg < 64 ? o_hi - seg : 64;
+        uint64_t bits = 0;
+        for (int j = 0; j < n; ++j) {
+          const uint64_t* m = T.mm + 2 * (seg + j);
+          bits |= (uint64_t)((b & m[0]) == m[1]) << j;
+        }
+        while (bits) {
+          const int j = seg + __builtin_ctzll(bits);
+          bits &= bits - 1;
+          const uint64_t img = b ^ T.fa[2 * j];
+          uint64_t s;
+          int h, cnt;
+          Canon<MODE>::run(op.g, L, img, s, h, cnt);
+          const int64_t idx = index_of(op.ix, W, s);
+          if (idx < 0) continue;
+          double wr = 1.0, wi = 0.0;
+          if constexpr (MODE != kGroupNone) {
+            const double sc = sqrt_own * L.sqcnt[cnt];
+            wr = L.ph_re[h] * sc;
+            if constexpr (CX) wi = L.ph_im[h] * sc;
+          }
           double sr, si;
           load_c(psi, idx, sr, si);
-          const double qr = wr * sr - wi * si, qi = wr * si + wi * sr;  // w * psi_s
-          vr = fr * qr + fi * qi;                                       // conj(phi) * (w psi_s)
-          vi = fr * qi - fi * qr;
+          const double qr = wr * sr - wi * si;  // w * psi_s
+          const int t = n_diag + j - t0;
+          if constexpr (CX) {
+            const double qi = wr * si + wi * sr;
+            atomicAdd(mine + t * 2, fr * qr + fi * qi);  // conj(phi_r) * q
+            atomicAdd(mine + t * 2 + 1, fr * qi - fi * qr);
+          } else {
+            atomicAdd(mine + t, fr * qr);
+          }
         }
-      }
-      vr = wave_sum(vr);
-      vi = wave_sum(vi);
-      if (lane == 0) {
-        acc[(wave * n_terms + op.n_diag + t) * 2] += vr;
-        acc[(wave * n_terms + op.n_diag + t) * 2 + 1] += vi;
       }
     }
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < n_terms * 2; i += kBlock) {
-    double s = 0.0;
-    for (int w = 0; w < kBlock / 64; ++w) s += acc[w * n_terms * 2 + i];
-    partial[(int64_t)blockIdx.x * n_terms * 2 + i] = s;
+  double* out = partial + (int64_t)blockIdx.x * n_terms * 2;
+  for (int i = threadIdx.x; i < nw * 2; i += kBlock) {
+    double v = 0.0;
+    if (CX || (i & 1) == 0)
+      for (int w = 0; w < kExpectWaves; ++w) v += bins[w * nw * NC + (CX ? i : (i >> 1))];
+    out[2 * t0 + i] = v;
   }
+}
+
+// Sum of the rows [64 c, 64 c + 64) of partial (n_parts x width) for 64
+// columns per block: inter[c * width + col], rows added in order.
+__global__ __launch_bounds__(64) void k_sum_parts_chunks(const double* __restrict__ partial, int64_t n_parts, int width,
+                                                         double* __restrict__ inter) {
+  const int col = blockIdx.x * 64 + threadIdx.x;
+  if (col >= width) return;
+  const int64_t p0 = (int64_t)blockIdx.y * 64;
+  const int64_t p1 = p0 + 64 < n_parts ? p0 + 64 : n_parts;
+  double s = 0.0;
+  for (int64_t p = p0; p < p1; ++p) s += partial[p * width + col];
+  inter[(int64_t)blockIdx.y * width + col] = s;
 }
 
 __global__ __launch_bounds__(kBlock) void k_reduce_partials(const double* __restrict__ partial, int64_t n_blocks, int width,
@@ -487,14 +517,26 @@ template <typename V, int MODE>
 hipError_t launch_expect_t(const OpDev& op, const void* phi, const void* psi, double* d_partial, int64_t n_blocks,
                            double* d_out, hipStream_t st) {
   const int n_terms = op.n_diag + op.n_off;
-  uint32_t lds = op.lds_total + (uint32_t)((kBlock / 64) * n_terms * 2 * sizeof(double));
-  hipLaunchKernelGGL((k_term_expect<V, MODE>), dim3((unsigned)n_blocks), dim3(kBlock), lds, st, op, (const V*)phi,
-                     (const V*)psi, d_partial);
+  const int nw_max = term_expect_window(op, VT<V>::cx ? 2 : 1);
+  if (nw_max <= 0) return hipErrorInvalidValue;
+  for (int t0 = 0; t0 < n_terms; t0 += nw_max) {
+    const int nw = n_terms - t0 < nw_max ? n_terms - t0 : nw_max;
+    const uint32_t lds = op.lds_total + (uint32_t)(kExpectWaves * nw * (VT<V>::cx ? 2 : 1) * 8);
+    hipLaunchKernelGGL((k_term_expect<V, MODE>), dim3((unsigned)n_blocks), dim3(kBlock), lds, st, op, (const V*)phi,
+                       (const V*)psi, t0, nw, d_partial);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  // fixed-order reduction of the per-block rows: chunks of 64 rows, then the chunks
+  const int width = n_terms * 2;
+  const int64_t n_chunks = (n_blocks + 63) / 64;
+  double* inter = d_partial + n_blocks * width;
+  hipLaunchKernelGGL(k_sum_parts_chunks, dim3((unsigned)((width + 63) / 64), (unsigned)n_chunks), dim3(64), 0, st,
+                     d_partial, n_blocks, width, inter);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  const int width = n_terms * 2;
-  hipLaunchKernelGGL(k_reduce_partials, dim3((unsigned)((width + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, d_partial,
-                     n_blocks, width, d_out);
+  hipLaunchKernelGGL(k_reduce_partials, dim3((unsigned)((width + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, inter,
+                     n_chunks, width, d_out);
   return hipGetLastError();
 }
 
@@ -603,11 +645,27 @@ hipError_t launch_sparse_compact(int64_t dim, int slot, const int64_t* d_colptr,
   return hipGetLastError();
 }
 
+// terms per pass: the per-wave bins (nc doubles per term) fill the LDS left
+// over by the operator and symmetry tables (64 KiB per block)
+int term_expect_window(const OpDev& op, int nc) {
+  const int64_t room = 64 * 1024 - (int64_t)op.lds_total;
+  const int64_t per_term = (int64_t)kExpectWaves * nc * 8;
+  const int64_t nw = room / per_term;
+  const int n_terms = op.n_diag + op.n_off;
+  if (nw < 1) return 0;
+  return nw < n_terms ? (int)nw : n_terms;
+}
+
 int64_t term_expect_blocks(const OpDev& op) {
   int64_t nb = (op.dim + kBlock - 1) / kBlock;
   if (nb > kExpectBlocksMax) nb = kExpectBlocksMax;
   if (nb < 1) nb = 1;
   return nb;
+}
+
+int64_t term_expect_scratch_rows(const OpDev& op) {
+  const int64_t nb = term_expect_blocks(op);
+  return nb + (nb + 63) / 64;
 }
 
 hipError_t launch_term_expect(const OpDev& op, int prec, const void* phi, const void* psi, double* d_partial,

@@ -137,3 +137,23 @@ def test_autograd_functions(ed):
     v, gc, _ = correlator_and_grad(model, c, J, O, tol=1e-13, minres_tol=1e-13)
     assert abs(val.item() - v) < 1e-10
     assert np.max(np.abs(th2.grad.numpy() - 2.0 * gc)) < 1e-8
+
+
+def test_term_expectations_bitwise_repeatable(ed):
+    """The fused per-term pass sums in a fixed order: two runs on the same
+    vectors give identical bits (4x6 XXZ, 2,704,156 states, 288 terms)."""
+    from edhip.grad import ground_state
+
+    hs, s, hsr, bonds, model, fn, theta = _xxz(ed, 4, 6, 7)
+    c, J = fn(theta)
+    H, r = ground_state(model, c, tol=1e-10)
+    psi = r.vectors[0]
+    a = model.expectations(psi, psi)
+    b = model.expectations(psi, psi)
+    assert np.array_equal(a, b)
+    import torch
+
+    phi = torch.roll(psi, 1)
+    x = model.expectations(phi, psi)
+    y = model.expectations(phi, psi)
+    assert np.array_equal(x, y)
