@@ -998,8 +998,9 @@ struct TRLanczos {
   edhip_op* op;
   int prec;        // storage precision of the Krylov vectors
   int64_t n;
+  int64_t ld = 0;  // stride of the basis vectors (n rounded up to 4 elements: 16-byte aligned vectors)
   int m;           // basis size
-  DevBuf V;        // (m + 1) vectors
+  DevBuf V;        // (m + 1) vectors of stride ld
   DevBuf tmp;      // restart combination scratch
   DevBuf coef_d;   // device coefficients
   const void* locked = nullptr;  // locked (deflated) vectors, same precision
@@ -1009,7 +1010,11 @@ struct TRLanczos {
   int iterations = 0, restarts = 0;
 
   size_t es() const { return elem_size(prec); }
-  void* vec(int j) const { return (char*)V.p + (size_t)j * n * es(); }
+  void set_size(int64_t dim) {
+    n = dim;
+    ld = (dim + 3) / 4 * 4;
+  }
+  void* vec(int j) const { return (char*)V.p + (size_t)j * ld * es(); }
 
   void matvec(const void* x, void* y) {
     edhip_ctx* ctx = op->ctx;
@@ -1034,17 +1039,17 @@ struct TRLanczos {
     return {h[0], h[1]};
   }
 
-  // w -= B (B^H w) twice over vectors B_0..B_{k-1} (stride n); returns the
+  // w -= B (B^H w) twice over vectors B_0..B_{k-1} (stride ldb); returns the
   // first-pass coefficients (host).
-  std::vector<double> orthogonalize(const void* B, int k, void* w) {
+  std::vector<double> orthogonalize(const void* B, int64_t ldb, int k, void* w) {
     edhip_ctx* ctx = op->ctx;
     std::vector<double> c1(2 * (size_t)k, 0.0);
     if (k <= 0) return c1;
     double* part = ctx->partial_buf((size_t)2 * k * 1024);
     coef_d.ensure((size_t)2 * k * sizeof(double));
     for (int pass = 0; pass < 2; ++pass) {
-      HIP_OR_THROW(vec_multi_dot(prec, n, k, B, n, w, part, coef_d.as<double>(), ctx->stream));
-      HIP_OR_THROW(vec_multi_axpy(prec, n, k, B, n, coef_d.as<double>(), w, ctx->stream));
+      HIP_OR_THROW(vec_multi_dot(prec, n, k, B, ldb, w, part, coef_d.as<double>(), ctx->stream));
+      HIP_OR_THROW(vec_multi_axpy(prec, n, k, B, ldb, coef_d.as<double>(), w, ctx->stream));
       if (pass == 0) {
         HIP_OR_THROW(hipMemcpyAsync(c1.data(), coef_d.p, 2 * (size_t)k * sizeof(double), hipMemcpyDeviceToHost,
                                     ctx->stream));
@@ -1091,8 +1096,8 @@ struct TRLanczos {
         void* wv = vec(j + 1);
         matvec(vj, wv);
         ++iterations;
-        if (n_locked) orthogonalize(locked, n_locked, wv);
-        std::vector<double> c = orthogonalize(V.p, j + 1, wv);
+        if (n_locked) orthogonalize(locked, n, n_locked, wv);
+        std::vector<double> c = orthogonalize(V.p, ld, j + 1, wv);
         T[(size_t)j * mm + j] = c[2 * (size_t)j];
         std::complex<double> nn = dot(wv, wv);
         double beta = std::sqrt(std::max(nn.real(), 0.0));
@@ -1148,9 +1153,9 @@ struct TRLanczos {
       coef_d.ensure(coef.size() * sizeof(double));
       HIP_OR_THROW(hipMemcpyAsync(coef_d.p, coef.data(), coef.size() * sizeof(double), hipMemcpyHostToDevice,
                                   ctx->stream));
-      tmp.ensure((size_t)keep * n * es());
-      HIP_OR_THROW(vec_combine(prec, prec, n, mm, V.p, n, keep, coef_d.as<double>(), tmp.p, n, ctx->stream));
-      HIP_OR_THROW(hipMemcpyAsync(V.p, tmp.p, (size_t)keep * n * es(), hipMemcpyDeviceToDevice, ctx->stream));
+      tmp.ensure((size_t)keep * ld * es());
+      HIP_OR_THROW(vec_combine(prec, prec, n, mm, V.p, ld, keep, coef_d.as<double>(), tmp.p, ld, ctx->stream));
+      HIP_OR_THROW(hipMemcpyAsync(V.p, tmp.p, (size_t)keep * ld * es(), hipMemcpyDeviceToDevice, ctx->stream));
       // residual direction: w = vec(mm) (unnormalised), beta_last its norm
       scale(vec(mm), 1.0 / beta_last);
       HIP_OR_THROW(hipMemcpyAsync(vec(keep), vec(mm), (size_t)n * es(), hipMemcpyDeviceToDevice, ctx->stream));
@@ -1173,7 +1178,7 @@ struct TRLanczos {
       coef_d.ensure(coef.size() * sizeof(double));
       HIP_OR_THROW(hipMemcpyAsync(coef_d.p, coef.data(), coef.size() * sizeof(double), hipMemcpyHostToDevice,
                                   ctx->stream));
-      HIP_OR_THROW(vec_combine(prec, out_prec, n, sz, V.p, n, nev, coef_d.as<double>(), out, n, ctx->stream));
+      HIP_OR_THROW(vec_combine(prec, out_prec, n, sz, V.p, ld, nev, coef_d.as<double>(), out, n, ctx->stream));
       ctx->sync();
     }
     (void)dim;
@@ -1754,9 +1759,9 @@ int edhip_lanczos(edhip_op* op, const edhip_lanczos_opts* opts, double* evals, v
     TRLanczos L;
     L.op = op;
     L.prec = prec_store;
-    L.n = n;
+    L.set_size(n);
     L.m = (int)std::min<int64_t>(opts->max_basis, n);
-    L.V.alloc((size_t)(L.m + 1) * n * elem_size(prec_store));
+    L.V.alloc((size_t)(L.m + 1) * L.ld * elem_size(prec_store));
     HIP_OR_THROW(vec_gaussian(prec_store, n, opts->seed, L.vec(0), ctx->stream));
     double nrm = std::sqrt(L.dot(L.vec(0), L.vec(0)).real());
     L.scale(L.vec(0), 1.0 / nrm);
@@ -1789,17 +1794,17 @@ int edhip_lanczos(edhip_op* op, const edhip_lanczos_opts* opts, double* evals, v
         TRLanczos R;
         R.op = op;
         R.prec = prec64;
-        R.n = n;
+        R.set_size(n);
         R.m = (int)std::min<int64_t>(std::max(opts->refine_basis, 2), n - i);
         R.criterion = opts->criterion;
         R.gap_hint = (theta32.size() > (size_t)i + 1) ? (theta32[i + 1] - theta32[i]) : 0.0;
-        R.V.alloc((size_t)(R.m + 1) * n * elem_size(prec64));
+        R.V.alloc((size_t)(R.m + 1) * R.ld * elem_size(prec64));
         R.locked = ritz.p;
         R.n_locked = i;
         void* start = (char*)ritz.p + (size_t)i * n * elem_size(prec64);
         HIP_OR_THROW(hipMemcpyAsync(R.vec(0), start, (size_t)n * elem_size(prec64), hipMemcpyDeviceToDevice,
                                     ctx->stream));
-        if (i) R.orthogonalize(ritz.p, i, R.vec(0));
+        if (i) R.orthogonalize(ritz.p, n, i, R.vec(0));
         double nr = std::sqrt(R.dot(R.vec(0), R.vec(0)).real());
         R.scale(R.vec(0), 1.0 / nr);
         std::vector<double> th1, rs1;
@@ -1869,13 +1874,13 @@ int edhip_minres(edhip_op* op, int prec, double shift, int n_deflate, const void
     TRLanczos H;  // reuse the vector helpers
     H.op = op;
     H.prec = prec;
-    H.n = n;
+    H.set_size(n);
     DevBuf buf;
     buf.alloc(bytes * 7);
     auto V = [&](int i) { return (void*)((char*)buf.p + (size_t)i * bytes); };
     void *r1 = V(0), *r2 = V(1), *y = V(2), *v = V(3), *w = V(4), *w1 = V(5), *w2 = V(6);
     auto project = [&](void* z) {
-      if (n_deflate) H.orthogonalize(deflate, n_deflate, z);
+      if (n_deflate) H.orthogonalize(deflate, n, n_deflate, z);
     };
     auto apply_A = [&](const void* in, void* out) {
       op_matvec(op, prec, 0, 0, in, out);

@@ -132,6 +132,136 @@ __global__ __launch_bounds__(kBlock) void k_multi_axpy(int64_t n, int k, const V
   }
 }
 
+// ---- 16-byte-load variants of the multi-dot / multi-axpy ----------------
+// Used when every vector starts on a 16-byte boundary (the Krylov basis is
+// stored with a padded stride): each lane moves 16 bytes (E elements) per
+// load instead of one element.
+template <typename V>
+struct Pack16 {
+  static constexpr int E = 16 / (int)sizeof(V);
+  __device__ __forceinline__ static void load(const V* p, double re[E], double im[E]) {
+    const uint4 u = *(const uint4*)p;
+    const V* v = (const V*)&u;
+#pragma unroll
+    for (int e = 0; e < E; ++e) ld(v, e, re[e], im[e]);
+  }
+  __device__ __forceinline__ static void store(V* p, const double re[E], const double im[E]) {
+    uint4 u;
+    V* v = (V*)&u;
+#pragma unroll
+    for (int e = 0; e < E; ++e) st(v, e, re[e], im[e]);
+    *(uint4*)p = u;
+  }
+};
+
+__device__ __forceinline__ double wave_reduce(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// partial[blk * 2k + 2 (j0 + q) + {0,1}] = sum over this block's elements of
+// conj(V_{j0+q}) w, for a tile of KT vectors (blockIdx.y); fixed grid, fixed
+// order: deterministic.
+template <typename V, int KT>
+__global__ __launch_bounds__(kBlock) void k_multi_dot16(int64_t n, int k, const V* __restrict__ Vm, int64_t ldv,
+                                                        const V* __restrict__ w, double* __restrict__ partial) {
+  constexpr int E = Pack16<V>::E;
+  constexpr bool CX = VT<V>::cx;
+  __shared__ double sh[kBlock / 64][KT][2];
+  const int j0 = blockIdx.y * KT;
+  double ar[KT], ai[KT];
+#pragma unroll
+  for (int q = 0; q < KT; ++q) ar[q] = ai[q] = 0.0;
+  const int64_t ng = n / E;
+  for (int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x; g < ng; g += (int64_t)gridDim.x * kBlock) {
+    double wr[E], wi[E];
+    Pack16<V>::load(w + g * E, wr, wi);
+#pragma unroll
+    for (int q = 0; q < KT; ++q) {
+      if (j0 + q < k) {
+        double vr[E], vi[E];
+        Pack16<V>::load(Vm + (int64_t)(j0 + q) * ldv + g * E, vr, vi);
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+          ar[q] += vr[e] * wr[e] + vi[e] * wi[e];
+          if constexpr (CX) ai[q] += vr[e] * wi[e] - vi[e] * wr[e];
+        }
+      }
+    }
+  }
+  if (blockIdx.x == 0 && threadIdx.x < n - ng * E) {  // the last n mod E elements
+    const int64_t i = ng * E + threadIdx.x;
+    double wr, wi;
+    ld(w, i, wr, wi);
+#pragma unroll
+    for (int q = 0; q < KT; ++q) {
+      if (j0 + q < k) {
+        double vr, vi;
+        ld(Vm + (int64_t)(j0 + q) * ldv, i, vr, vi);
+        ar[q] += vr * wr + vi * wi;
+        if constexpr (CX) ai[q] += vr * wi - vi * wr;
+      }
+    }
+  }
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+  for (int q = 0; q < KT; ++q) {
+    const double a = wave_reduce(ar[q]);
+    const double b = CX ? wave_reduce(ai[q]) : 0.0;
+    if (lane == 0) {
+      sh[wave][q][0] = a;
+      sh[wave][q][1] = b;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < 2 * KT) {
+    const int q = threadIdx.x >> 1, c = threadIdx.x & 1;
+    if (j0 + q < k) {
+      double v = 0.0;
+      for (int wv = 0; wv < kBlock / 64; ++wv) v += sh[wv][q][c];
+      partial[(int64_t)blockIdx.x * 2 * k + 2 * (j0 + q) + c] = v;
+    }
+  }
+}
+
+// w -= sum_j coef_j V_j, 16 bytes per lane and vector
+template <typename V>
+__global__ __launch_bounds__(kBlock) void k_multi_axpy16(int64_t n, int k, const V* __restrict__ Vm, int64_t ldv,
+                                                         const double* __restrict__ coef, V* __restrict__ w) {
+  constexpr int E = Pack16<V>::E;
+  constexpr bool CX = VT<V>::cx;
+  const int64_t ng = n / E;
+  const int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (g < ng) {
+    double wr[E], wi[E];
+    Pack16<V>::load(w + g * E, wr, wi);
+#pragma unroll 4
+    for (int j = 0; j < k; ++j) {
+      double vr[E], vi[E];
+      Pack16<V>::load(Vm + (int64_t)j * ldv + g * E, vr, vi);
+      const double cr = coef[2 * j], ci = CX ? coef[2 * j + 1] : 0.0;
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        wr[e] -= cr * vr[e] - ci * vi[e];
+        if constexpr (CX) wi[e] -= cr * vi[e] + ci * vr[e];
+      }
+    }
+    Pack16<V>::store(w + g * E, wr, wi);
+  } else if (g < ng + (n - ng * E)) {  // the last n mod E elements, one per lane
+    const int64_t i = ng * E + (g - ng);
+    double wr, wi;
+    ld(w, i, wr, wi);
+    for (int j = 0; j < k; ++j) {
+      double vr, vi;
+      ld(Vm + (int64_t)j * ldv, i, vr, vi);
+      const double cr = coef[2 * j], ci = coef[2 * j + 1];
+      wr -= cr * vr - ci * vi;
+      wi -= cr * vi + ci * vr;
+    }
+    st(w, i, wr, wi);
+  }
+}
+
 template <typename V>
 __global__ __launch_bounds__(kBlock) void k_axpby(int64_t n, double ar, double ai, const V* __restrict__ x, double br,
                                                   double bi, V* __restrict__ y) {
@@ -238,10 +368,23 @@ struct DotOp {
   }
 };
 
+inline bool aligned16(const void* p) { return ((uintptr_t)p & 15u) == 0; }
+
 template <typename V>
 struct MultiDotOp {
   static hipError_t run(int64_t n, int k, const void* Vm, int64_t ldv, const void* w, double* partial, double* out,
                         hipStream_t s) {
+    if (aligned16(Vm) && aligned16(w) && ((ldv * (int64_t)sizeof(V)) & 15) == 0) {
+      constexpr int KT = VT<V>::cx ? 8 : 16;
+      const int nb = kRedBlocks;
+      dim3 grid(nb, (k + KT - 1) / KT);
+      hipLaunchKernelGGL((k_multi_dot16<V, KT>), grid, dim3(kBlock), 0, s, n, k, (const V*)Vm, ldv, (const V*)w,
+                         partial);
+      const int width = 2 * k;
+      hipLaunchKernelGGL(k_sum_partials, dim3((width + kBlock - 1) / kBlock), dim3(kBlock), 0, s, partial, nb, width,
+                         out);
+      return hipGetLastError();
+    }
     int nb = kRedBlocks / 4;
     dim3 grid(nb, (k + kTile - 1) / kTile);
     hipLaunchKernelGGL(k_multi_dot<V>, grid, dim3(kBlock), 0, s, n, k, (const V*)Vm, ldv, (const V*)w, partial);
@@ -254,6 +397,14 @@ struct MultiDotOp {
 template <typename V>
 struct MultiAxpyOp {
   static hipError_t run(int64_t n, int k, const void* Vm, int64_t ldv, const double* coef, void* w, hipStream_t s) {
+    if (aligned16(Vm) && aligned16(w) && ((ldv * (int64_t)sizeof(V)) & 15) == 0) {
+      constexpr int E = Pack16<V>::E;
+      const int64_t ng = n / E;
+      const int64_t threads = ng + (n - ng * E);
+      hipLaunchKernelGGL(k_multi_axpy16<V>, dim3((unsigned)((threads + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, n, k,
+                         (const V*)Vm, ldv, coef, (V*)w);
+      return hipGetLastError();
+    }
     hipLaunchKernelGGL(k_multi_axpy<V>, dim3(grid_stride_blocks(n)), dim3(kBlock), 0, s, n, k, (const V*)Vm, ldv, coef,
                        (V*)w);
     return hipGetLastError();
