@@ -54,6 +54,9 @@ from .representation import (
     symmetry_reduce_parallel,
     symmetry_reduce_serial,
     symmetry_unreduce,
+    GlobalBitFlip,
+    DirectProductOperation,
+    make_product_irrep,
 )
 from .solve import EigsResult, eigs, minres, term_expectations
 

@@ -169,11 +169,15 @@ struct GroupHost {
   int mode = kGroupNone;
   int Lx = 0, Ly = 0;
   std::vector<int32_t> perm;  // G x n_bits
+  std::vector<uint64_t> xr;   // G: word XORed into the permuted word (0 or all bits: global flip)
   std::vector<std::complex<double>> chi;
   std::vector<int32_t> inv;
   bool complex_phase = false;
+  bool has_xor = false;
+  bool flip_all = false;  // torus translations x global flip
   // device tables
-  DevBuf nib, minrot, pair, rot_hi, rot_lo, g_of_ab, inv_d, trivial, ph_re, ph_im, invsqrt, sqcnt;
+  DevBuf nib, minrot, pair, rot_hi, rot_lo, g_of_ab, inv_d, trivial, ph_re, ph_im, invsqrt, sqcnt, xr_d, flip_of_d,
+      inv_flip_d;
   GroupDev dev{};
   GroupLdsLayout lay{};
 };
@@ -386,31 +390,65 @@ void build_group(GroupHost& gh, int nb, double tol, hipStream_t st) {
     double a = std::abs(gh.chi[g]);
     if (!(std::fabs(a - 1.0) <= rtol * std::max(a, 1.0))) fail(EDHIP_ERR_ARGUMENT, "all amplitudes need to have norm 1");
   }
-  // closure and inverses
-  std::map<std::vector<int32_t>, int> index;
+  // XOR masks: none, or the global flip (all bits) on some elements
+  const uint64_t full = (nb >= 64) ? ~0ull : ((1ull << nb) - 1ull);
+  if (gh.xr.empty()) gh.xr.assign(G, 0ull);
+  if (gh.xr[0] != 0) fail(EDHIP_ERR_ARGUMENT, "the first symmetry operation must be the identity");
+  gh.has_xor = false;
+  for (int g = 0; g < G; ++g) {
+    if (gh.xr[g] != 0 && gh.xr[g] != full) fail(EDHIP_ERR_ARGUMENT, "a bit flip must flip every bit (GlobalBitFlip)");
+    if (gh.xr[g]) gh.has_xor = true;
+  }
+  // closure and inverses; an element is (permutation, mask):
+  // (p1, x1)(p2, x2) b = p1(p2(b) ^ x2) ^ x1 = p1(p2(b)) ^ (p1(x2) ^ x1), and p(full) = full
+  std::map<std::pair<std::vector<int32_t>, uint64_t>, int> index;
   for (int g = 0; g < G; ++g) {
     std::vector<int32_t> p(gh.perm.begin() + (size_t)g * nb, gh.perm.begin() + (size_t)(g + 1) * nb);
-    if (index.count(p)) fail(EDHIP_ERR_ARGUMENT, "repeated symmetry operation");
-    index[p] = g;
+    auto key = std::make_pair(p, gh.xr[g]);
+    if (index.count(key)) fail(EDHIP_ERR_ARGUMENT, "repeated symmetry operation");
+    index[key] = g;
   }
   gh.inv.assign(G, -1);
   std::vector<int32_t> comp(nb);
   for (int g1 = 0; g1 < G; ++g1)
     for (int g2 = 0; g2 < G; ++g2) {
       for (int i = 0; i < nb; ++i) comp[i] = gh.perm[(size_t)g1 * nb + gh.perm[(size_t)g2 * nb + i]];
-      auto it = index.find(comp);
+      auto it = index.find(std::make_pair(comp, gh.xr[g1] ^ gh.xr[g2]));
       if (it == index.end()) fail(EDHIP_ERR_ARGUMENT, "symmetry operations must form a group");
       if (it->second == 0) gh.inv[g1] = g2;
     }
-  // canonicaliser choice
+  // canonicaliser choice: translations of a torus (possibly times the global
+  // flip: then the unflipped half must be the translations and the flipped
+  // half the same permutations with the flip)
+  std::vector<int32_t> tsub;  // indices of the unflipped elements
+  for (int g = 0; g < G; ++g)
+    if (gh.xr[g] == 0) tsub.push_back(g);
+  const int GT = (int)tsub.size();
+  std::vector<int32_t> tperm((size_t)GT * nb);
+  for (int i = 0; i < GT; ++i)
+    std::copy(gh.perm.begin() + (size_t)tsub[i] * nb, gh.perm.begin() + (size_t)(tsub[i] + 1) * nb,
+              tperm.begin() + (size_t)i * nb);
+  std::vector<int32_t> flip_of(G, -1);
+  bool flip_ok = !gh.has_xor;
+  if (gh.has_xor && 2 * GT == G) {
+    flip_ok = true;
+    for (int i = 0; i < GT && flip_ok; ++i) {
+      const int t = tsub[i];
+      std::vector<int32_t> p(gh.perm.begin() + (size_t)t * nb, gh.perm.begin() + (size_t)(t + 1) * nb);
+      auto it = index.find(std::make_pair(p, full));
+      if (it == index.end()) flip_ok = false;
+      else flip_of[t] = it->second;
+    }
+  }
   std::vector<int32_t> g_of_ab;
   int bestLx = 0, bestLy = 0;
   std::vector<int32_t> best_map;
-  for (int Ly = 1; Ly <= nb; ++Ly) {
+  for (int Ly = 1; Ly <= nb && flip_ok; ++Ly) {
     if (nb % Ly) continue;
     int Lx = nb / Ly;
     if (Lx < 2) continue;
-    if (!match_torus(nb, G, gh.perm, Lx, Ly, g_of_ab)) continue;
+    if (!match_torus(nb, GT, tperm, Lx, Ly, g_of_ab)) continue;
+    for (auto& v : g_of_ab) v = tsub[v];  // indices into the whole group
     bool better = false;
     if (bestLx == 0)
       better = true;
@@ -426,6 +464,7 @@ void build_group(GroupHost& gh, int nb, double tol, hipStream_t st) {
   }
   gh.mode = (bestLx > 0) ? kGroupTorus : kGroupGeneric;
   if (G == 1) gh.mode = kGroupGeneric;
+  gh.flip_all = gh.mode == kGroupTorus && gh.has_xor;
 
   // phases, triviality, 1/sqrt(N)
   std::vector<double> pr(G), pi(G), isq(G + 1, 0.0), sqc(G + 1, 0.0);
@@ -449,6 +488,8 @@ void build_group(GroupHost& gh, int nb, double tol, hipStream_t st) {
   d.unit_phases = 1;
   for (int h = 0; h < G; ++h)
     if (pr[h] != 1.0 || pi[h] != 0.0) d.unit_phases = 0;
+  d.has_xor = gh.has_xor ? 1 : 0;
+  d.flip_all = gh.flip_all ? 1 : 0;
   d.full_mask = (nb >= 64) ? ~0ull : ((1ull << nb) - 1);
   GroupLdsLayout& L = gh.lay;
   L = GroupLdsLayout{};
@@ -480,6 +521,27 @@ void build_group(GroupHost& gh, int nb, double tol, hipStream_t st) {
   place(L.off_sqcnt, L.bytes_invsqrt);
   place(L.off_inv, L.bytes_inv);
   place(L.off_trivial, L.bytes_trivial);
+  if (gh.has_xor && gh.mode == kGroupGeneric) {
+    upload(gh.xr_d, gh.xr, st);
+    d.xr = gh.xr_d.as<uint64_t>();
+    L.bytes_xr = (uint32_t)(G * 8);
+    place(L.off_xr, L.bytes_xr);
+  }
+  if (gh.flip_all) {
+    std::vector<int32_t> fo(G, 0), ivf(G, 0);
+    for (int e = 0; e < G; ++e)
+      if (flip_of[e] >= 0) {
+        fo[e] = flip_of[e];
+        ivf[e] = gh.inv[flip_of[e]];
+      }
+    upload(gh.flip_of_d, fo, st);
+    upload(gh.inv_flip_d, ivf, st);
+    d.flip_of = gh.flip_of_d.as<int32_t>();
+    d.inv_flip = gh.inv_flip_d.as<int32_t>();
+    L.bytes_flip = (uint32_t)(G * 4);
+    place(L.off_flip_of, L.bytes_flip);
+    place(L.off_inv_flip, L.bytes_flip);
+  }
 
   if (gh.mode == kGroupTorus) {
     gh.Lx = bestLx;
@@ -629,6 +691,24 @@ void check_space(const edhip_space* s) {
   if (!s) fail(EDHIP_ERR_ARGUMENT, "null space");
 }
 
+// A group with the global flip needs a parent closed under it: a popcount
+// sector with p allowed iff n_bits - p is, or a list equal to its complement.
+void check_flip_closed(const edhip_space* p) {
+  const uint64_t full = (p->n_bits >= 64) ? ~0ull : ((1ull << p->n_bits) - 1ull);
+  if (p->is_sector) {
+    for (int q : p->popcounts)
+      if (std::find(p->popcounts.begin(), p->popcounts.end(), p->n_bits - q) == p->popcounts.end())
+        fail(EDHIP_ERR_ARGUMENT, "the global bit flip maps the sector outside itself");
+    return;
+  }
+  p->materialize();
+  std::vector<uint64_t> k((size_t)p->dim);
+  if (p->dim) HIP_OR_THROW(hipMemcpy(k.data(), p->keys.p, (size_t)p->dim * 8, hipMemcpyDeviceToHost));
+  for (int64_t i = 0; i < p->dim; ++i)
+    if ((k[(size_t)i] ^ full) != k[(size_t)(p->dim - 1 - i)])
+      fail(EDHIP_ERR_ARGUMENT, "the global bit flip maps the basis outside itself");
+}
+
 OpDev make_opdev(edhip_op* op, bool transpose) {
   const edhip_space* sp = op->space;
   OpDev d{};
@@ -758,7 +838,7 @@ bool fast_path_usable(const edhip_op* op) {
   if (!op->pairs_ok) return false;
   const GroupHost* gh = op->space->group.get();
   if (!gh) return true;
-  return gh->mode == kGroupTorus && gh->dev.minrot_table && gh->Ly <= 8;
+  return gh->mode == kGroupTorus && gh->dev.minrot_table && gh->Ly <= 8 && !gh->flip_all;
 }
 
 FastOpDev make_fast(edhip_op* op, int transpose, const HashDev* hash) {
@@ -1316,6 +1396,12 @@ int edhip_space_from_list(edhip_ctx* ctx, int n_bits, int64_t n, const uint64_t*
 
 int edhip_space_reduce(edhip_ctx* ctx, const edhip_space* parent, int n_group, const int32_t* bit_perm,
                        const double* chi_re, const double* chi_im, double tol, edhip_space** out) {
+  return edhip_space_reduce_ex(ctx, parent, n_group, bit_perm, nullptr, chi_re, chi_im, tol, out);
+}
+
+int edhip_space_reduce_ex(edhip_ctx* ctx, const edhip_space* parent, int n_group, const int32_t* bit_perm,
+                          const uint64_t* flip_host, const double* chi_re, const double* chi_im, double tol,
+                          edhip_space** out) {
   if (!ctx || !out) return EDHIP_ERR_ARGUMENT;
   *out = nullptr;
   auto sp = std::make_unique<edhip_space>();
@@ -1328,9 +1414,11 @@ int edhip_space_reduce(edhip_ctx* ctx, const edhip_space* parent, int n_group, c
     auto gh = std::make_unique<GroupHost>();
     gh->G = n_group;
     gh->perm.assign(bit_perm, bit_perm + (size_t)n_group * nb);
+    if (flip_host) gh->xr.assign(flip_host, flip_host + n_group);
     gh->chi.resize(n_group);
     for (int g = 0; g < n_group; ++g) gh->chi[g] = {chi_re[g], chi_im[g]};
     build_group(*gh, nb, tol, ctx->stream);
+    if (gh->has_xor) check_flip_closed(parent);
     {
       uint32_t need = gh->lay.total + (uint32_t)((nb + 1) * (nb + 1) * 8);
       if (need > 64 * 1024)

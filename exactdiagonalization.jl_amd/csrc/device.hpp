@@ -208,7 +208,9 @@ struct GroupDev {
   int minrot_table;     // torus: 1 if the per-row minimal-rotation table is used
   int pair_table;       // torus: 1 if the adjacent-row-pair table is used (Lx <= 6, 2 <= Ly <= 8)
   int unit_phases;      // 1 if every phase is exactly 1 (all characters 1)
-  int pad3_;
+  int has_xor;          // some element XORs a mask into the permuted word (global spin flip)
+  int flip_all;         // torus: the group is the translations times the global flip
+  int pad4_;
   uint64_t row_mask;    // torus: (1 << Lx) - 1
   uint64_t full_mask;   // (1 << nb) - 1
   const uint64_t* nib;  // generic: G * n_nib * 16 images of 4-bit chunks
@@ -220,6 +222,9 @@ struct GroupDev {
   const uint64_t* rot_lo;  // torus: Lx entries, per-row bits wrapped around
   const int32_t* g_of_ab;  // torus: Lx*Ly -> group index of T(a, b)
   const int32_t* inv;      // G: index of the inverse element
+  const uint64_t* xr;      // G: word XORed after the permutation (has_xor)
+  const int32_t* flip_of;  // torus flip: index of (T, flip) for the translation T (flip_all)
+  const int32_t* inv_flip; // torus flip: inv[flip_of[e]] (flip_all)
   const uint8_t* trivial;  // G: |chi(g) - 1| <= tol
   const double* ph_re;     // G: phase[h] = conj(chi(inv(h)))
   const double* ph_im;
@@ -241,6 +246,9 @@ struct GroupLds {
   const int32_t* g_of_ab;
   const int32_t* inv;
   const uint8_t* trivial;
+  const uint64_t* xr;
+  const int32_t* flip_of;
+  const int32_t* inv_flip;
 };
 
 __device__ __forceinline__ uint64_t perm_apply_nib(const uint64_t* tab, int n_nib, uint64_t b) {
@@ -289,9 +297,11 @@ __device__ __forceinline__ void row_minrot(const GroupDev& g, const GroupLds& L,
 // unless one of them has several such candidates.  Same s, h and cnt as the
 // generic torus path below.
 // LXC, LYC > 0 fix the torus shape at compile time; 0 reads it from g.
+// invt: the inverse-index table used to break ties (L.inv, or L.inv_flip for
+// the flipped half of a translation x flip group).
 template <int LXC = 0, int LYC = 0>
-__device__ __forceinline__ void canonicalize_torus_pairs(const GroupDev& g, const GroupLds& L, uint64_t b, uint64_t& s,
-                                                         int& h, int& cnt) {
+__device__ __forceinline__ void canonicalize_torus_pairs_t(const GroupDev& g, const GroupLds& L, const int32_t* invt,
+                                                           uint64_t b, uint64_t& s, int& h, int& cnt) {
   const int Lx = LXC ? LXC : g.Lx;
   const int Ly = LYC ? LYC : g.Ly;
   const int NB = LXC ? LXC * LYC : g.nb;
@@ -331,13 +341,84 @@ __device__ __forceinline__ void canonicalize_torus_pairs(const GroupDev& g, cons
     if (eq) {  // a tie (s has a nontrivial stabiliser): keep the largest inverse index
       const int e_new = L.g_of_ab[a + Lx * (Ly - 1 - (bit >> 3))];
       const int e_cur = L.g_of_ab[(win & 7) + Lx * (Ly - 1 - (win >> 3))];
-      if (L.inv[e_new] > L.inv[e_cur]) win = bit;
+      if (invt[e_new] > invt[e_cur]) win = bit;
     }
     cnt = lt ? 1 : cnt + (eq ? 1 : 0);
     win = lt ? bit : win;
     s = lt ? img : s;
   }
   h = L.g_of_ab[(win & 7) + Lx * (Ly - 1 - (win >> 3))];
+}
+
+// Translations times the global flip: the flipped images of b are the
+// translation images of ~b, reached by the elements flip_of[e].  Combines
+// the two halves (s, tie-break by the inverse index, stabiliser size).
+__device__ __forceinline__ void combine_flip(const GroupLds& L, uint64_t s1, int h1, int c1, uint64_t s2, int e2, int c2,
+                                             uint64_t& s, int& h, int& cnt) {
+  const int h2 = L.flip_of[e2];
+  if (s2 < s1) {
+    s = s2;
+    h = h2;
+    cnt = c2;
+  } else if (s1 < s2) {
+    s = s1;
+    h = h1;
+    cnt = c1;
+  } else {
+    s = s1;
+    cnt = c1 + c2;
+    h = L.inv[h2] > L.inv[h1] ? h2 : h1;
+  }
+}
+
+template <int LXC = 0, int LYC = 0>
+__device__ __forceinline__ void canonicalize_torus_pairs(const GroupDev& g, const GroupLds& L, uint64_t b, uint64_t& s,
+                                                         int& h, int& cnt) {
+  canonicalize_torus_pairs_t<LXC, LYC>(g, L, L.inv, b, s, h, cnt);
+  if (g.flip_all) {
+    const uint64_t FULL = LXC ? ((1ull << (LXC * LYC)) - 1ull) : g.full_mask;
+    uint64_t s2;
+    int e2, c2;
+    canonicalize_torus_pairs_t<LXC, LYC>(g, L, L.inv_flip, b ^ FULL, s2, e2, c2);
+    combine_flip(L, s, h, cnt, s2, e2, c2, s, h, cnt);
+  }
+}
+
+// Translation part of the torus canonicaliser without the row-pair table.
+__device__ __forceinline__ void canonicalize_torus_rows(const GroupDev& g, const GroupLds& L, const int32_t* invt,
+                                                        uint64_t b, uint64_t& s, int& h, int& cnt) {
+  // pass 1: the smallest possible top row
+  uint64_t best_top = ~0ull;
+  for (int y = 0; y < g.Ly; ++y) {
+    uint64_t mv, am;
+    row_minrot(g, L, (b >> (y * g.Lx)) & g.row_mask, mv, am);
+    best_top = mv < best_top ? mv : best_top;
+  }
+  // pass 2: full images of the candidates reaching it
+  s = ~0ull;
+  h = 0;
+  cnt = 0;
+  for (int y = 0; y < g.Ly; ++y) {
+    uint64_t mv, am;
+    row_minrot(g, L, (b >> (y * g.Lx)) & g.row_mask, mv, am);
+    if (mv != best_top) continue;
+    int bb = g.Ly - 1 - y;  // this row lands on top under T(a, bb)
+    uint64_t shifted = rows_shift(b, bb, g.Lx, g.nb, g.full_mask);
+    while (am) {
+      int a = __builtin_ctzll(am);
+      am &= am - 1;
+      uint64_t img = rows_rotl(shifted, a, g.Lx, L.rot_hi, L.rot_lo);
+      int e = L.g_of_ab[a + g.Lx * bb];
+      if (img < s) {
+        s = img;
+        h = e;
+        cnt = 1;
+      } else if (img == s) {
+        ++cnt;
+        if (invt[e] > invt[h]) h = e;
+      }
+    }
+  }
 }
 
 // Canonical form s = min_h h(b), an element h with h(b) = s (among ties, the
@@ -352,6 +433,7 @@ __device__ __forceinline__ void canonicalize(const GroupDev& g, const GroupLds& 
     const int stride = g.n_nib << 4;
     for (int e = 1; e < g.G; ++e) {
       uint64_t img = perm_apply_nib(L.nib + e * stride, g.n_nib, b);
+      if (g.has_xor) img ^= L.xr[e];
       if (img < s) {
         s = img;
         h = e;
@@ -368,37 +450,12 @@ __device__ __forceinline__ void canonicalize(const GroupDev& g, const GroupLds& 
       canonicalize_torus_pairs(g, L, b, s, h, cnt);
       return;
     }
-    // pass 1: the smallest possible top row
-    uint64_t best_top = ~0ull;
-    for (int y = 0; y < g.Ly; ++y) {
-      uint64_t mv, am;
-      row_minrot(g, L, (b >> (y * g.Lx)) & g.row_mask, mv, am);
-      best_top = mv < best_top ? mv : best_top;
-    }
-    // pass 2: full images of the candidates reaching it
-    s = ~0ull;
-    h = 0;
-    cnt = 0;
-    for (int y = 0; y < g.Ly; ++y) {
-      uint64_t mv, am;
-      row_minrot(g, L, (b >> (y * g.Lx)) & g.row_mask, mv, am);
-      if (mv != best_top) continue;
-      int bb = g.Ly - 1 - y;  // this row lands on top under T(a, bb)
-      uint64_t shifted = rows_shift(b, bb, g.Lx, g.nb, g.full_mask);
-      while (am) {
-        int a = __builtin_ctzll(am);
-        am &= am - 1;
-        uint64_t img = rows_rotl(shifted, a, g.Lx, L.rot_hi, L.rot_lo);
-        int e = L.g_of_ab[a + g.Lx * bb];
-        if (img < s) {
-          s = img;
-          h = e;
-          cnt = 1;
-        } else if (img == s) {
-          ++cnt;
-          if (L.inv[e] > L.inv[h]) h = e;
-        }
-      }
+    canonicalize_torus_rows(g, L, L.inv, b, s, h, cnt);
+    if (g.flip_all) {
+      uint64_t s2;
+      int e2, c2;
+      canonicalize_torus_rows(g, L, L.inv_flip, b ^ g.full_mask, s2, e2, c2);
+      combine_flip(L, s, h, cnt, s2, e2, c2, s, h, cnt);
     }
     return;
   }
@@ -410,12 +467,51 @@ __device__ __forceinline__ void canonicalize(const GroupDev& g, const GroupLds& 
 // Representative test of symmetry_reduce (symmetry_reduce_generic.jl:56-69):
 // b is kept iff no image is smaller and every element fixing b has a trivial
 // character.  On success orbit = G / |Stab(b)|.
+// Translation images of w against a candidate representative b: false if
+// one is smaller than b; for each image equal to b, the element (map[e], or
+// e when map is null) must have a trivial character, and stab counts it.
+__device__ __forceinline__ bool torus_scan(const GroupDev& g, const GroupLds& L, uint64_t w, uint64_t b,
+                                           const int32_t* map, int& stab) {
+  const uint64_t top = (b >> ((g.Ly - 1) * g.Lx)) & g.row_mask;
+  uint64_t best_top = ~0ull;
+  for (int y = 0; y < g.Ly; ++y) {
+    uint64_t mv, am;
+    row_minrot(g, L, (w >> (y * g.Lx)) & g.row_mask, mv, am);
+    best_top = mv < best_top ? mv : best_top;
+  }
+  if (best_top < top) return false;
+  if (best_top > top) return true;  // every image is larger than b
+  for (int y = 0; y < g.Ly; ++y) {
+    uint64_t mv, am;
+    row_minrot(g, L, (w >> (y * g.Lx)) & g.row_mask, mv, am);
+    if (mv != best_top) continue;
+    int bb = g.Ly - 1 - y;
+    uint64_t shifted = rows_shift(w, bb, g.Lx, g.nb, g.full_mask);
+    while (am) {
+      int a = __builtin_ctzll(am);
+      am &= am - 1;
+      uint64_t img = rows_rotl(shifted, a, g.Lx, L.rot_hi, L.rot_lo);
+      if (img < b) return false;
+      if (img == b) {
+        const int e = L.g_of_ab[a + g.Lx * bb];
+        if (!L.trivial[map ? map[e] : e]) return false;
+        ++stab;
+      }
+    }
+  }
+  return true;
+}
+
+// Representative test of symmetry_reduce (symmetry_reduce_generic.jl:56-69):
+// b is kept iff no image is smaller and every element fixing b has a trivial
+// character.  On success orbit = G / |Stab(b)|.
 __device__ __forceinline__ bool is_representative(const GroupDev& g, const GroupLds& L, uint64_t b, int& orbit) {
   if (g.mode == kGroupGeneric) {
     int stab = 1;
     const int stride = g.n_nib << 4;
     for (int e = 1; e < g.G; ++e) {
       uint64_t img = perm_apply_nib(L.nib + e * stride, g.n_nib, b);
+      if (g.has_xor) img ^= L.xr[e];
       if (img < b) return false;
       if (img == b) {
         if (!L.trivial[e]) return false;
@@ -426,32 +522,9 @@ __device__ __forceinline__ bool is_representative(const GroupDev& g, const Group
     return true;
   }
   if (g.mode == kGroupTorus) {
-    uint64_t top = (b >> ((g.Ly - 1) * g.Lx)) & g.row_mask;
-    uint64_t best_top = ~0ull;
-    for (int y = 0; y < g.Ly; ++y) {
-      uint64_t mv, am;
-      row_minrot(g, L, (b >> (y * g.Lx)) & g.row_mask, mv, am);
-      best_top = mv < best_top ? mv : best_top;
-    }
-    if (best_top < top) return false;
     int stab = 0;
-    for (int y = 0; y < g.Ly; ++y) {
-      uint64_t mv, am;
-      row_minrot(g, L, (b >> (y * g.Lx)) & g.row_mask, mv, am);
-      if (mv != best_top) continue;
-      int bb = g.Ly - 1 - y;
-      uint64_t shifted = rows_shift(b, bb, g.Lx, g.nb, g.full_mask);
-      while (am) {
-        int a = __builtin_ctzll(am);
-        am &= am - 1;
-        uint64_t img = rows_rotl(shifted, a, g.Lx, L.rot_hi, L.rot_lo);
-        if (img < b) return false;
-        if (img == b) {
-          if (!L.trivial[L.g_of_ab[a + g.Lx * bb]]) return false;
-          ++stab;
-        }
-      }
-    }
+    if (!torus_scan(g, L, b, b, nullptr, stab)) return false;
+    if (g.flip_all && !torus_scan(g, L, b ^ g.full_mask, b, L.flip_of, stab)) return false;
     orbit = g.G / stab;
     return true;
   }
@@ -464,8 +537,8 @@ __device__ __forceinline__ bool is_representative(const GroupDev& g, const Group
 // ----------------------------------------------------------------------------
 struct GroupLdsLayout {
   uint32_t off_nib, off_minrot, off_pair, off_rot_hi, off_rot_lo, off_g_of_ab, off_inv, off_trivial;
-  uint32_t off_ph_re, off_ph_im, off_invsqrt, off_sqcnt;
-  uint32_t bytes_nib, bytes_minrot, bytes_pair, bytes_rot, bytes_g_of_ab, bytes_inv, bytes_trivial;
+  uint32_t off_ph_re, off_ph_im, off_invsqrt, off_sqcnt, off_xr, off_flip_of, off_inv_flip;
+  uint32_t bytes_nib, bytes_minrot, bytes_pair, bytes_rot, bytes_g_of_ab, bytes_inv, bytes_trivial, bytes_xr, bytes_flip;
   uint32_t bytes_ph, bytes_invsqrt;
   uint32_t total;
 };
@@ -492,6 +565,9 @@ __device__ __forceinline__ GroupLds stage_group(const GroupDev& g, const GroupLd
   lds_copy(lds + lay.off_ph_im, g.ph_im, lay.bytes_ph);
   lds_copy(lds + lay.off_invsqrt, g.invsqrt, lay.bytes_invsqrt);
   lds_copy(lds + lay.off_sqcnt, g.sqcnt, lay.bytes_invsqrt);
+  lds_copy(lds + lay.off_xr, g.xr, lay.bytes_xr);
+  lds_copy(lds + lay.off_flip_of, g.flip_of, lay.bytes_flip);
+  lds_copy(lds + lay.off_inv_flip, g.inv_flip, lay.bytes_flip);
   L.sqcnt = (const double*)(lds + lay.off_sqcnt);
   L.ph_re = (const double*)(lds + lay.off_ph_re);
   L.ph_im = (const double*)(lds + lay.off_ph_im);
@@ -504,6 +580,9 @@ __device__ __forceinline__ GroupLds stage_group(const GroupDev& g, const GroupLd
   L.g_of_ab = (const int32_t*)(lds + lay.off_g_of_ab);
   L.inv = (const int32_t*)(lds + lay.off_inv);
   L.trivial = (const uint8_t*)(lds + lay.off_trivial);
+  L.xr = (const uint64_t*)(lds + lay.off_xr);
+  L.flip_of = (const int32_t*)(lds + lay.off_flip_of);
+  L.inv_flip = (const int32_t*)(lds + lay.off_inv_flip);
   return L;
 }
 

@@ -193,9 +193,16 @@ __global__ __launch_bounds__(kBlock) void k_vec_reduce(const uint64_t* __restric
       const uint64_t shifted = rows_shift(s, bb, g.Lx, g.nb, g.full_mask);
       for (int a = 0; a < g.Lx; ++a) add(L.g_of_ab[a + g.Lx * bb], rows_rotl(shifted, a, g.Lx, L.rot_hi, L.rot_lo));
     }
+    if (g.flip_all) {  // (T, flip)(s) = T(~s)
+      for (int bb = 0; bb < g.Ly; ++bb) {
+        const uint64_t shifted = rows_shift(s ^ g.full_mask, bb, g.Lx, g.nb, g.full_mask);
+        for (int a = 0; a < g.Lx; ++a)
+          add(L.flip_of[L.g_of_ab[a + g.Lx * bb]], rows_rotl(shifted, a, g.Lx, L.rot_hi, L.rot_lo));
+      }
+    }
   } else {
     const int stride = g.n_nib << 4;
-    for (int e = 0; e < g.G; ++e) add(e, perm_apply_nib(L.nib + e * stride, g.n_nib, s));
+    for (int e = 0; e < g.G; ++e) add(e, perm_apply_nib(L.nib + e * stride, g.n_nib, s) ^ (g.has_xor ? L.xr[e] : 0ull));
   }
   small[r] = acc;
 }

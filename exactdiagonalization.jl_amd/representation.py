@@ -153,6 +153,8 @@ class SitePermutation:
         return len(self.map)
 
     def __mul__(self, other):  # (self * other)(i) = self(other(i))
+        if isinstance(other, (GlobalBitFlip, DirectProductOperation)):
+            return DirectProductOperation(self, other)
         return SitePermutation([self.map[other.map[i] - 1] for i in range(len(self.map))])
 
     def __eq__(self, other):
@@ -165,12 +167,113 @@ class SitePermutation:
         return f"SitePermutation({self.map})"
 
 
-def symmetry_apply(hs, perm: SitePermutation, x):
+class GlobalBitFlip:
+    """GlobalBitFlip(value=False): flips every bit of the word when value is
+    true (bitflipsymmetry.jl:8-38): the spin flip of a spin-1/2 system."""
+
+    def __init__(self, value: bool = False):
+        self.value = bool(value)
+
+    def __mul__(self, other):
+        if isinstance(other, GlobalBitFlip):
+            return GlobalBitFlip(self.value ^ other.value)
+        return DirectProductOperation(self, other)
+
+    def __eq__(self, other):
+        return isinstance(other, GlobalBitFlip) and self.value == other.value
+
+    def __hash__(self):
+        return hash(("GlobalBitFlip", self.value))
+
+    def isidentity(self) -> bool:
+        return not self.value
+
+    def __repr__(self):
+        return f"GlobalBitFlip({self.value})"
+
+
+class DirectProductOperation:
+    """A site permutation combined with a global bit flip (the two commute);
+    symmetry_apply.jl:65-76 applies the factors right to left.  Kept in the
+    normal form (permutation, flip)."""
+
+    def __init__(self, *ops):
+        perm, flip = None, False
+        for op in ops:
+            if isinstance(op, DirectProductOperation):
+                p, f = op.perm, op.flip
+            elif isinstance(op, GlobalBitFlip):
+                p, f = None, op.value
+            elif isinstance(op, SitePermutation):
+                p, f = op, False
+            else:
+                raise ArgumentError(f"unsupported symmetry operation {op!r}")
+            if p is not None:
+                perm = p if perm is None else perm * p
+            flip ^= f
+        self.perm, self.flip = perm, flip
+
+    def __mul__(self, other):
+        return DirectProductOperation(self, other)
+
+    def __eq__(self, other):
+        return isinstance(other, DirectProductOperation) and self.perm == other.perm and self.flip == other.flip
+
+    def __hash__(self):
+        return hash((self.perm, self.flip))
+
+    def __repr__(self):
+        return f"DirectProductOperation({self.perm!r}, GlobalBitFlip({self.flip}))"
+
+
+def _perm_and_flip(hs, op):
+    """(bit permutation, xor word) of a symmetry operation."""
+    hs = basespace(hs)
+    nb = hs.bitwidth()
+    ident = SitePermutation(list(range(1, len(hs.sites) + 1)))
+    if isinstance(op, SitePermutation):
+        return bit_permutation(hs, op), 0
+    if isinstance(op, GlobalBitFlip):
+        return bit_permutation(hs, ident), ((1 << nb) - 1) if op.value else 0
+    if isinstance(op, DirectProductOperation):
+        perm = op.perm if op.perm is not None else ident
+        return bit_permutation(hs, perm), ((1 << nb) - 1) if op.flip else 0
+    raise ArgumentError(f"unsupported symmetry operation {op!r}")
+
+
+def make_product_irrep(*irreps):
+    """Direct product of one-dimensional irreps given as lists of
+    (operation, amplitude): every combination, operations multiplied and
+    amplitudes multiplied, the identity first when each factor has it first
+    (the role of LatticeTools' make_product_irrep in
+    examples/spinhalf_triangular.jl:87-96)."""
+    out = [(None, 1.0)]
+    for irrep in irreps:
+        items = list(irrep.symops_and_amplitudes()) if hasattr(irrep, "symops_and_amplitudes") else list(irrep)
+        out = [(op if acc is None else acc * op, a * b) for acc, a in out for op, b in items]
+    return out
+
+
+def symmetry_apply(hs, perm, x):
     """symmetry_apply(hs, ::SitePermutation, bitrep | PureOperator | SumOperator):
-    symmetry_apply.jl:54-106.  Host-side (used to validate groups and operators)."""
+    symmetry_apply.jl:54-106; GlobalBitFlip and DirectProductOperation on a
+    word (bitflipsymmetry.jl:26-38, symmetry_apply.jl:65-76).  Host-side (used
+    to validate groups and operators)."""
     from .operators import PureOperator, SumOperator
 
     hs = basespace(hs)
+    if isinstance(perm, (GlobalBitFlip, DirectProductOperation)):
+        if isinstance(x, (NullOperator, PureOperator, SumOperator)):
+            raise ArgumentError("a bit flip acts on words only")
+        b = int(x)
+        nb = hs.bitwidth()
+        if b >> nb:
+            raise ArgumentError("binary representation too small")
+        if isinstance(perm, DirectProductOperation):
+            if perm.flip:
+                b = ((1 << nb) - 1) & ~b
+            return symmetry_apply(hs, perm.perm, b) if perm.perm is not None else (b, 1)
+        return ((((1 << nb) - 1) & ~b) if perm.value else b, 1)
     if isinstance(x, NullOperator):
         return x
     if isinstance(x, PureOperator):
@@ -289,14 +392,17 @@ def symmetry_reduce(hsr, symops_and_amplitudes, tol: float = RTOL_DEFAULT):
     ops = _symops(symops_and_amplitudes)
     if not ops:
         raise ArgumentError("empty symmetry group")
-    perms = np.ascontiguousarray(np.stack([bit_permutation(hsr.hilbert_space, p) for p, _ in ops]), dtype=np.int32)
+    pf = [_perm_and_flip(hsr.hilbert_space, p) for p, _ in ops]
+    perms = np.ascontiguousarray(np.stack([p for p, _ in pf]), dtype=np.int32)
+    flips = np.ascontiguousarray(np.array([f for _, f in pf], dtype=np.uint64))
     chi = np.array([complex(a) for _, a in ops], dtype=np.complex128)
     cre, cim = np.ascontiguousarray(chi.real), np.ascontiguousarray(chi.imag)
     h = ctypes.c_void_p()
     ctx = hsr.ctx
     ctx.check(
-        _lib.lib().edhip_space_reduce(
-            ctx.handle, hsr._handle, len(ops), ptr(perms, ctypes.c_int32), ptr(cre, ctypes.c_double),
+        _lib.lib().edhip_space_reduce_ex(
+            ctx.handle, hsr._handle, len(ops), ptr(perms, ctypes.c_int32),
+            ptr(flips, ctypes.c_uint64) if flips.any() else None, ptr(cre, ctypes.c_double),
             ptr(cim, ctypes.c_double), float(tol), ctypes.byref(h),
         )
     )

@@ -96,6 +96,17 @@ int edhip_space_from_list(edhip_ctx* ctx, int n_bits, int64_t n, const uint64_t*
 int edhip_space_reduce(edhip_ctx* ctx, const edhip_space* parent, int n_group, const int32_t* bit_perm_host,
                        const double* chi_re_host, const double* chi_im_host, double tol, edhip_space** out);
 
+/* symmetry_reduce with operations that may include the global spin flip:
+ * GlobalBitFlip (src/Symmetry/bitflipsymmetry.jl:8-38) composed with a site
+ * permutation (DirectProductOperation, symmetry_apply.jl:65-76), i.e.
+ * g(b) = perm_g(b) ^ flip[g] with flip[g] = 0 or all n_bits bits.
+ * flip_host: n_group words or NULL (then as edhip_space_reduce).  The parent
+ * must be closed under the flip (a sector whose popcounts p and n_bits - p
+ * are allowed together, or a basis list equal to its complement). */
+int edhip_space_reduce_ex(edhip_ctx* ctx, const edhip_space* parent, int n_group, const int32_t* bit_perm_host,
+                          const uint64_t* flip_host, const double* chi_re_host, const double* chi_im_host,
+                          double tol, edhip_space** out);
+
 void edhip_space_destroy(edhip_space* space);
 /* dimension(hsr) / dimension(rhsr): hilbert_space_representation.jl:90,
  * reduced_hilbert_space_representation.jl:38 */

@@ -69,6 +69,11 @@ def lib():
             i64, u64p, ctypes.c_int, i32p, i32p, ctypes.c_int, i32p, dp, dp, ctypes.c_double,
             u64p, i64p, dp, dp,
         ]
+        L.orc_symmetry_reduce_x.restype = i64
+        L.orc_symmetry_reduce_x.argtypes = [
+            i64, u64p, ctypes.c_int, i32p, i32p, ctypes.c_int, i32p, u64p, dp, dp, ctypes.c_double,
+            u64p, i64p, dp, dp,
+        ]
         opr_args = [i64, u64p, ctypes.c_int, i64, u64p, i64p, dp, dp, i64, u64p, u64p, u64p, dp, dp]
         L.orc_row_iterator.restype = i64
         L.orc_row_iterator.argtypes = opr_args + [i64, i64p, dp, dp]
@@ -217,10 +222,12 @@ class Reduced:
         return int(self.basis_list.size)
 
 
-def symmetry_reduce(parent, n_sites, perms, chars, tol=RTOL_DEFAULT, bitwidths=None):
+def symmetry_reduce(parent, n_sites, perms, chars, tol=RTOL_DEFAULT, bitwidths=None, flips=None):
     """symmetry_reduce_serial(hsr, symops_and_amplitudes; tol) (symmetry_reduce_generic.jl:22-106).
 
     perms: (G, n_sites) 0-based image sites, identity first; chars: complex (G,).
+    flips: optional (G,) words XORed into each permuted word (GlobalBitFlip
+    composed with the permutation, bitflipsymmetry.jl:26-38).
     """
     parent = np.ascontiguousarray(parent, dtype=np.uint64)
     perms = np.ascontiguousarray(perms, dtype=np.int32)
@@ -235,10 +242,12 @@ def symmetry_reduce(parent, n_sites, perms, chars, tol=RTOL_DEFAULT, bitwidths=N
     midx = np.empty(max(n, 1), dtype=np.int64)
     are = np.empty(max(n, 1), dtype=np.float64)
     aim = np.empty(max(n, 1), dtype=np.float64)
-    d = lib().orc_symmetry_reduce(
+    fl = None if flips is None else np.ascontiguousarray(np.asarray(flips, dtype=np.uint64))
+    d = lib().orc_symmetry_reduce_x(
         n, _p(parent, ctypes.c_uint64), n_sites, _p(offsets, ctypes.c_int32), _p(widths, ctypes.c_int32), G,
-        _p(perms, ctypes.c_int32), _p(cre, ctypes.c_double), _p(cim, ctypes.c_double), float(tol),
-        _p(red, ctypes.c_uint64), _p(midx, ctypes.c_int64), _p(are, ctypes.c_double), _p(aim, ctypes.c_double),
+        _p(perms, ctypes.c_int32), None if fl is None else _p(fl, ctypes.c_uint64), _p(cre, ctypes.c_double),
+        _p(cim, ctypes.c_double), float(tol), _p(red, ctypes.c_uint64), _p(midx, ctypes.c_int64),
+        _p(are, ctypes.c_double), _p(aim, ctypes.c_double),
     )
     if d == -1:
         raise ValueError("all amplitudes need to have norm 1")

@@ -78,6 +78,10 @@ uint64_t orc_symmetry_apply(int n_sites, const int32_t* bitoffsets, const int32_
 /* src/Symmetry/symmetry_reduce_generic.jl:22-106.                          */
 /*                                                                          */
 /* maps: n_group x n_sites (0-based images), identity first.               */
+/* flips (orc_symmetry_reduce_x, may be NULL): per element, a word XORed     */
+/* into the permuted word: GlobalBitFlip(true) composed with the permutation */
+/* (bitflipsymmetry.jl:26-38, DirectProductOperation symmetry_apply.jl:65-76;*/
+/* a global flip commutes with every site permutation).                     */
 /* chi:  the amplitudes y of (symop, y); the stored phase is conj(y*sgn)    */
 /*       (:67) with sgn = 1 for site permutations (symmetry_apply.jl:91).   */
 /* Outputs: reduced basis (ascending), mapping index (parent length, -1 if  */
@@ -85,11 +89,11 @@ uint64_t orc_symmetry_apply(int n_sites, const int32_t* bitoffsets, const int32_
 /* Returns the reduced dimension, -1 if some |y| != 1 (ArgumentError, :27-29)*/
 /* or -2 if an image falls outside the parent basis (KeyError at :83).      */
 /* ---------------------------------------------------------------------- */
-int64_t orc_symmetry_reduce(int64_t n_basis, const uint64_t* basis, int n_sites,
-                            const int32_t* bitoffsets, const int32_t* bitwidths, int n_group,
-                            const int32_t* maps, const double* chi_re, const double* chi_im,
-                            double tol, uint64_t* reduced_out, int64_t* mapping_index,
-                            double* amp_re, double* amp_im) {
+int64_t orc_symmetry_reduce_x(int64_t n_basis, const uint64_t* basis, int n_sites,
+                              const int32_t* bitoffsets, const int32_t* bitwidths, int n_group,
+                              const int32_t* maps, const uint64_t* flips, const double* chi_re,
+                              const double* chi_im, double tol, uint64_t* reduced_out,
+                              int64_t* mapping_index, double* amp_re, double* amp_im) {
   const double rtol_default = sqrt(2.220446049250313e-16);
   for (int g = 0; g < n_group; ++g) {
     double a = cabs(chi_re[g] + I * chi_im[g]);
@@ -115,6 +119,7 @@ int64_t orc_symmetry_reduce(int64_t n_basis, const uint64_t* basis, int n_sites,
     int compatible = 1;
     for (int i = 1; i < n_group; ++i) {
       uint64_t bp = orc_symmetry_apply(n_sites, bitoffsets, bitwidths, maps + (size_t)i * n_sites, bvec);
+      if (flips) bp ^= flips[i];
       cplx ampl = chi_re[i] + I * chi_im[i];
       if (bp < bvec) {
         compatible = 0;
@@ -170,6 +175,15 @@ int64_t orc_symmetry_reduce(int64_t n_basis, const uint64_t* basis, int n_sites,
   free(states);
   free(phases);
   return status == 0 ? n_red : status;
+}
+
+int64_t orc_symmetry_reduce(int64_t n_basis, const uint64_t* basis, int n_sites,
+                            const int32_t* bitoffsets, const int32_t* bitwidths, int n_group,
+                            const int32_t* maps, const double* chi_re, const double* chi_im,
+                            double tol, uint64_t* reduced_out, int64_t* mapping_index,
+                            double* amp_re, double* amp_im) {
+  return orc_symmetry_reduce_x(n_basis, basis, n_sites, bitoffsets, bitwidths, n_group, maps, NULL, chi_re,
+                               chi_im, tol, reduced_out, mapping_index, amp_re, amp_im);
 }
 
 /* ---------------------------------------------------------------------- */

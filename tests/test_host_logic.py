@@ -177,3 +177,26 @@ def test_lanczos_default_opts(ed):
     o = _lib.LanczosOpts()
     _lib.lib().edhip_lanczos_default_opts(ctypes.byref(o))
     assert o.nev == 1 and o.max_basis == 120 and o.tol == 1e-10 and o.seed == 20241202
+
+
+def test_global_bit_flip_known(ed):
+    """test_bitflip.jl:4-56: products and images of GlobalBitFlip and of its
+    products with a site permutation (4 spins)."""
+    b0, b1 = ed.GlobalBitFlip(), ed.GlobalBitFlip(True)
+    assert b0 == ed.GlobalBitFlip(False) and b0 != ed.GlobalBitFlip(True)
+    assert b0 * b0 == b0 and b0 * b1 == b1 and b1 * b0 == b1 and b1 * b1 == b0
+    assert b0.isidentity() and not b1.isidentity()
+    hs, _ = ed.spin_half_system(4)
+    p = ed.SitePermutation([2, 3, 4, 1])
+    assert ed.symmetry_apply(hs, b0, 0b0001)[0] == 0b0001
+    assert ed.symmetry_apply(hs, b1, 0b0001)[0] == 0b1110
+    assert ed.symmetry_apply(hs, p * b0, 0b0001)[0] == 0b0010
+    assert ed.symmetry_apply(hs, p * b1, 0b0001)[0] == 0b1101
+    for b in range(16):
+        bp = ed.symmetry_apply(hs, p, b)[0]
+        for f in (b0, b1):
+            assert ed.symmetry_apply(hs, f, bp) == ed.symmetry_apply(hs, p * f, b) == ed.symmetry_apply(hs, f * p, b)
+    # direct products: every (T^a, flip) once, identity first, amplitudes multiplied
+    irr = ed.make_product_irrep([(b0, 1.0), (b1, -1.0)], ed.TranslationIrrep(4, 1))
+    assert len(irr) == 8 and irr[0][1] == 1.0
+    assert len({ed.symmetry_apply(hs, op, 0b0011)[0] for op, _ in irr}) == 4

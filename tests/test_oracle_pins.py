@@ -303,3 +303,27 @@ def test_eigenvalues_brute_force(oracle, golden):
     parent = oracle.sector_basis(L, 0)
     o = oracle.Opr(parent, heisenberg_terms_pauli(chain_bonds(L), weights=[0.25] * L))
     assert np.allclose(oracle.lowest_eigenvalues(o, 4), ev["chain12"], atol=1e-10)
+
+
+@pytest.mark.parametrize("L", [8, 10])
+def test_spectrum_union_with_spin_flip(oracle, L):
+    """Translations x global spin flip (GlobalBitFlip, bitflipsymmetry.jl):
+    the reduced dimensions over (k, parity) add up to C(L, L/2) and the union
+    of the reduced spectra is the Sz=0 spectrum (dense brute force)."""
+    parent = oracle.sector_basis(L, 0)
+    terms = heisenberg_terms_pauli(chain_bonds(L))
+    full = np.linalg.eigvalsh(dense_from_terms(L, parent, terms))
+    perms = np.concatenate([translation_group_perms(L)] * 2)
+    full_mask = (1 << L) - 1
+    flips = np.array([0] * L + [full_mask] * L, np.uint64)
+    ev, dims = [], 0
+    for k in range(L):
+        chi_t = translation_chars(L, 1, k, 0)
+        for parity in (1, -1):
+            chars = np.concatenate([chi_t, parity * chi_t])
+            red = oracle.symmetry_reduce(parent, L, perms, chars, flips=flips)
+            dims += red.dimension
+            if red.dimension:
+                ev.extend(np.linalg.eigvalsh(oracle.Opr(red, terms).dense()))
+    assert dims == parent.size
+    assert np.allclose(np.sort(ev), full, atol=1e-9)
