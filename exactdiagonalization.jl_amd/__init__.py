@@ -15,7 +15,7 @@ from ._lib import (
     default_context,
 )
 from .hilbert import HilbertSpace, HilbertSpaceSector, Site, State, basespace
-from .lattice import TranslationIrrep, chain_bonds, square_bonds, translation_group, translation_perm
+from .lattice import SquareSpaceIrrep, TranslationIrrep, chain_bonds, point_perm, square_bonds, translation_group, translation_perm
 from .operators import (
     NullOperator,
     PureOperator,

@@ -174,10 +174,10 @@ struct GroupHost {
   std::vector<int32_t> inv;
   bool complex_phase = false;
   bool has_xor = false;
-  bool flip_all = false;  // torus translations x global flip
+  int n_cosets = 1;  // torus: cosets T p of the translations
   // device tables
-  DevBuf nib, minrot, pair, rot_hi, rot_lo, g_of_ab, inv_d, trivial, ph_re, ph_im, invsqrt, sqcnt, xr_d, flip_of_d,
-      inv_flip_d;
+  DevBuf nib, minrot, pair, rot_hi, rot_lo, g_of_ab, inv_d, trivial, ph_re, ph_im, invsqrt, sqcnt, xr_d, coset_nib_d,
+      coset_pidx_d, coset_xr_d, coset_kind_d, coset_elem_d, coset_inv_d;
   GroupDev dev{};
   GroupLdsLayout lay{};
 };
@@ -417,38 +417,33 @@ void build_group(GroupHost& gh, int nb, double tol, hipStream_t st) {
       if (it == index.end()) fail(EDHIP_ERR_ARGUMENT, "symmetry operations must form a group");
       if (it->second == 0) gh.inv[g1] = g2;
     }
-  // canonicaliser choice: translations of a torus (possibly times the global
-  // flip: then the unflipped half must be the translations and the flipped
-  // half the same permutations with the flip)
-  std::vector<int32_t> tsub;  // indices of the unflipped elements
-  for (int g = 0; g < G; ++g)
-    if (gh.xr[g] == 0) tsub.push_back(g);
-  const int GT = (int)tsub.size();
-  std::vector<int32_t> tperm((size_t)GT * nb);
-  for (int i = 0; i < GT; ++i)
-    std::copy(gh.perm.begin() + (size_t)tsub[i] * nb, gh.perm.begin() + (size_t)(tsub[i] + 1) * nb,
-              tperm.begin() + (size_t)i * nb);
-  std::vector<int32_t> flip_of(G, -1);
-  bool flip_ok = !gh.has_xor;
-  if (gh.has_xor && 2 * GT == G) {
-    flip_ok = true;
-    for (int i = 0; i < GT && flip_ok; ++i) {
-      const int t = tsub[i];
-      std::vector<int32_t> p(gh.perm.begin() + (size_t)t * nb, gh.perm.begin() + (size_t)(t + 1) * nb);
-      auto it = index.find(std::make_pair(p, full));
-      if (it == index.end()) flip_ok = false;
-      else flip_of[t] = it->second;
+  // canonicaliser choice: if every translation of an Lx x Ly torus (no flip)
+  // is an element, the group is a union of cosets T p and canonicalises as
+  // min over p of the translation-canonical form of p(b); else generic tables.
+  auto find_translations = [&](int Lx, int Ly, std::vector<int32_t>& gab) {
+    gab.assign((size_t)nb, -1);
+    for (int g = 0; g < G; ++g) {
+      if (gh.xr[g] != 0) continue;
+      const int img0 = gh.perm[(size_t)g * nb + 0];
+      const int a = img0 % Lx, b = img0 / Lx;
+      bool ok = true;
+      for (int y = 0; y < Ly && ok; ++y)
+        for (int x = 0; x < Lx && ok; ++x)
+          ok = gh.perm[(size_t)g * nb + x + Lx * y] == ((x + a) % Lx) + Lx * ((y + b) % Ly);
+      if (ok) gab[a + Lx * b] = g;
     }
-  }
+    for (int v : gab)
+      if (v < 0) return false;
+    return true;
+  };
   std::vector<int32_t> g_of_ab;
   int bestLx = 0, bestLy = 0;
   std::vector<int32_t> best_map;
-  for (int Ly = 1; Ly <= nb && flip_ok; ++Ly) {
+  for (int Ly = 1; Ly <= nb && G % nb == 0; ++Ly) {
     if (nb % Ly) continue;
     int Lx = nb / Ly;
     if (Lx < 2) continue;
-    if (!match_torus(nb, GT, tperm, Lx, Ly, g_of_ab)) continue;
-    for (auto& v : g_of_ab) v = tsub[v];  // indices into the whole group
+    if (!find_translations(Lx, Ly, g_of_ab)) continue;
     bool better = false;
     if (bestLx == 0)
       better = true;
@@ -462,9 +457,25 @@ void build_group(GroupHost& gh, int nb, double tol, hipStream_t st) {
       best_map = g_of_ab;
     }
   }
+  // cosets T p, representatives in element order (the identity first)
+  std::vector<int32_t> coset_rep, coset_elem;
+  if (bestLx > 0) {
+    std::vector<char> covered(G, 0);
+    for (int g = 0; g < G; ++g) {
+      if (covered[g]) continue;
+      coset_rep.push_back(g);
+      for (int q = 0; q < nb; ++q) {
+        const int t = best_map[q];
+        for (int i = 0; i < nb; ++i) comp[i] = gh.perm[(size_t)t * nb + gh.perm[(size_t)g * nb + i]];
+        auto it = index.find(std::make_pair(comp, gh.xr[g]));  // T(q) p keeps p's flip
+        if (it == index.end() || covered[it->second]) fail(EDHIP_ERR_ARGUMENT, "symmetry operations must form a group");
+        covered[it->second] = 1;
+        coset_elem.push_back(it->second);
+      }
+    }
+  }
   gh.mode = (bestLx > 0) ? kGroupTorus : kGroupGeneric;
   if (G == 1) gh.mode = kGroupGeneric;
-  gh.flip_all = gh.mode == kGroupTorus && gh.has_xor;
 
   // phases, triviality, 1/sqrt(N)
   std::vector<double> pr(G), pi(G), isq(G + 1, 0.0), sqc(G + 1, 0.0);
@@ -489,7 +500,7 @@ void build_group(GroupHost& gh, int nb, double tol, hipStream_t st) {
   for (int h = 0; h < G; ++h)
     if (pr[h] != 1.0 || pi[h] != 0.0) d.unit_phases = 0;
   d.has_xor = gh.has_xor ? 1 : 0;
-  d.flip_all = gh.flip_all ? 1 : 0;
+  d.n_cosets = 1;
   d.full_mask = (nb >= 64) ? ~0ull : ((1ull << nb) - 1);
   GroupLdsLayout& L = gh.lay;
   L = GroupLdsLayout{};
@@ -517,8 +528,7 @@ void build_group(GroupHost& gh, int nb, double tol, hipStream_t st) {
   L.bytes_trivial = (uint32_t)triv.size();
   place(L.off_ph_re, L.bytes_ph);
   place(L.off_ph_im, L.bytes_ph);
-  place(L.off_invsqrt, L.bytes_invsqrt);
-  place(L.off_sqcnt, L.bytes_invsqrt);
+  place(L.off_sqcnt, L.bytes_invsqrt);  // 1/sqrt(N) itself is read from global memory
   place(L.off_inv, L.bytes_inv);
   place(L.off_trivial, L.bytes_trivial);
   if (gh.has_xor && gh.mode == kGroupGeneric) {
@@ -527,22 +537,6 @@ void build_group(GroupHost& gh, int nb, double tol, hipStream_t st) {
     L.bytes_xr = (uint32_t)(G * 8);
     place(L.off_xr, L.bytes_xr);
   }
-  if (gh.flip_all) {
-    std::vector<int32_t> fo(G, 0), ivf(G, 0);
-    for (int e = 0; e < G; ++e)
-      if (flip_of[e] >= 0) {
-        fo[e] = flip_of[e];
-        ivf[e] = gh.inv[flip_of[e]];
-      }
-    upload(gh.flip_of_d, fo, st);
-    upload(gh.inv_flip_d, ivf, st);
-    d.flip_of = gh.flip_of_d.as<int32_t>();
-    d.inv_flip = gh.inv_flip_d.as<int32_t>();
-    L.bytes_flip = (uint32_t)(G * 4);
-    place(L.off_flip_of, L.bytes_flip);
-    place(L.off_inv_flip, L.bytes_flip);
-  }
-
   if (gh.mode == kGroupTorus) {
     gh.Lx = bestLx;
     gh.Ly = bestLy;
@@ -561,6 +555,64 @@ void build_group(GroupHost& gh, int nb, double tol, hipStream_t st) {
     upload(gh.rot_hi, hi, st);
     upload(gh.rot_lo, lo, st);
     upload(gh.g_of_ab, best_map, st);
+    // coset tables
+    const int nc = (int)coset_rep.size();
+    d.n_cosets = nc;
+    gh.n_cosets = nc;
+    const int n_nib = (nb + 3) / 4;
+    d.n_nib = n_nib;
+    // 4-bit chunk tables of the distinct permutations among the representatives
+    // (a representative and its product with the flip share one)
+    std::vector<uint64_t> cnib, cxr(nc, 0);
+    std::vector<int32_t> ckind(nc, 0), cpidx(nc, 0), cinv((size_t)nc * nb, 0);
+    std::map<std::vector<int32_t>, int> pidx;
+    for (int c = 0; c < nc; ++c) {
+      const int g = coset_rep[c];
+      std::vector<int32_t> pg(gh.perm.begin() + (size_t)g * nb, gh.perm.begin() + (size_t)(g + 1) * nb);
+      bool ident = true;
+      for (int i = 0; i < nb; ++i) ident = ident && pg[i] == i;
+      cxr[c] = gh.xr[g];
+      ckind[c] = ident ? (gh.xr[g] ? 1 : 0) : 2;
+      if (!ident) {
+        auto it = pidx.find(pg);
+        if (it == pidx.end()) {
+          it = pidx.emplace(pg, (int)pidx.size()).first;
+          for (int k = 0; k < n_nib; ++k)
+            for (int v = 0; v < 16; ++v) {
+              uint64_t o = 0;
+              for (int q = 0; q < 4; ++q) {
+                int bit = 4 * k + q;
+                if (bit < nb && ((v >> q) & 1)) o |= 1ull << pg[bit];
+              }
+              cnib.push_back(o);
+            }
+        }
+        cpidx[c] = it->second;
+      }
+      for (int q = 0; q < nb; ++q) cinv[(size_t)c * nb + q] = gh.inv[coset_elem[(size_t)c * nb + q]];
+    }
+    upload(gh.coset_nib_d, cnib, st);
+    upload(gh.coset_pidx_d, cpidx, st);
+    upload(gh.coset_xr_d, cxr, st);
+    upload(gh.coset_kind_d, ckind, st);
+    upload(gh.coset_elem_d, coset_elem, st);
+    upload(gh.coset_inv_d, cinv, st);
+    d.coset_nib = gh.coset_nib_d.as<uint64_t>();
+    d.coset_pidx = gh.coset_pidx_d.as<int32_t>();
+    d.coset_xr = gh.coset_xr_d.as<uint64_t>();
+    d.coset_kind = gh.coset_kind_d.as<int32_t>();
+    d.coset_elem = gh.coset_elem_d.as<int32_t>();
+    d.coset_inv = gh.coset_inv_d.as<int32_t>();
+    L.bytes_cnib = (uint32_t)(cnib.size() * 8);
+    L.bytes_cxr = (uint32_t)(nc * 8);
+    L.bytes_ckind = (uint32_t)(nc * 4);
+    L.bytes_celem = (uint32_t)(nc * nb * 4);
+    if (L.bytes_cnib) place(L.off_cnib, L.bytes_cnib);
+    place(L.off_cpidx, L.bytes_ckind);
+    place(L.off_cxr, L.bytes_cxr);
+    place(L.off_ckind, L.bytes_ckind);
+    place(L.off_celem, L.bytes_celem);
+    place(L.off_cinv, L.bytes_celem);
     d.rot_hi = gh.rot_hi.as<uint64_t>();
     d.rot_lo = gh.rot_lo.as<uint64_t>();
     d.g_of_ab = gh.g_of_ab.as<int32_t>();
@@ -838,7 +890,7 @@ bool fast_path_usable(const edhip_op* op) {
   if (!op->pairs_ok) return false;
   const GroupHost* gh = op->space->group.get();
   if (!gh) return true;
-  return gh->mode == kGroupTorus && gh->dev.minrot_table && gh->Ly <= 8 && !gh->flip_all;
+  return gh->mode == kGroupTorus && gh->dev.minrot_table && gh->Ly <= 8 && gh->n_cosets == 1;
 }
 
 FastOpDev make_fast(edhip_op* op, int transpose, const HashDev* hash) {

@@ -209,7 +209,7 @@ struct GroupDev {
   int pair_table;       // torus: 1 if the adjacent-row-pair table is used (Lx <= 6, 2 <= Ly <= 8)
   int unit_phases;      // 1 if every phase is exactly 1 (all characters 1)
   int has_xor;          // some element XORs a mask into the permuted word (global spin flip)
-  int flip_all;         // torus: the group is the translations times the global flip
+  int n_cosets;         // torus: G / (Lx Ly) cosets T p of the translations (1: translations only)
   int pad4_;
   uint64_t row_mask;    // torus: (1 << Lx) - 1
   uint64_t full_mask;   // (1 << nb) - 1
@@ -223,8 +223,14 @@ struct GroupDev {
   const int32_t* g_of_ab;  // torus: Lx*Ly -> group index of T(a, b)
   const int32_t* inv;      // G: index of the inverse element
   const uint64_t* xr;      // G: word XORed after the permutation (has_xor)
-  const int32_t* flip_of;  // torus flip: index of (T, flip) for the translation T (flip_all)
-  const int32_t* inv_flip; // torus flip: inv[flip_of[e]] (flip_all)
+  // torus cosets: every element is T(pos) p for a translation T(pos), pos =
+  // a + Lx b, and a coset representative p (p = 0: the identity)
+  const uint64_t* coset_nib;   // distinct permutations of the representatives, n_nib x 16 each
+  const int32_t* coset_pidx;   // n_cosets: which of them p uses (kind 2)
+  const uint64_t* coset_xr;    // n_cosets: word XORed after the permutation
+  const int32_t* coset_kind;   // n_cosets: 0 identity, 1 XOR only, 2 permutation (+ XOR)
+  const int32_t* coset_elem;   // n_cosets x (Lx Ly): element index of T(pos) p
+  const int32_t* coset_inv;    // n_cosets x (Lx Ly): inv[coset_elem]
   const uint8_t* trivial;  // G: |chi(g) - 1| <= tol
   const double* ph_re;     // G: phase[h] = conj(chi(inv(h)))
   const double* ph_im;
@@ -236,7 +242,6 @@ struct GroupDev {
 struct GroupLds {
   const double* ph_re;
   const double* ph_im;
-  const double* invsqrt;
   const double* sqcnt;
   const uint64_t* nib;
   const uint32_t* minrot;
@@ -247,8 +252,12 @@ struct GroupLds {
   const int32_t* inv;
   const uint8_t* trivial;
   const uint64_t* xr;
-  const int32_t* flip_of;
-  const int32_t* inv_flip;
+  const uint64_t* coset_nib;
+  const int32_t* coset_pidx;
+  const uint64_t* coset_xr;
+  const int32_t* coset_kind;
+  const int32_t* coset_elem;
+  const int32_t* coset_inv;
 };
 
 __device__ __forceinline__ uint64_t perm_apply_nib(const uint64_t* tab, int n_nib, uint64_t b) {
@@ -297,11 +306,13 @@ __device__ __forceinline__ void row_minrot(const GroupDev& g, const GroupLds& L,
 // unless one of them has several such candidates.  Same s, h and cnt as the
 // generic torus path below.
 // LXC, LYC > 0 fix the torus shape at compile time; 0 reads it from g.
-// invt: the inverse-index table used to break ties (L.inv, or L.inv_flip for
-// the flipped half of a translation x flip group).
+// Translation part.  Returns the smallest translation image s of b, the
+// position pos = a + Lx * bb of a translation T(pos) reaching it (among ties
+// the one with the largest invt[pos]) and cnt = #{T : T(b) = s}.
+// invt: inverse indices of the elements T(pos) p of the coset being scanned.
 template <int LXC = 0, int LYC = 0>
 __device__ __forceinline__ void canonicalize_torus_pairs_t(const GroupDev& g, const GroupLds& L, const int32_t* invt,
-                                                           uint64_t b, uint64_t& s, int& h, int& cnt) {
+                                                           uint64_t b, uint64_t& s, int& pos, int& cnt) {
   const int Lx = LXC ? LXC : g.Lx;
   const int Ly = LYC ? LYC : g.Ly;
   const int NB = LXC ? LXC * LYC : g.nb;
@@ -339,54 +350,21 @@ __device__ __forceinline__ void canonicalize_torus_pairs_t(const GroupDev& g, co
     const uint64_t img = ((t << a) & L.rot_hi[a]) | ((t >> (Lx - a)) & L.rot_lo[a]);
     const bool lt = img < s, eq = img == s;
     if (eq) {  // a tie (s has a nontrivial stabiliser): keep the largest inverse index
-      const int e_new = L.g_of_ab[a + Lx * (Ly - 1 - (bit >> 3))];
-      const int e_cur = L.g_of_ab[(win & 7) + Lx * (Ly - 1 - (win >> 3))];
-      if (invt[e_new] > invt[e_cur]) win = bit;
+      const int p_new = a + Lx * (Ly - 1 - (bit >> 3));
+      const int p_cur = (win & 7) + Lx * (Ly - 1 - (win >> 3));
+      if (invt[p_new] > invt[p_cur]) win = bit;
     }
     cnt = lt ? 1 : cnt + (eq ? 1 : 0);
     win = lt ? bit : win;
     s = lt ? img : s;
   }
-  h = L.g_of_ab[(win & 7) + Lx * (Ly - 1 - (win >> 3))];
+  pos = (win & 7) + Lx * (Ly - 1 - (win >> 3));
 }
 
-// Translations times the global flip: the flipped images of b are the
-// translation images of ~b, reached by the elements flip_of[e].  Combines
-// the two halves (s, tie-break by the inverse index, stabiliser size).
-__device__ __forceinline__ void combine_flip(const GroupLds& L, uint64_t s1, int h1, int c1, uint64_t s2, int e2, int c2,
-                                             uint64_t& s, int& h, int& cnt) {
-  const int h2 = L.flip_of[e2];
-  if (s2 < s1) {
-    s = s2;
-    h = h2;
-    cnt = c2;
-  } else if (s1 < s2) {
-    s = s1;
-    h = h1;
-    cnt = c1;
-  } else {
-    s = s1;
-    cnt = c1 + c2;
-    h = L.inv[h2] > L.inv[h1] ? h2 : h1;
-  }
-}
 
-template <int LXC = 0, int LYC = 0>
-__device__ __forceinline__ void canonicalize_torus_pairs(const GroupDev& g, const GroupLds& L, uint64_t b, uint64_t& s,
-                                                         int& h, int& cnt) {
-  canonicalize_torus_pairs_t<LXC, LYC>(g, L, L.inv, b, s, h, cnt);
-  if (g.flip_all) {
-    const uint64_t FULL = LXC ? ((1ull << (LXC * LYC)) - 1ull) : g.full_mask;
-    uint64_t s2;
-    int e2, c2;
-    canonicalize_torus_pairs_t<LXC, LYC>(g, L, L.inv_flip, b ^ FULL, s2, e2, c2);
-    combine_flip(L, s, h, cnt, s2, e2, c2, s, h, cnt);
-  }
-}
-
-// Translation part of the torus canonicaliser without the row-pair table.
+// Translation part without the row-pair table (same outputs).
 __device__ __forceinline__ void canonicalize_torus_rows(const GroupDev& g, const GroupLds& L, const int32_t* invt,
-                                                        uint64_t b, uint64_t& s, int& h, int& cnt) {
+                                                        uint64_t b, uint64_t& s, int& pos, int& cnt) {
   // pass 1: the smallest possible top row
   uint64_t best_top = ~0ull;
   for (int y = 0; y < g.Ly; ++y) {
@@ -396,7 +374,7 @@ __device__ __forceinline__ void canonicalize_torus_rows(const GroupDev& g, const
   }
   // pass 2: full images of the candidates reaching it
   s = ~0ull;
-  h = 0;
+  pos = 0;
   cnt = 0;
   for (int y = 0; y < g.Ly; ++y) {
     uint64_t mv, am;
@@ -408,17 +386,64 @@ __device__ __forceinline__ void canonicalize_torus_rows(const GroupDev& g, const
       int a = __builtin_ctzll(am);
       am &= am - 1;
       uint64_t img = rows_rotl(shifted, a, g.Lx, L.rot_hi, L.rot_lo);
-      int e = L.g_of_ab[a + g.Lx * bb];
+      const int q = a + g.Lx * bb;
       if (img < s) {
         s = img;
-        h = e;
+        pos = q;
         cnt = 1;
       } else if (img == s) {
         ++cnt;
-        if (invt[e] > invt[h]) h = e;
+        if (invt[q] > invt[pos]) pos = q;
       }
     }
   }
+}
+
+// p(b) for the coset representative p
+__device__ __forceinline__ uint64_t apply_coset(const GroupDev& g, const GroupLds& L, int p, uint64_t b) {
+  const int kind = L.coset_kind[p];
+  if (kind == 0) return b;
+  uint64_t w = b;
+  if (kind == 2) w = perm_apply_nib(L.coset_nib + L.coset_pidx[p] * (g.n_nib << 4), g.n_nib, b);
+  return w ^ L.coset_xr[p];
+}
+
+// Torus group: min over the cosets T p of the translation-canonical form of
+// p(b); s, h (largest inverse index among ties) and cnt = |Stab(s)|.
+template <int LXC = 0, int LYC = 0, bool PAIRS = true>
+__device__ __forceinline__ void canonicalize_torus_cosets(const GroupDev& g, const GroupLds& L, uint64_t b,
+                                                          uint64_t& s, int& h, int& cnt) {
+  const int nT = (LXC ? LXC : g.Lx) * (LYC ? LYC : g.Ly);
+  int pos;
+  if constexpr (PAIRS)
+    canonicalize_torus_pairs_t<LXC, LYC>(g, L, L.coset_inv, b, s, pos, cnt);
+  else
+    canonicalize_torus_rows(g, L, L.coset_inv, b, s, pos, cnt);
+  h = L.coset_elem[pos];
+  for (int p = 1; p < g.n_cosets; ++p) {
+    const uint64_t w = apply_coset(g, L, p, b);
+    uint64_t s2;
+    int pos2, c2;
+    if constexpr (PAIRS)
+      canonicalize_torus_pairs_t<LXC, LYC>(g, L, L.coset_inv + p * nT, w, s2, pos2, c2);
+    else
+      canonicalize_torus_rows(g, L, L.coset_inv + p * nT, w, s2, pos2, c2);
+    const int h2 = L.coset_elem[p * nT + pos2];
+    if (s2 < s) {
+      s = s2;
+      h = h2;
+      cnt = c2;
+    } else if (s2 == s) {
+      cnt += c2;
+      if (L.inv[h2] > L.inv[h]) h = h2;
+    }
+  }
+}
+
+template <int LXC = 0, int LYC = 0>
+__device__ __forceinline__ void canonicalize_torus_pairs(const GroupDev& g, const GroupLds& L, uint64_t b, uint64_t& s,
+                                                         int& h, int& cnt) {
+  canonicalize_torus_cosets<LXC, LYC, true>(g, L, b, s, h, cnt);
 }
 
 // Canonical form s = min_h h(b), an element h with h(b) = s (among ties, the
@@ -446,17 +471,10 @@ __device__ __forceinline__ void canonicalize(const GroupDev& g, const GroupLds& 
     return;
   }
   if (g.mode == kGroupTorus) {
-    if (g.pair_table) {
-      canonicalize_torus_pairs(g, L, b, s, h, cnt);
-      return;
-    }
-    canonicalize_torus_rows(g, L, L.inv, b, s, h, cnt);
-    if (g.flip_all) {
-      uint64_t s2;
-      int e2, c2;
-      canonicalize_torus_rows(g, L, L.inv_flip, b ^ g.full_mask, s2, e2, c2);
-      combine_flip(L, s, h, cnt, s2, e2, c2, s, h, cnt);
-    }
+    if (g.pair_table)
+      canonicalize_torus_cosets<0, 0, true>(g, L, b, s, h, cnt);
+    else
+      canonicalize_torus_cosets<0, 0, false>(g, L, b, s, h, cnt);
     return;
   }
   s = b;
@@ -468,10 +486,10 @@ __device__ __forceinline__ void canonicalize(const GroupDev& g, const GroupLds& 
 // b is kept iff no image is smaller and every element fixing b has a trivial
 // character.  On success orbit = G / |Stab(b)|.
 // Translation images of w against a candidate representative b: false if
-// one is smaller than b; for each image equal to b, the element (map[e], or
-// e when map is null) must have a trivial character, and stab counts it.
+// one is smaller than b; for each image T(pos)(w) equal to b, the element
+// elem[pos] must have a trivial character, and stab counts it.
 __device__ __forceinline__ bool torus_scan(const GroupDev& g, const GroupLds& L, uint64_t w, uint64_t b,
-                                           const int32_t* map, int& stab) {
+                                           const int32_t* elem, int& stab) {
   const uint64_t top = (b >> ((g.Ly - 1) * g.Lx)) & g.row_mask;
   uint64_t best_top = ~0ull;
   for (int y = 0; y < g.Ly; ++y) {
@@ -493,8 +511,7 @@ __device__ __forceinline__ bool torus_scan(const GroupDev& g, const GroupLds& L,
       uint64_t img = rows_rotl(shifted, a, g.Lx, L.rot_hi, L.rot_lo);
       if (img < b) return false;
       if (img == b) {
-        const int e = L.g_of_ab[a + g.Lx * bb];
-        if (!L.trivial[map ? map[e] : e]) return false;
+        if (!L.trivial[elem[a + g.Lx * bb]]) return false;
         ++stab;
       }
     }
@@ -523,8 +540,9 @@ __device__ __forceinline__ bool is_representative(const GroupDev& g, const Group
   }
   if (g.mode == kGroupTorus) {
     int stab = 0;
-    if (!torus_scan(g, L, b, b, nullptr, stab)) return false;
-    if (g.flip_all && !torus_scan(g, L, b ^ g.full_mask, b, L.flip_of, stab)) return false;
+    const int nT = g.Lx * g.Ly;
+    for (int p = 0; p < g.n_cosets; ++p)
+      if (!torus_scan(g, L, apply_coset(g, L, p, b), b, L.coset_elem + p * nT, stab)) return false;
     orbit = g.G / stab;
     return true;
   }
@@ -537,8 +555,10 @@ __device__ __forceinline__ bool is_representative(const GroupDev& g, const Group
 // ----------------------------------------------------------------------------
 struct GroupLdsLayout {
   uint32_t off_nib, off_minrot, off_pair, off_rot_hi, off_rot_lo, off_g_of_ab, off_inv, off_trivial;
-  uint32_t off_ph_re, off_ph_im, off_invsqrt, off_sqcnt, off_xr, off_flip_of, off_inv_flip;
-  uint32_t bytes_nib, bytes_minrot, bytes_pair, bytes_rot, bytes_g_of_ab, bytes_inv, bytes_trivial, bytes_xr, bytes_flip;
+  uint32_t off_ph_re, off_ph_im, off_invsqrt, off_sqcnt, off_xr, off_cnib, off_cpidx, off_cxr, off_ckind, off_celem,
+      off_cinv;
+  uint32_t bytes_nib, bytes_minrot, bytes_pair, bytes_rot, bytes_g_of_ab, bytes_inv, bytes_trivial, bytes_xr;
+  uint32_t bytes_cnib, bytes_cxr, bytes_ckind, bytes_celem;
   uint32_t bytes_ph, bytes_invsqrt;
   uint32_t total;
 };
@@ -563,15 +583,17 @@ __device__ __forceinline__ GroupLds stage_group(const GroupDev& g, const GroupLd
   lds_copy(lds + lay.off_trivial, g.trivial, lay.bytes_trivial);
   lds_copy(lds + lay.off_ph_re, g.ph_re, lay.bytes_ph);
   lds_copy(lds + lay.off_ph_im, g.ph_im, lay.bytes_ph);
-  lds_copy(lds + lay.off_invsqrt, g.invsqrt, lay.bytes_invsqrt);
   lds_copy(lds + lay.off_sqcnt, g.sqcnt, lay.bytes_invsqrt);
   lds_copy(lds + lay.off_xr, g.xr, lay.bytes_xr);
-  lds_copy(lds + lay.off_flip_of, g.flip_of, lay.bytes_flip);
-  lds_copy(lds + lay.off_inv_flip, g.inv_flip, lay.bytes_flip);
+  lds_copy(lds + lay.off_cnib, g.coset_nib, lay.bytes_cnib);
+  lds_copy(lds + lay.off_cpidx, g.coset_pidx, lay.bytes_ckind);
+  lds_copy(lds + lay.off_cxr, g.coset_xr, lay.bytes_cxr);
+  lds_copy(lds + lay.off_ckind, g.coset_kind, lay.bytes_ckind);
+  lds_copy(lds + lay.off_celem, g.coset_elem, lay.bytes_celem);
+  lds_copy(lds + lay.off_cinv, g.coset_inv, lay.bytes_celem);
   L.sqcnt = (const double*)(lds + lay.off_sqcnt);
   L.ph_re = (const double*)(lds + lay.off_ph_re);
   L.ph_im = (const double*)(lds + lay.off_ph_im);
-  L.invsqrt = (const double*)(lds + lay.off_invsqrt);
   L.nib = (const uint64_t*)(lds + lay.off_nib);
   L.minrot = (const uint32_t*)(lds + lay.off_minrot);
   L.pair = (const uint32_t*)(lds + lay.off_pair);
@@ -581,8 +603,12 @@ __device__ __forceinline__ GroupLds stage_group(const GroupDev& g, const GroupLd
   L.inv = (const int32_t*)(lds + lay.off_inv);
   L.trivial = (const uint8_t*)(lds + lay.off_trivial);
   L.xr = (const uint64_t*)(lds + lay.off_xr);
-  L.flip_of = (const int32_t*)(lds + lay.off_flip_of);
-  L.inv_flip = (const int32_t*)(lds + lay.off_inv_flip);
+  L.coset_nib = (const uint64_t*)(lds + lay.off_cnib);
+  L.coset_pidx = (const int32_t*)(lds + lay.off_cpidx);
+  L.coset_xr = (const uint64_t*)(lds + lay.off_cxr);
+  L.coset_kind = (const int32_t*)(lds + lay.off_ckind);
+  L.coset_elem = (const int32_t*)(lds + lay.off_celem);
+  L.coset_inv = (const int32_t*)(lds + lay.off_cinv);
   return L;
 }
 

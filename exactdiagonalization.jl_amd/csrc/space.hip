@@ -189,15 +189,13 @@ __global__ __launch_bounds__(kBlock) void k_vec_reduce(const uint64_t* __restric
     vfma(acc, (R)(L.ph_re[ie] * scale), (R)(-L.ph_im[ie] * scale), large[p]);
   };
   if (g.mode == kGroupTorus) {
-    for (int bb = 0; bb < g.Ly; ++bb) {
-      const uint64_t shifted = rows_shift(s, bb, g.Lx, g.nb, g.full_mask);
-      for (int a = 0; a < g.Lx; ++a) add(L.g_of_ab[a + g.Lx * bb], rows_rotl(shifted, a, g.Lx, L.rot_hi, L.rot_lo));
-    }
-    if (g.flip_all) {  // (T, flip)(s) = T(~s)
+    const int nT = g.Lx * g.Ly;
+    for (int p = 0; p < g.n_cosets; ++p) {  // elements T(pos) p
+      const uint64_t sp = apply_coset(g, L, p, s);
       for (int bb = 0; bb < g.Ly; ++bb) {
-        const uint64_t shifted = rows_shift(s ^ g.full_mask, bb, g.Lx, g.nb, g.full_mask);
+        const uint64_t shifted = rows_shift(sp, bb, g.Lx, g.nb, g.full_mask);
         for (int a = 0; a < g.Lx; ++a)
-          add(L.flip_of[L.g_of_ab[a + g.Lx * bb]], rows_rotl(shifted, a, g.Lx, L.rot_hi, L.rot_lo));
+          add(L.coset_elem[p * nT + a + g.Lx * bb], rows_rotl(shifted, a, g.Lx, L.rot_hi, L.rot_lo));
       }
     }
   } else {

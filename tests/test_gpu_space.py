@@ -118,10 +118,10 @@ def test_reduce_square4x4_all_k(ed, oracle, golden):
 
 @pytest.mark.parametrize("k,parity", [(0, 1), (0, -1), (8, 1), (8, -1)])
 def test_reduce_dihedral_generic(ed, oracle, k, parity):
-    """Translations x inversion of a 16-site ring (32 elements, not a torus
-    translation group): the generic LDS-table canonicaliser.  The 1D irreps
-    have chi(T^a) = exp(-2 pi i k a / L) real (k = 0, L/2) and
-    chi(T^a I) = parity * chi(T^a)."""
+    """Translations x inversion of a 16-site ring (32 elements): the
+    translations are all present, so the torus canonicaliser runs over the two
+    cosets {T, T I}.  The 1D irreps have chi(T^a) = exp(-2 pi i k a / L) real
+    (k = 0, L/2) and chi(T^a I) = parity * chi(T^a)."""
     L = 16
     hs, _ = ed.spin_half_system(L)
     hsr = ed.represent(ed.HilbertSpaceSector(hs, 0))
@@ -130,6 +130,23 @@ def test_reduce_dihedral_generic(ed, oracle, k, parity):
     irrep = ed.TranslationIrrep(L, k)
     ch = [irrep.character(a).real for a in range(L)]
     ops = [(t, ch[a]) for a, t in enumerate(T)] + [(t * inv, parity * ch[a]) for a, t in enumerate(T)]
+    r, _ = _check_reduce(ed, oracle, hs, hsr, oracle.sector_basis(L, 0), ops, L)
+    assert r.canonicaliser() == "torus"
+
+
+@pytest.mark.parametrize("k,parity", [(0, 1), (0, -1), (4, 1), (4, -1)])
+def test_reduce_even_dihedral_generic(ed, oracle, k, parity):
+    """Even translations x inversion of a 16-site ring (16 elements, not all
+    translations): the generic LDS-table canonicaliser; chi(T^2a) =
+    exp(-2 pi i k 2a / L) is real for k = 0, 4."""
+    L = 16
+    hs, _ = ed.spin_half_system(L)
+    hsr = ed.represent(ed.HilbertSpaceSector(hs, 0))
+    T = ed.translation_group(L)
+    inv = ed.SitePermutation([((-i) % L) + 1 for i in range(L)])
+    irrep = ed.TranslationIrrep(L, k)
+    ch = [irrep.character(a).real for a in range(L)]
+    ops = [(T[a], ch[a]) for a in range(0, L, 2)] + [(T[a] * inv, parity * ch[a]) for a in range(0, L, 2)]
     r, _ = _check_reduce(ed, oracle, hs, hsr, oracle.sector_basis(L, 0), ops, L)
     assert r.canonicaliser() == "generic"
 

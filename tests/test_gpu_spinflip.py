@@ -40,12 +40,18 @@ def _system(ed, oracle, shape, k, parity, generic=False):
         bonds = j1 + j2
         w = [1.0] * len(j1) + [0.5] * len(j2)
     H = _heis(ed, hs, bonds, w)
-    if generic:
-        # translations x inversion x flip of a ring: 4L elements, generic tables
+    if generic == "dihedral":
+        # translations x inversion x flip of a ring: 4L elements, 4 cosets T p
         T = ed.translation_group(n)
         inv = ed.SitePermutation([((-i) % n) + 1 for i in range(n)])
         ch = [ed.TranslationIrrep(n, k[0]).character(a).real for a in range(n)]
         base = [(t, ch[a]) for a, t in enumerate(T)] + [(t * inv, ch[a]) for a, t in enumerate(T)]
+    elif generic:
+        # even translations x inversion x flip: 2L elements, generic tables with XOR
+        T = ed.translation_group(n)
+        inv = ed.SitePermutation([((-i) % n) + 1 for i in range(n)])
+        ch = [ed.TranslationIrrep(n, k[0]).character(a).real for a in range(n)]
+        base = [(T[a], ch[a]) for a in range(0, n, 2)] + [(T[a] * inv, ch[a]) for a in range(0, n, 2)]
     else:
         base = ed.TranslationIrrep(shape if Ly > 1 else n, k if Ly > 1 else k[0])
     ops = _flip_irrep(ed, base, parity)
@@ -58,13 +64,14 @@ def _system(ed, oracle, shape, k, parity, generic=False):
 CASES = [((12, 1), (0, 0), 1, False), ((12, 1), (3, 0), -1, False), ((12, 1), (6, 0), 1, False),
          ((4, 4), (0, 0), -1, False), ((4, 4), (1, 2), 1, False), ((4, 4), (2, 0), -1, False),
          ((6, 4), (3, 0), 1, False), ((3, 4), (1, 1), -1, False),
-         ((12, 1), (0, 0), 1, True), ((12, 1), (6, 0), -1, True)]
+         ((12, 1), (0, 0), 1, "dihedral"), ((12, 1), (6, 0), -1, "dihedral"),
+         ((12, 1), (0, 0), 1, True), ((12, 1), (3, 0), -1, True)]
 
 
 @pytest.mark.parametrize("shape,k,parity,generic", CASES)
 def test_reduce_and_apply_vs_oracle(ed, oracle, shape, k, parity, generic):
     hs, H, r, red = _system(ed, oracle, shape, k, parity, generic)
-    assert r.canonicaliser() == ("generic" if generic else "torus")
+    assert r.canonicaliser() == ("generic" if generic is True else "torus")
     assert np.array_equal(r.basis_list, red.basis_list)
     assert np.array_equal(r.basis_mapping_index, red.basis_mapping_index)
     assert np.abs(r.basis_mapping_amplitude - red.basis_mapping_amplitude).max(initial=0) < 1e-14
@@ -126,3 +133,54 @@ def test_torus6x6_parity_sectors(ed):
             e = ed.eigs(opr, 1, tol=1e-10, max_basis=100, criterion="eigenvalue").values[0]
             assert abs(e - (-18.137147540)) < 1e-8
     assert dims[1] + dims[-1] == 252091362
+
+
+@pytest.mark.parametrize("k", [(0, 0), (2, 2)])
+@pytest.mark.parametrize("point", ["A1", "A2", "B1", "B2"])
+@pytest.mark.parametrize("parity", [0, 1, -1])
+def test_space_group_4x4_vs_oracle(ed, oracle, k, point, parity):
+    """Translations x C4v (128 elements), optionally x spin flip (256): the
+    torus canonicaliser over cosets T p; basis, mapping and products vs the
+    oracle's generic reduction."""
+    n = 16
+    hs, _ = ed.spin_half_system(n)
+    j1, j2 = ed.square_bonds(4, 4)
+    H = _heis(ed, hs, j1 + j2, [1.0] * len(j1) + [0.5] * len(j2))
+    base = ed.SquareSpaceIrrep(4, k, point)
+    ops = base.symops_and_amplitudes() if parity == 0 else _flip_irrep(ed, base, parity)
+    r = ed.symmetry_reduce(ed.represent(ed.HilbertSpaceSector(hs, 0)), ops)
+    red = _oracle_reduce(ed, oracle, hs, ops, n)
+    assert r.canonicaliser() == "torus"
+    assert np.array_equal(r.basis_list, red.basis_list)
+    assert np.array_equal(r.basis_mapping_index, red.basis_mapping_index)
+    assert np.abs(r.basis_mapping_amplitude - red.basis_mapping_amplitude).max(initial=0) < 1e-14
+    if r.dimension == 0:
+        return
+    opr = ed.represent(r, H)
+    o = oracle.Opr(red, ed.terms_arrays(H))
+    x = np.random.default_rng(1).standard_normal(r.dimension)
+    ref = o.apply(np.zeros(r.dimension, complex), x.astype(complex))
+    out = np.zeros(r.dimension)
+    ed.mul_(out, opr, x)
+    assert np.linalg.norm(out - ref) <= 1e-13 * np.linalg.norm(ref)
+    E = ed.eigs(opr, 1, tol=1e-12, max_basis=min(100, r.dimension)).values[0]
+    assert abs(E - oracle.lowest_eigenvalues(o, 1)[0]) < 1e-11 * abs(E)
+
+
+@pytest.mark.parametrize("flip", [False, True])
+def test_space_group_6x6_a1(ed, flip):
+    """6x6 J1-J2, k = (0,0), point irrep A1 of C4v (288 elements; 576 with the
+    even spin flip): the ground state of the k = (0,0) translation sector lies
+    in it (E0 = -18.137147540, the translation-only value)."""
+    hs, _ = ed.spin_half_system(36)
+    j1, j2 = ed.square_bonds(6, 6)
+    H = _heis(ed, hs, j1 + j2, [1.0] * len(j1) + [0.5] * len(j2))
+    base = ed.SquareSpaceIrrep(6, (0, 0), "A1")
+    ops = _flip_irrep(ed, base, 1) if flip else base
+    r = ed.symmetry_reduce(ed.represent(ed.HilbertSpaceSector(hs, 0)), ops)
+    assert r.canonicaliser() == "torus"
+    assert r.dimension == (15804956 if flip else 31566122)
+    orb = r.orbit_sizes.astype(np.int64)
+    assert np.all((576 if flip else 288) % orb == 0)
+    e = ed.eigs(ed.represent(r, H), 1, tol=1e-10, max_basis=100, criterion="eigenvalue", mode="mixed").values[0]
+    assert abs(e - (-18.137147540)) < 1e-8
