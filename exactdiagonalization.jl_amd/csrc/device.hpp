@@ -408,12 +408,59 @@ __device__ __forceinline__ uint64_t apply_coset(const GroupDev& g, const GroupLd
   return w ^ L.coset_xr[p];
 }
 
+// Smallest (top row, second row) pair over the translation images of w, as
+// the row-pair table's value part (entry | 0xff).
+template <int LXC = 0, int LYC = 0>
+__device__ __forceinline__ uint32_t torus_pair_min(const GroupDev& g, const GroupLds& L, uint64_t w) {
+  const int Lx = LXC ? LXC : g.Lx;
+  const int Ly = LYC ? LYC : g.Ly;
+  const int NB = LXC ? LXC * LYC : g.nb;
+  const uint32_t m2 = (1u << (2 * Lx)) - 1u;
+  const uint64_t ww = (w << Lx) | (w >> (NB - Lx));
+  uint32_t best = 0xffffffffu;
+#pragma unroll
+  for (int y = 0; y < 8; ++y)
+    if (y < Ly) best = min(best, L.pair[(uint32_t)(ww >> (y * Lx)) & m2] | 0xffu);
+  return best;
+}
+
 // Torus group: min over the cosets T p of the translation-canonical form of
-// p(b); s, h (largest inverse index among ties) and cnt = |Stab(s)|.
+// p(b); s, h (largest inverse index among ties) and cnt = |Stab(s)|.  With
+// the row-pair table and several cosets, a first pass finds the cosets whose
+// images reach the smallest (top, second) row pair; only those are scanned.
 template <int LXC = 0, int LYC = 0, bool PAIRS = true>
 __device__ __forceinline__ void canonicalize_torus_cosets(const GroupDev& g, const GroupLds& L, uint64_t b,
                                                           uint64_t& s, int& h, int& cnt) {
   const int nT = (LXC ? LXC : g.Lx) * (LYC ? LYC : g.Ly);
+  if (PAIRS && g.n_cosets > 1 && g.n_cosets <= 64) {
+    uint32_t best = 0xffffffffu;
+    uint64_t sel = 0;  // cosets reaching best
+    for (int p = 0; p < g.n_cosets; ++p) {
+      const uint32_t v = torus_pair_min<LXC, LYC>(g, L, apply_coset(g, L, p, b));
+      sel = v < best ? (1ull << p) : (v == best ? sel | (1ull << p) : sel);
+      best = v < best ? v : best;
+    }
+    s = ~0ull;
+    h = 0;
+    cnt = 0;
+    while (sel) {
+      const int p = __builtin_ctzll(sel);
+      sel &= sel - 1;
+      uint64_t s2;
+      int pos2, c2;
+      canonicalize_torus_pairs_t<LXC, LYC>(g, L, L.coset_inv + p * nT, apply_coset(g, L, p, b), s2, pos2, c2);
+      const int h2 = L.coset_elem[p * nT + pos2];
+      if (s2 < s) {
+        s = s2;
+        h = h2;
+        cnt = c2;
+      } else if (s2 == s) {
+        cnt += c2;
+        if (L.inv[h2] > L.inv[h]) h = h2;
+      }
+    }
+    return;
+  }
   int pos;
   if constexpr (PAIRS)
     canonicalize_torus_pairs_t<LXC, LYC>(g, L, L.coset_inv, b, s, pos, cnt);
