@@ -21,7 +21,7 @@ the 4x6 lattice at k = (pi, 0).
 workload's operator.
 
 Options: --steps K --warmup W --gpus 1 [--config 4x6|6x6] [--no-cpu]
-[--no-grad] [--no-e0].
+[--no-grad] [--no-e0] [--space-group].
 """
 from __future__ import annotations
 
@@ -217,6 +217,38 @@ def grad_times(ed):
     }
 
 
+def space_group_times(ed, cfg, steps, warmup):
+    """The same J1-J2 Hamiltonian on the k = (0,0) A1 sector of the space group
+    (translations x C4v), without and with the even spin flip: dimension,
+    reduction time, FP32/FP64 matvec time and mixed-precision time to E0."""
+    from edhip import _lib
+
+    Lx, Ly = cfg["Lx"], cfg["Ly"]
+    if Lx != Ly or tuple(cfg["k"]) != (0, 0):
+        return None
+    n = Lx * Ly
+    hs, _ = ed.spin_half_system(n)
+    j1, j2 = ed.square_bonds(Lx, Ly)
+    terms = [ed.heisenberg_bond(hs, i, j, 1.0) for i, j in j1] + [ed.heisenberg_bond(hs, i, j, cfg["J2"]) for i, j in j2]
+    H = ed.simplify(sum(terms, ed.NullOperator()))
+    hsr = ed.represent(ed.HilbertSpaceSector(hs, 0))
+    out = {}
+    base = ed.SquareSpaceIrrep(Lx, (0, 0), "A1")
+    flip = ed.make_product_irrep([(ed.GlobalBitFlip(False), 1.0), (ed.GlobalBitFlip(True), 1.0)], base)
+    for name, ops, order in (("A1", base, 8 * n), ("A1_even_flip", flip, 16 * n)):
+        t0 = time.perf_counter()
+        r = ed.symmetry_reduce(hsr, ops)
+        t_red = time.perf_counter() - t0
+        opr = ed.represent(r, H)
+        ms32, _ = time_matvecs(ed, opr, _lib.F32, steps, warmup)
+        ms64, _ = time_matvecs(ed, opr, _lib.F64, steps, warmup)
+        res = ed.eigs(opr, 1, mode="mixed", tol=1e-10, max_basis=100, max_restarts=20, criterion="eigenvalue")
+        out[name] = {"group_order": order, "dim": r.dimension, "reduce_seconds": t_red, "matvec_ms_f32": ms32,
+                     "matvec_ms_f64": ms64, "E0": float(res.values[0]), "time_to_E0_mixed_s": res.t_total,
+                     "matvecs": res.matvecs, "matvecs_fp64": res.matvecs_fp64}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -226,6 +258,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-e0", action="store_true", help="skip the time-to-E0 runs (about 40 s)")
     ap.add_argument("--no-grad", action="store_true")
+    ap.add_argument("--space-group", action="store_true",
+                    help="also time the space-group (translations x C4v, optionally x spin flip) A1 sectors")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run one plain 1 GiB device copy (elementwise kernel) for HBM-counter calibration and exit")
     args = ap.parse_args()
@@ -311,6 +345,8 @@ def main():
         line["time_to_E0"] = lanczos_times(ed, opr)
     if not args.no_grad:
         line["grad"] = grad_times(ed)
+    if args.space_group:
+        line["space_group"] = space_group_times(ed, cfg, args.steps, args.warmup)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(ed)
     print(json.dumps(line), flush=True)
