@@ -117,6 +117,7 @@ SIGNATURES = {
     "edhip_memcpy": (_i32, [_vp, _vp, _vp, _u64, _i32]),
     "edhip_space_sector": (_i32, [_vp, _i32, _i32, _P(ctypes.c_int32), _P(_vp)]),
     "edhip_space_from_list": (_i32, [_vp, _i32, _i64, _P(_u64), _P(_vp)]),
+    "edhip_space_sector_qn": (_i32, [_vp, _i32, _P(_i32), _P(_i32), _i32, _P(_i32), _i32, _P(_i32), _P(_vp)]),
     "edhip_space_reduce": (_i32, [_vp, _vp, _i32, _P(ctypes.c_int32), _P(_dbl), _P(_dbl), _dbl, _P(_vp)]),
     "edhip_space_reduce_ex": (_i32, [_vp, _vp, _i32, _P(ctypes.c_int32), _P(ctypes.c_uint64), _P(_dbl), _P(_dbl), _dbl,
                                      _P(_vp)]),

@@ -1446,6 +1446,117 @@ int edhip_space_from_list(edhip_ctx* ctx, int n_bits, int64_t n, const uint64_t*
   return rc;
 }
 
+int edhip_space_sector_qn(edhip_ctx* ctx, int n_sites, const int32_t* bitoffsets, const int32_t* n_states, int qn_dim,
+                          const int32_t* state_qns, int n_allowed, const int32_t* allowed_qns, edhip_space** out) {
+  if (!ctx || !out) return EDHIP_ERR_ARGUMENT;
+  *out = nullptr;
+  auto sp = std::make_unique<edhip_space>();
+  int rc = guarded(ctx, [&] {
+    if (n_sites < 0 || (n_sites > 0 && (!bitoffsets || !n_states || !state_qns)))
+      fail(EDHIP_ERR_ARGUMENT, "bad site description");
+    if (qn_dim < 1) fail(EDHIP_ERR_ARGUMENT, "quantum numbers need at least one component");
+    if (n_allowed < 0 || (n_allowed > 0 && !allowed_qns)) fail(EDHIP_ERR_ARGUMENT, "bad allowed quantum numbers");
+    const int n_bits = n_sites ? bitoffsets[n_sites] : 0;
+    if (n_sites && bitoffsets[0] != 0) fail(EDHIP_ERR_ARGUMENT, "bit offsets must start at 0");
+    if (n_bits < 0 || n_bits > 63) fail(EDHIP_ERR_ARGUMENT, "type UInt64 not enough to represent the hilbert space");
+    std::vector<int> first_state(n_sites + 1, 0);
+    for (int j = 0; j < n_sites; ++j) {
+      const int wdt = bitoffsets[j + 1] - bitoffsets[j];
+      if (wdt < 0 || n_states[j] < 1 || (wdt < 31 && n_states[j] > (1 << wdt)))
+        fail(EDHIP_ERR_ARGUMENT, "site states do not fit the site's bits");
+      first_state[j + 1] = first_state[j] + n_states[j];
+    }
+    // Sums of quantum-number tuples as vectors of int64.
+    using QN = std::vector<int64_t>;
+    auto qn_of = [&](int j, int s) {
+      QN q(qn_dim);
+      for (int d = 0; d < qn_dim; ++d) q[d] = state_qns[(size_t)(first_state[j] + s) * qn_dim + d];
+      return q;
+    };
+    auto add = [](QN a, const QN& b) {
+      for (size_t d = 0; d < a.size(); ++d) a[d] += b[d];
+      return a;
+    };
+    std::vector<QN> allow;
+    for (int a = 0; a < n_allowed; ++a) {
+      QN q(qn_dim);
+      for (int d = 0; d < qn_dim; ++d) q[d] = allowed_qns[(size_t)a * qn_dim + d];
+      if (std::find(allow.begin(), allow.end(), q) == allow.end()) allow.push_back(q);
+    }
+    // fw[i][q]: number of configurations of sites 0..i-1 with total q
+    // (hilbert_space_representation.jl:137-151 builds the same reachable sets).
+    std::vector<std::map<QN, uint64_t>> fw(n_sites + 1);
+    fw[0][QN(qn_dim, 0)] = 1;
+    for (int j = 0; j < n_sites; ++j)
+      for (const auto& [q, c] : fw[j])
+        for (int s = 0; s < n_states[j]; ++s) fw[j + 1][add(q, qn_of(j, s))] += c;
+    // completions(i, u): words of sites 0..i-1 that bring upper sum u to an allowed total
+    auto completions = [&](int i, const QN& u) {
+      uint64_t c = 0;
+      for (const QN& Q : allow) {
+        QN need(qn_dim);
+        for (int d = 0; d < qn_dim; ++d) need[d] = Q[d] - u[d];
+        auto it = fw[i].find(need);
+        if (it != fw[i].end()) c += it->second;
+      }
+      return c;
+    };
+    const uint64_t dim = completions(n_sites, QN(qn_dim, 0));
+    if (dim >= 0xffffffffull) fail(EDHIP_ERR_ARGUMENT, "more than 2^32-1 basis states");
+    // Levels from the top site down; upper sums with no completion are dropped.
+    std::vector<std::vector<QN>> upper(n_sites + 1);
+    std::vector<std::map<QN, int>> upper_idx(n_sites + 1);
+    if (dim) upper[n_sites].push_back(QN(qn_dim, 0));
+    std::vector<QnSite> sites(n_sites);
+    std::vector<uint64_t> cnt;
+    std::vector<int32_t> nxt;
+    for (int j = n_sites - 1; j >= 0; --j) {
+      sites[j] = QnSite{(int32_t)cnt.size(), n_states[j], bitoffsets[j], std::max<int32_t>(1, (int32_t)upper[j + 1].size())};
+      for (const QN& u : upper[j + 1])
+        for (int s = 0; s < n_states[j]; ++s) {
+          const QN v = add(u, qn_of(j, s));
+          const uint64_t c = completions(j, v);
+          if (!c) {
+            cnt.push_back(0);
+            nxt.push_back(-1);
+            continue;
+          }
+          auto it = upper_idx[j].find(v);
+          if (it == upper_idx[j].end()) {
+            it = upper_idx[j].emplace(v, (int)upper[j].size()).first;
+            upper[j].push_back(v);
+          }
+          cnt.push_back(c);
+          nxt.push_back(it->second);
+        }
+      if (upper[j + 1].empty())  // empty sector: keep every level's entry range valid
+        for (int s = 0; s < n_states[j]; ++s) {
+          cnt.push_back(0);
+          nxt.push_back(-1);
+        }
+      if (cnt.size() > (size_t)INT32_MAX / 2) fail(EDHIP_ERR_ARGUMENT, "quantum-number tables too large");
+    }
+    sp->ctx = ctx;
+    sp->n_bits = n_bits;
+    sp->dim = (int64_t)dim;
+    sp->keys.alloc((size_t)std::max<uint64_t>(dim, 1) * 8);
+    if (dim) {
+      DevBuf d_sites, d_cnt, d_nxt;
+      upload(d_sites, sites, ctx->stream);
+      upload(d_cnt, cnt, ctx->stream);
+      upload(d_nxt, nxt, ctx->stream);
+      HIP_OR_THROW(launch_enumerate_qn(dim, n_sites, d_sites.as<QnSite>(), (int)cnt.size(), d_cnt.as<uint64_t>(),
+                                       d_nxt.as<int32_t>(), sp->keys.as<uint64_t>(), ctx->stream));
+      ctx->sync();
+    }
+    sp->materialized = true;
+    sp->build_index();
+    ctx->sync();
+  });
+  if (rc == EDHIP_OK) *out = sp.release();
+  return rc;
+}
+
 int edhip_space_reduce(edhip_ctx* ctx, const edhip_space* parent, int n_group, const int32_t* bit_perm,
                        const double* chi_re, const double* chi_im, double tol, edhip_space** out) {
   return edhip_space_reduce_ex(ctx, parent, n_group, bit_perm, nullptr, chi_re, chi_im, tol, out);

@@ -12,6 +12,16 @@ namespace edhip {
 // words[i] = unrank(first + i), i < count
 hipError_t launch_enumerate_sector(uint64_t first, uint64_t count, int n_bits, const uint64_t* d_W, uint64_t* d_out,
                                    hipStream_t st);
+// One site of a multi-quantum-number sector walk (k_enumerate_qn): its table
+// entries start at off, ns states, bit offset shift, n_upper upper sums.
+struct QnSite {
+  int32_t off, ns, shift, n_upper;
+};
+// Tables up to this size are staged in LDS by the enumeration kernel.
+constexpr size_t kQnStageBytes = 48 * 1024;
+// words[i] = i-th word of a multi-QN sector in ascending order, i < count
+hipError_t launch_enumerate_qn(uint64_t count, int n_sites, const QnSite* d_sites, int n_entries, const uint64_t* d_cnt,
+                               const int32_t* d_nxt, uint64_t* d_out, hipStream_t st);
 // prefix[j] = lower_bound(keys, j << shift), j <= n_buckets
 hipError_t launch_build_prefix(const uint64_t* d_keys, int64_t dim, int shift, uint64_t n_buckets, uint32_t* d_prefix,
                                hipStream_t st);

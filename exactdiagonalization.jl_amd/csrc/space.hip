@@ -22,6 +22,55 @@ __global__ __launch_bounds__(kBlock) void k_enumerate_sector(uint64_t first, uin
   out[i] = unrank_word(first + i, n_bits, Wl);
 }
 
+// Multi-quantum-number sectors (sites of any number of states, QN tuples):
+// words[i] = the i-th word of the sector in ascending order.  The most
+// significant site is the last one, so ascending words are the lexicographic
+// order of (state of site n-1, ..., state of site 0); the walk fixes the sites
+// from the top, skipping the counted completions of every smaller state.
+// Table entry sites[j].off + u*sites[j].ns + s: the number of words whose sites
+// above j sum to upper-sum u (an index into the sums reachable at that level)
+// and whose site j is in state s, and the index of the new upper sum (-1 when
+// no word has it; such an entry has count 0 and is never taken).
+template <bool STAGE>
+__global__ __launch_bounds__(kBlock) void k_enumerate_qn(uint64_t count, int n_sites, const QnSite* __restrict__ sites,
+                                                         int n_entries, const uint64_t* __restrict__ cnt_g,
+                                                         const int32_t* __restrict__ nxt_g, uint64_t* __restrict__ out) {
+  extern __shared__ unsigned char lds[];
+  const uint64_t* cnt = cnt_g;
+  const int32_t* nxt = nxt_g;
+  if (STAGE) {
+    uint64_t* c = (uint64_t*)lds;
+    int32_t* x = (int32_t*)(c + n_entries);
+    for (int i = threadIdx.x; i < n_entries; i += blockDim.x) {
+      c[i] = cnt_g[i];
+      x[i] = nxt_g[i];
+    }
+    __syncthreads();
+    cnt = c;
+    nxt = x;
+  }
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  uint64_t r = i, w = 0;
+  int u = 0;
+  for (int j = n_sites - 1; j >= 0; --j) {
+    const QnSite st = sites[j];
+    const int e = st.off + min(u, st.n_upper - 1) * st.ns;
+    int pick = st.ns - 1;
+    for (int s = 0; s < st.ns - 1; ++s) {
+      const uint64_t c = cnt[e + s];
+      if (r < c) {
+        pick = s;
+        break;
+      }
+      r -= c;
+    }
+    w |= (uint64_t)pick << st.shift;
+    u = max(nxt[e + pick], 0);
+  }
+  out[i] = w;
+}
+
 __global__ __launch_bounds__(kBlock) void k_build_prefix(const uint64_t* __restrict__ keys, int64_t dim, int shift,
                                                          uint64_t n_buckets, uint32_t* __restrict__ prefix) {
   uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -268,6 +317,19 @@ hipError_t launch_enumerate_sector(uint64_t first, uint64_t count, int n_bits, c
   if (count == 0) return hipSuccess;
   hipLaunchKernelGGL(k_enumerate_sector, dim3(grid_for(count)), dim3(kBlock), w_bytes(n_bits), st, first, count, n_bits,
                      d_W, d_out);
+  return hipGetLastError();
+}
+
+hipError_t launch_enumerate_qn(uint64_t count, int n_sites, const QnSite* d_sites, int n_entries, const uint64_t* d_cnt,
+                               const int32_t* d_nxt, uint64_t* d_out, hipStream_t st) {
+  if (count == 0) return hipSuccess;
+  const size_t lds = (size_t)n_entries * (sizeof(uint64_t) + sizeof(int32_t));
+  if (lds <= kQnStageBytes)
+    hipLaunchKernelGGL(k_enumerate_qn<true>, dim3(grid_for(count)), dim3(kBlock), (uint32_t)lds, st, count, n_sites,
+                       d_sites, n_entries, d_cnt, d_nxt, d_out);
+  else
+    hipLaunchKernelGGL(k_enumerate_qn<false>, dim3(grid_for(count)), dim3(kBlock), 0, st, count, n_sites, d_sites,
+                       n_entries, d_cnt, d_nxt, d_out);
   return hipGetLastError();
 }
 

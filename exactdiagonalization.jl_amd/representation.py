@@ -131,13 +131,37 @@ def represent_array(space, basis_list=None, *, BR=None, ctx=None):
     else:
         pops = popcount_sets(space)
         if pops is None:
-            raise NotImplementedError(
-                "device enumeration covers sites of two states whose quantum numbers fix the number of set bits; "
-                "pass an explicit basis_list for other sites"
-            )
-        p = np.ascontiguousarray(pops, dtype=np.int32)
-        ctx.check(_lib.lib().edhip_space_sector(ctx.handle, nb, p.size, ptr(p, ctypes.c_int32), ctypes.byref(h)))
+            _sector_qn(space, ctx, h)
+        else:
+            p = np.ascontiguousarray(pops, dtype=np.int32)
+            ctx.check(_lib.lib().edhip_space_sector(ctx.handle, nb, p.size, ptr(p, ctypes.c_int32), ctypes.byref(h)))
     return HilbertSpaceRepresentation(space, h, ctx)
+
+
+def _sector_qn(space, ctx, h):
+    """Device enumeration of a sector of arbitrary sites and quantum-number
+    tuples (hs_get_basis_list, hilbert_space_representation.jl:109-206)."""
+    base = basespace(space)
+    qd = base.qntype_len()
+    allowed = (space.allowed_quantum_numbers if isinstance(space, HilbertSpaceSector)
+               else set(base.quantum_number_sectors()))
+    offs = np.ascontiguousarray(base.bitoffsets, dtype=np.int32)
+    nst = np.ascontiguousarray([len(s.states) for s in base.sites], dtype=np.int32)
+    qns = np.ascontiguousarray([c for s in base.sites for st in s.states for c in st.quantum_number], dtype=np.int64)
+    al = np.ascontiguousarray([c for q in sorted(allowed) for c in q], dtype=np.int64)
+    for a in (qns, al):
+        if a.size and (a.min() < -(2**31) or a.max() >= 2**31):
+            raise ArgumentError("quantum numbers must fit in 32 bits")
+    qns, al = qns.astype(np.int32), al.astype(np.int32)
+    if not qns.size:
+        qns = np.zeros(1, dtype=np.int32)
+    if not al.size:
+        al = np.zeros(1, dtype=np.int32)
+    if not nst.size:
+        nst = np.zeros(1, dtype=np.int32)
+    ctx.check(_lib.lib().edhip_space_sector_qn(
+        ctx.handle, base.n_sites, ptr(offs, ctypes.c_int32), ptr(nst, ctypes.c_int32), qd,
+        ptr(qns, ctypes.c_int32), len(allowed), ptr(al, ctypes.c_int32), ctypes.byref(h)))
 
 
 class SitePermutation:

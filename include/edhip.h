@@ -75,6 +75,19 @@ int edhip_memcpy(edhip_ctx* ctx, void* dst, const void* src, uint64_t bytes, int
 int edhip_space_sector(edhip_ctx* ctx, int n_bits, int n_popcounts, const int32_t* popcounts_host,
                        edhip_space** out);
 
+/* represent(HilbertSpaceSector(hs, qns)) and represent(hs) for any sites:
+ *   hs_get_basis_list(hss, BR) at src/Representation/hilbert_space_representation.jl:127-206
+ *   (and hs_get_basis_list(hs, BR) at :109-119 when every reachable total is allowed).
+ * Site j has n_states[j] states and occupies bits [bitoffsets[j], bitoffsets[j+1])
+ * (bitoffsets has n_sites + 1 entries, HilbertSpace.bitoffsets); its state s is
+ * stored as the number s in those bits and carries the quantum-number tuple
+ * state_qns[(sum_{i<j} n_states[i] + s) * qn_dim + 0 .. qn_dim-1].
+ * allowed_qns: n_allowed tuples of qn_dim components; a word belongs to the
+ * sector iff the sum of its states' tuples is one of them.  The ascending basis
+ * is enumerated on the device by unranking over per-site completion counts. */
+int edhip_space_sector_qn(edhip_ctx* ctx, int n_sites, const int32_t* bitoffsets, const int32_t* n_states, int qn_dim,
+                          const int32_t* state_qns, int n_allowed, const int32_t* allowed_qns, edhip_space** out);
+
 /* represent(hs, basis_list) / represent_array(hs, basis_list):
  *   hilbert_space_representation.jl:239-256.  The list must be sorted and
  * unique (FrozenSortedArrayIndex validation, src/frozensortedarray.jl:14-22);
