@@ -142,6 +142,8 @@ SIGNATURES = {
     "edhip_iterate": (_i32, [_vp, _i32, _i64, _P(_i64), _P(_i64), _P(_dbl), _P(_dbl)]),
     "edhip_sparse_build": (_i32, [_vp, _dbl, _P(_i64)]),
     "edhip_sparse_export": (_i32, [_vp, _P(_i64), _P(_i64), _P(_dbl), _P(_dbl)]),
+    "edhip_get_vectors": (_i32, [_vp, _i32, _i64, _P(_i64), _dbl, _P(_i64), _P(_i64), _P(_dbl), _P(_dbl)]),
+    "edhip_matrix": (_i32, [_vp, _vp]),
     "edhip_lanczos_default_opts": (None, [_P(LanczosOpts)]),
     "edhip_lanczos": (_i32, [_vp, _P(LanczosOpts), _P(_dbl), _vp, _P(LanczosStats)]),
     "edhip_term_expect": (_i32, [_vp, _i32, _vp, _vp, _P(_dbl)]),

@@ -1936,8 +1936,8 @@ int edhip_sparse_build(edhip_op* op, double tol, int64_t* nnz) {
     sim.alloc((size_t)d * slot * 8);
     cnt.alloc((size_t)(d + 1) * 8);
     HIP_OR_THROW(hipMemsetAsync(cnt.p, 0, (size_t)(d + 1) * 8, st));
-    HIP_OR_THROW(launch_sparse_columns(op->dev_col, tol, slot, sidx.as<int64_t>(), sre.as<double>(), sim.as<double>(),
-                                       cnt.as<int64_t>(), st));
+    HIP_OR_THROW(launch_sparse_columns(op->dev_col, 1, tol, slot, d, nullptr, sidx.as<int64_t>(), sre.as<double>(),
+                                       sim.as<double>(), cnt.as<int64_t>(), st));
     op->sp_colptr.alloc((size_t)(d + 1) * 8);
     size_t tb = 0;
     HIP_OR_THROW(exclusive_scan_i64(cnt.as<int64_t>(), op->sp_colptr.as<int64_t>(), (uint64_t)(d + 1), nullptr, tb, st));
@@ -1955,6 +1955,52 @@ int edhip_sparse_build(edhip_op* op, double tol, int64_t* nnz) {
     op->ctx->sync();
     op->sp_nnz = total;
     *nnz = total;
+  });
+}
+
+int edhip_get_vectors(edhip_op* op, int transpose, int64_t n, const int64_t* which, double tol, int64_t* counts,
+                      int64_t* idx, double* re, double* im) {
+  if (!op) return EDHIP_ERR_ARGUMENT;
+  return guarded(op->ctx, [&] {
+    if (n < 0 || (n && (!which || !counts || !idx || !re || !im))) fail(EDHIP_ERR_ARGUMENT, "bad arguments");
+    const int64_t d = op->space->dim;
+    for (int64_t k = 0; k < n; ++k)
+      if (which[k] < 0 || which[k] >= d) fail(EDHIP_ERR_BOUNDS, "index out of bounds");
+    if (n == 0) return;
+    hipStream_t st = op->ctx->stream;
+    const int slot = (int)(1 + op->n_terms);  // the documented stride; 1 + n_off entries are used at most
+    const size_t ne = (size_t)n * slot;
+    DevBuf sel, sidx, sre, sim, cnt;
+    sel.alloc((size_t)n * 8);
+    sidx.alloc(ne * 8);
+    sre.alloc(ne * 8);
+    sim.alloc(ne * 8);
+    cnt.alloc((size_t)n * 8);
+    HIP_OR_THROW(hipMemcpyAsync(sel.p, which, (size_t)n * 8, hipMemcpyHostToDevice, st));
+    HIP_OR_THROW(launch_sparse_columns(transpose ? op->dev_col : op->dev_row, transpose ? 1 : 0, tol, slot, n,
+                                       sel.as<int64_t>(), sidx.as<int64_t>(), sre.as<double>(), sim.as<double>(),
+                                       cnt.as<int64_t>(), st));
+    HIP_OR_THROW(hipMemcpyAsync(counts, cnt.p, (size_t)n * 8, hipMemcpyDeviceToHost, st));
+    HIP_OR_THROW(hipMemcpyAsync(idx, sidx.p, ne * 8, hipMemcpyDeviceToHost, st));
+    HIP_OR_THROW(hipMemcpyAsync(re, sre.p, ne * 8, hipMemcpyDeviceToHost, st));
+    HIP_OR_THROW(hipMemcpyAsync(im, sim.p, ne * 8, hipMemcpyDeviceToHost, st));
+    op->ctx->sync();
+  });
+}
+
+int edhip_matrix(edhip_op* op, void* out) {
+  if (!op) return EDHIP_ERR_ARGUMENT;
+  return guarded(op->ctx, [&] {
+    const int64_t d = op->space->dim;
+    if (d && !out) fail(EDHIP_ERR_ARGUMENT, "null output");
+    if (d > (int64_t)1 << 20) fail(EDHIP_ERR_ARGUMENT, "dimension too large for a dense matrix");
+    if (d == 0) return;
+    hipStream_t st = op->ctx->stream;
+    const int cx = op->is_complex() ? 1 : 0;
+    const size_t bytes = (size_t)d * (size_t)d * (cx ? 16 : 8);
+    HIP_OR_THROW(hipMemsetAsync(out, 0, bytes, st));
+    HIP_OR_THROW(launch_dense_columns(op->dev_col, cx, (double*)out, st));
+    op->ctx->sync();
   });
 }
 

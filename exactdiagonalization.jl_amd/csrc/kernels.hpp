@@ -120,9 +120,15 @@ hipError_t launch_count_images(const OpDev& op, unsigned long long* d_total, hip
 hipError_t launch_iterate(const OpDev& op, const OffTerm* all_terms, int n_all, int transpose, int64_t n_rows,
                           const int64_t* d_rows, int64_t* d_idx, double* d_re, double* d_im, hipStream_t st);
 
-// Sparse (CSC) build, column form, merge + chop + sort per column.
-hipError_t launch_sparse_columns(const OpDev& op, double tol, int slot, int64_t* d_scratch_idx, double* d_scratch_re,
+// Merged, chopped, ascending entries of rows (row form, transpose = 0) or
+// columns (column form, transpose = 1): entry k is vector d_sel[k] (k itself
+// when d_sel is null), up to slot entries each, d_count[k] kept.
+hipError_t launch_sparse_columns(const OpDev& op, int transpose, double tol, int slot, int64_t n_sel,
+                                 const int64_t* d_sel, int64_t* d_scratch_idx, double* d_scratch_re,
                                  double* d_scratch_im, int64_t* d_count, hipStream_t st);
+// Dense column-major matrix (column form operator) added into a cleared
+// dim x dim buffer of doubles (complex_out: interleaved re, im pairs).
+hipError_t launch_dense_columns(const OpDev& op, int complex_out, double* d_out, hipStream_t st);
 hipError_t launch_sparse_compact(int64_t dim, int slot, const int64_t* d_colptr, const int64_t* d_scratch_idx,
                                  const double* d_scratch_re, const double* d_scratch_im, int64_t* d_rowval,
                                  double* d_re, double* d_im, hipStream_t st);

@@ -276,20 +276,24 @@ __global__ __launch_bounds__(kBlock) void k_iterate(OpDev op, const OffTerm* __r
 // ---------------------------------------------------------------------------
 // Sparse export: per column, collect (row, value), merge duplicates, drop
 // |v| < tol, sort rows (sparse_serial, abstract_operator_representation.jl:145-169).
+// The same walk over selected rows (row-form operator, TRANS = false) or
+// columns (column form, TRANS = true) gives get_row / get_column (:217-243).
 // ---------------------------------------------------------------------------
-template <int MODE>
-__global__ __launch_bounds__(kBlock) void k_sparse_columns(OpDev op, double tol, int slot, int64_t* __restrict__ sidx,
+template <int MODE, bool TRANS>
+__global__ __launch_bounds__(kBlock) void k_sparse_columns(OpDev op, double tol, int slot, int64_t n_sel,
+                                                           const int64_t* __restrict__ sel, int64_t* __restrict__ sidx,
                                                            double* __restrict__ sre, double* __restrict__ sim,
                                                            int64_t* __restrict__ count) {
   extern __shared__ unsigned char lds[];
   Staged S = stage_op<MODE>(op, lds);
   __syncthreads();
-  const int64_t col = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (col >= op.dim) return;
+  const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (k >= n_sel) return;
+  const int64_t col = sel ? sel[k] : k;
   const uint64_t b = op.rows[col];
-  int64_t* I = sidx + col * (int64_t)slot;
-  double* Re = sre + col * (int64_t)slot;
-  double* Im = sim + col * (int64_t)slot;
+  int64_t* I = sidx + k * (int64_t)slot;
+  double* Re = sre + k * (int64_t)slot;
+  double* Im = sim + k * (int64_t)slot;
   int n = 0;
   double dre = 0.0, dim_ = 0.0;
   bool any_diag = false;
@@ -319,7 +323,7 @@ __global__ __launch_bounds__(kBlock) void k_sparse_columns(OpDev op, double tol,
     const int64_t idx = index_of(op.ix, S.W, s);
     if (idx < 0) continue;
     double wr, wi;
-    element_weight<MODE, true>(op, S, t, h, cnt, sqrt_own, wr, wi);
+    element_weight<MODE, TRANS>(op, S, t, h, cnt, sqrt_own, wr, wi);
     I[n] = idx;
     Re[n] = wr;
     Im[n] = wi;
@@ -357,7 +361,52 @@ __global__ __launch_bounds__(kBlock) void k_sparse_columns(OpDev op, double tol,
     Im[m] = si;
     ++m;
   }
-  count[col] = m;
+  count[k] = m;
+}
+
+// Matrix(opr) (abstract_operator_representation.jl:121-132): column icol of the
+// column-major dense matrix is the sum of column icol's iterator entries,
+// valid rows only, in iterator order (diagonal terms, then off-diagonal terms).
+// One lane per column: no two lanes write the same element.  out is cleared.
+template <int MODE, bool CX>
+__global__ __launch_bounds__(kBlock) void k_dense_columns(OpDev op, double* __restrict__ out) {
+  extern __shared__ unsigned char lds[];
+  Staged S = stage_op<MODE>(op, lds);
+  __syncthreads();
+  const int64_t col = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (col >= op.dim) return;
+  const uint64_t b = op.rows[col];
+  const int64_t base = col * op.dim;
+  for (int t = 0; t < op.n_diag; ++t) {
+    const DiagTerm d = op.diag[t];
+    if ((b & d.mask) != d.match) continue;
+    if (CX) {
+      out[2 * (base + col)] += d.are;
+      out[2 * (base + col) + 1] += d.aim;
+    } else {
+      out[base + col] += d.are;
+    }
+  }
+  double sqrt_own = 1.0;
+  if constexpr (MODE != kGroupNone) sqrt_own = sqrt((double)op.orbit[col]);
+  for (int ti = 0; ti < op.n_off; ++ti) {
+    const OffTerm t = S.off[ti];
+    if ((b & t.mask) != t.match) continue;
+    const uint64_t img = (b & ~t.mask) | t.image;
+    uint64_t s;
+    int h, cnt;
+    Canon<MODE>::run(op.g, S.L, img, s, h, cnt);
+    const int64_t idx = index_of(op.ix, S.W, s);
+    if (idx < 0) continue;
+    double wr, wi;
+    element_weight<MODE, true>(op, S, t, h, cnt, sqrt_own, wr, wi);
+    if (CX) {
+      out[2 * (base + idx)] += wr;
+      out[2 * (base + idx) + 1] += wi;
+    } else {
+      out[base + idx] += wr;
+    }
+  }
 }
 
 __global__ __launch_bounds__(kBlock) void k_sparse_compact(int64_t dim, int slot, const int64_t* __restrict__ colptr,
@@ -620,19 +669,37 @@ hipError_t launch_iterate(const OpDev& op, const OffTerm* all_terms, int n_all, 
   return hipGetLastError();
 }
 
-hipError_t launch_sparse_columns(const OpDev& op, double tol, int slot, int64_t* d_scratch_idx, double* d_scratch_re,
+hipError_t launch_sparse_columns(const OpDev& op, int transpose, double tol, int slot, int64_t n_sel,
+                                 const int64_t* d_sel, int64_t* d_scratch_idx, double* d_scratch_re,
                                  double* d_scratch_im, int64_t* d_count, hipStream_t st) {
+  if (n_sel == 0) return hipSuccess;
+  dim3 grid((unsigned)((n_sel + kBlock - 1) / kBlock));
+#define EDHIP_SC(M, T)                                                                                                 \
+  hipLaunchKernelGGL((k_sparse_columns<M, T>), grid, dim3(kBlock), op.lds_total, st, op, tol, slot, n_sel, d_sel,     \
+                     d_scratch_idx, d_scratch_re, d_scratch_im, d_count)
+  if (op.g.mode == kGroupNone) {
+    if (transpose) EDHIP_SC(kGroupNone, true); else EDHIP_SC(kGroupNone, false);
+  } else if (op.g.mode == kGroupGeneric) {
+    if (transpose) EDHIP_SC(kGroupGeneric, true); else EDHIP_SC(kGroupGeneric, false);
+  } else {
+    if (transpose) EDHIP_SC(kGroupTorus, true); else EDHIP_SC(kGroupTorus, false);
+  }
+#undef EDHIP_SC
+  return hipGetLastError();
+}
+
+hipError_t launch_dense_columns(const OpDev& op, int complex_out, double* d_out, hipStream_t st) {
   if (op.dim == 0) return hipSuccess;
   dim3 grid((unsigned)((op.dim + kBlock - 1) / kBlock));
-  if (op.g.mode == kGroupNone)
-    hipLaunchKernelGGL((k_sparse_columns<kGroupNone>), grid, dim3(kBlock), op.lds_total, st, op, tol, slot,
-                       d_scratch_idx, d_scratch_re, d_scratch_im, d_count);
-  else if (op.g.mode == kGroupGeneric)
-    hipLaunchKernelGGL((k_sparse_columns<kGroupGeneric>), grid, dim3(kBlock), op.lds_total, st, op, tol, slot,
-                       d_scratch_idx, d_scratch_re, d_scratch_im, d_count);
-  else
-    hipLaunchKernelGGL((k_sparse_columns<kGroupTorus>), grid, dim3(kBlock), op.lds_total, st, op, tol, slot,
-                       d_scratch_idx, d_scratch_re, d_scratch_im, d_count);
+#define EDHIP_DC(M, C) hipLaunchKernelGGL((k_dense_columns<M, C>), grid, dim3(kBlock), op.lds_total, st, op, d_out)
+  if (op.g.mode == kGroupNone) {
+    if (complex_out) EDHIP_DC(kGroupNone, true); else EDHIP_DC(kGroupNone, false);
+  } else if (op.g.mode == kGroupGeneric) {
+    if (complex_out) EDHIP_DC(kGroupGeneric, true); else EDHIP_DC(kGroupGeneric, false);
+  } else {
+    if (complex_out) EDHIP_DC(kGroupTorus, true); else EDHIP_DC(kGroupTorus, false);
+  }
+#undef EDHIP_DC
   return hipGetLastError();
 }
 

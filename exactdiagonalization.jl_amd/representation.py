@@ -703,21 +703,71 @@ class AbstractOperatorRepresentation:
     sparse_parallel = sparse
 
     def Matrix(self):
-        """Matrix(opr): dense sum of the column iterators (no chop)."""
-        return self.sparse(0.0).toarray()
+        """Matrix(opr) (abstract_operator_representation.jl:121-132): the dense
+        sum of the column iterators, no chop, built on the device."""
+        n = self.dimension
+        cx = self.is_complex
+        if n == 0:
+            return np.zeros((0, 0), np.complex128 if cx else np.float64)
+        flat = np.empty(n * n * (2 if cx else 1))
+        L = _lib.lib()
+        buf = ctypes.c_void_p()
+        self.ctx.check(L.edhip_malloc(self.ctx.handle, flat.nbytes, ctypes.byref(buf)))
+        try:
+            self.ctx.check(L.edhip_matrix(self._handle, buf))
+            self.ctx.check(L.edhip_memcpy(self.ctx.handle, flat.ctypes.data_as(ctypes.c_void_p), buf, flat.nbytes, 2))
+        finally:
+            self.ctx.check(L.edhip_free(self.ctx.handle, buf))
+        m = flat.view(np.complex128) if cx else flat
+        return m.reshape(n, n).T.copy()  # column-major on the device
+
+    def _vectors(self, which, transpose, tol):
+        """Merged, chopped, ascending (index, value) arrays of rows or columns."""
+        which = np.ascontiguousarray(np.atleast_1d(which), dtype=np.int64)
+        for i in which:
+            self._check_index(i)
+        slot = self._n_terms + 1
+        cnt = np.empty(which.size, np.int64)
+        idx = np.empty(which.size * slot, np.int64)
+        re = np.empty(which.size * slot)
+        im = np.empty(which.size * slot)
+        self.ctx.check(_lib.lib().edhip_get_vectors(
+            self._handle, int(transpose), which.size, ptr(which, ctypes.c_int64), float(tol), ptr(cnt, ctypes.c_int64),
+            ptr(idx, ctypes.c_int64), ptr(re, ctypes.c_double), ptr(im, ctypes.c_double)))
+        out = []
+        for k in range(which.size):
+            s = slice(k * slot, k * slot + int(cnt[k]))
+            v = re[s] + 1j * im[s] if self.is_complex else re[s].copy()
+            out.append((idx[s].copy(), v))
+        return out
 
     def get_row(self, irow):
-        self._check_index(irow)
-        return self.sparse().getrow(int(irow))
+        """get_row(opr, irow) (abstract_operator_representation.jl:217-229): the
+        merged row iterator with choptol!(rtoldefault); a 1 x dim sparse row."""
+        import scipy.sparse as sp
+
+        (i, v), = self._vectors([irow], False, RTOL_DEFAULT)
+        n = self.dimension
+        return sp.csr_matrix((v, i, np.array([0, i.size])), shape=(1, n))
 
     def get_column(self, icol):
-        self._check_index(icol)
-        return self.sparse().getcol(int(icol))
+        """get_column(opr, icol) (:232-243): the merged column iterator with
+        choptol!(rtoldefault); a dim x 1 sparse column."""
+        import scipy.sparse as sp
+
+        (i, v), = self._vectors([icol], True, RTOL_DEFAULT)
+        n = self.dimension
+        return sp.csc_matrix((v, i, np.array([0, i.size])), shape=(n, 1))
 
     def get_element(self, irow, icol):
+        """get_element(opr, irow, icol) (operator_representation.jl:109-119,
+        reduced_operator_representation.jl:120-138): the sum of column icol's
+        iterator entries at row irow."""
         self._check_index(irow)
-        self._check_index(icol)
-        return self.sparse(0.0)[int(irow), int(icol)]
+        (i, v), = self._vectors([icol], True, -1.0)
+        hit = np.nonzero(i == int(irow))[0]
+        zero = 0j if self.is_complex else 0.0
+        return v[hit[0]] if hit.size else zero
 
     def __getitem__(self, key):
         if isinstance(key, tuple) and len(key) == 2:

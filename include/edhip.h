@@ -217,6 +217,24 @@ int edhip_sparse_build(edhip_op* op, double tol, int64_t* nnz);
 int edhip_sparse_export(edhip_op* op, int64_t* colptr_host, int64_t* rowval_host, double* re_host,
                         double* im_host);
 
+/* get_row / get_column (abstract_operator_representation.jl:217-243) and the
+ * column sums behind get_element (operator_representation.jl:109-119,
+ * reduced_operator_representation.jl:120-138): for each of the n indices in
+ * which_host, the row (transpose = 0, row iterator) or column (transpose = 1,
+ * column iterator) of the matrix with duplicates merged, entries with |v| < tol
+ * dropped (choptol!, src/util.jl:88-94; tol < 0 keeps every merged entry) and
+ * ascending element indices.  Entry k fills counts_host[k] slots starting at
+ * k * (n_terms + 1) of idx_host / re_host / im_host (n_terms as passed to
+ * edhip_op_create).  EDHIP_ERR_BOUNDS for an index outside [0, dim). */
+int edhip_get_vectors(edhip_op* op, int transpose, int64_t n, const int64_t* which_host, double tol,
+                      int64_t* counts_host, int64_t* idx_host, double* re_host, double* im_host);
+
+/* Matrix(opr) (abstract_operator_representation.jl:121-132): the dense matrix,
+ * column-major (Julia's layout), into the device buffer out of dim * dim
+ * doubles (real operators) or dim * dim interleaved (re, im) pairs (complex
+ * operators, edhip_op_is_complex).  Dimensions above 2^20 are refused. */
+int edhip_matrix(edhip_op* op, void* out_device);
+
 /* ---- eigensolver (the reference's caller uses Arpack.eigs(opr; nev,
  * which=:SR), docs/src/examples/spinhalf.md:26, through mul!) ------------- */
 #define EDHIP_LANCZOS_FP64 0  /* FP64 (or ComplexF64) vectors throughout */

@@ -152,3 +152,72 @@ def test_spectrum_union_equals_full(ed):
         assert np.allclose(M, M.conj().T, atol=1e-13)
         ev.extend(np.linalg.eigvalsh(M))
     assert np.allclose(np.sort(ev), full, atol=1e-10)
+
+
+def _merged(pairs, n, tol):
+    """Host merge of iterator pairs: valid indices, duplicates summed, |v| < tol dropped."""
+    acc = {}
+    for i, v in pairs:
+        if 0 <= i < n:
+            acc[i] = acc.get(i, 0) + v
+    return {i: v for i, v in acc.items() if abs(v) >= tol}
+
+
+@pytest.mark.parametrize("k", [0, 3])
+def test_rows_columns_elements_vs_oracle(ed, oracle, k):
+    """get_row / get_column / get_element / Matrix from the device walks over
+    selected rows and columns against the oracle's dense matrix, on a
+    non-Hermitian reduced operator (rows and columns differ)."""
+    L = 10
+    hs, s = ed.spin_half_system(L)
+    op = ed.simplify(sum(((0.7 + 0.2j) * s(i, "+") * s((i + 1) % L, "-") + 0.3 * s(i, "z") * s((i + 2) % L, "z")
+                          for i in range(L)), ed.NullOperator()))
+    opr, o = _reduced(ed, oracle, hs, L, ed.TranslationIrrep(L, k), op)
+    D = o.dense()
+    n = opr.dimension
+    M = opr.Matrix()
+    assert np.abs(M - D).max() < 1e-14
+    tol = np.sqrt(np.finfo(float).eps)
+    for r in range(0, n, 5):
+        row = opr.get_row(r)
+        assert row.shape == (1, n)
+        dr = D[r].copy()
+        dr[np.abs(dr) < tol] = 0
+        assert np.abs(row.toarray().ravel() - dr).max() < 1e-14
+        assert np.array_equal(np.sort(row.indices), np.nonzero(dr)[0])
+        col = opr.get_column(r)
+        assert col.shape == (n, 1)
+        dc = D[:, r].copy()
+        dc[np.abs(dc) < tol] = 0
+        assert np.abs(col.toarray().ravel() - dc).max() < 1e-14
+        for c in range(0, n, 7):
+            assert abs(opr.get_element(r, c) - o.get_element(r, c)) < 1e-14
+            assert abs(opr[r, c] - D[r, c]) < 1e-14
+    with pytest.raises(ed.BoundsError):
+        opr.get_row(n)
+    with pytest.raises(ed.BoundsError):
+        opr.get_column(-1)
+    with pytest.raises(ed.BoundsError):
+        opr.get_element(0, n)
+
+
+def test_get_row_6x6_without_sparse(ed):
+    """A row and a column of the 6x6 J1-J2 k=(0,0) operator (252,091,362
+    representatives) straight from the device walk: equal to the host merge of
+    the row iterator, and row = column (a real symmetric matrix)."""
+    hs, _ = ed.spin_half_system(36)
+    j1, j2 = ed.square_bonds(6, 6)
+    H = _heis(ed, hs, j1 + j2, [1.0] * len(j1) + [0.5] * len(j2))
+    r = ed.symmetry_reduce(ed.represent(ed.HilbertSpaceSector(hs, 0)), ed.TranslationIrrep((6, 6), (0, 0)))
+    opr = ed.represent(r, H)
+    tol = np.sqrt(np.finfo(float).eps)
+    for irow in (0, 12345, 98765432, r.dimension - 1):
+        row = opr.get_row(irow)
+        ref = _merged(opr.get_row_iterator(irow), r.dimension, tol)
+        assert sorted(ref) == row.indices.tolist()
+        got = dict(zip(row.indices.tolist(), row.data.tolist()))
+        assert max(abs(got[i] - ref[i]) for i in ref) < 1e-13
+        col = opr.get_column(irow)
+        assert col.indices.tolist() == row.indices.tolist()
+        assert np.abs(col.data - row.data).max() < 1e-13
+        assert abs(opr.get_element(irow, irow) - ref.get(irow, 0.0)) < 1e-13
