@@ -58,6 +58,6 @@ from .representation import (
     DirectProductOperation,
     make_product_irrep,
 )
-from .solve import EigsResult, eigs, minres, term_expectations
+from .solve import EigsResult, eigs, minres, release_workspace, term_expectations
 
 __all__ = [n for n in dir() if not n.startswith("_")]

@@ -76,7 +76,7 @@ class LanczosOpts(ctypes.Structure):
         ("max_refine", ctypes.c_int),
         ("check_every", ctypes.c_int),
         ("criterion", ctypes.c_int),
-        ("pad_", ctypes.c_int),
+        ("reorth", ctypes.c_int),
         ("seed", ctypes.c_uint64),
         ("tol", ctypes.c_double),
     ]
@@ -89,7 +89,7 @@ class LanczosStats(ctypes.Structure):
         ("matvecs", ctypes.c_int),
         ("matvecs_fp64", ctypes.c_int),
         ("converged", ctypes.c_int),
-        ("pad_", ctypes.c_int),
+        ("second_passes", ctypes.c_int),
         ("residual", ctypes.c_double * 16),
         ("unrefined", ctypes.c_double * 16),
         ("t_total", ctypes.c_double),
@@ -114,6 +114,7 @@ SIGNATURES = {
     "edhip_synchronize": (_i32, [_vp]),
     "edhip_malloc": (_i32, [_vp, _u64, _P(_vp)]),
     "edhip_free": (_i32, [_vp, _vp]),
+    "edhip_release_workspace": (_i32, [_vp]),
     "edhip_memcpy": (_i32, [_vp, _vp, _vp, _u64, _i32]),
     "edhip_space_sector": (_i32, [_vp, _i32, _i32, _P(ctypes.c_int32), _P(_vp)]),
     "edhip_space_from_list": (_i32, [_vp, _i32, _i64, _P(_u64), _P(_vp)]),

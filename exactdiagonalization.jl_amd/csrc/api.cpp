@@ -103,7 +103,15 @@ struct edhip_ctx {
   std::string last_error;
   DevBuf partial;  // reduction partials
   DevBuf small;    // small results
+  // Eigensolver / MINRES vectors, kept between calls (grow-only): repeated
+  // allocations of hundreds of GB after frees cost seconds per call.
+  DevBuf krylov;
   hipEvent_t ev[16] = {};
+
+  char* krylov_arena(size_t bytes) {
+    krylov.ensure(bytes);
+    return krylov.as<char>();
+  }
 
   double* partial_buf(size_t doubles) {
     partial.ensure(doubles * sizeof(double));
@@ -1126,14 +1134,23 @@ struct Timer {
 };
 
 // Thick-restart Lanczos on one operator in one storage precision.
+// Ritz vectors kept at a thick restart of an mm-vector basis when n_ret
+// values are wanted (nondecreasing in both).
+inline int keep_count(int mm, int n_ret) {
+  return std::max(1, std::min(mm - 1, std::max(n_ret + 2, std::min(mm / 3, n_ret + 16))));
+}
+
+inline size_t align256(size_t b) { return (b + 255) / 256 * 256; }
+
 struct TRLanczos {
   edhip_op* op;
   int prec;        // storage precision of the Krylov vectors
   int64_t n;
   int64_t ld = 0;  // stride of the basis vectors (n rounded up to 4 elements: 16-byte aligned vectors)
   int m;           // basis size
-  DevBuf V;        // (m + 1) vectors of stride ld
-  DevBuf tmp;      // restart combination scratch
+  void* Vp = nullptr;    // (m + 1) vectors of stride ld
+  void* tmpp = nullptr;  // restart combination scratch
+  size_t tmp_bytes = 0;
   DevBuf coef_d;   // device coefficients
   const void* locked = nullptr;  // locked (deflated) vectors, same precision
   int n_locked = 0;
@@ -1146,7 +1163,7 @@ struct TRLanczos {
     n = dim;
     ld = (dim + 3) / 4 * 4;
   }
-  void* vec(int j) const { return (char*)V.p + (size_t)j * ld * es(); }
+  void* vec(int j) const { return (char*)Vp + (size_t)j * ld * es(); }
 
   void matvec(const void* x, void* y) {
     edhip_ctx* ctx = op->ctx;
@@ -1171,23 +1188,47 @@ struct TRLanczos {
     return {h[0], h[1]};
   }
 
-  // w -= B (B^H w) twice over vectors B_0..B_{k-1} (stride ldb); returns the
-  // first-pass coefficients (host).
+  // Reorthogonalisation: 0 = one classical Gram-Schmidt pass, repeated when
+  // the pass removed most of w; 1 = always two passes.
+  int reorth = 0;
+  // The second pass runs when ||w - B c||^2 < kReorthKeep ||w||^2, i.e. the
+  // projection cancelled more than 1 - 1/kappa of w with kappa = 10 (a looser
+  // threshold than the Daniel-Gragg-Kaufman-Stewart 1/sqrt(2)): one pass leaves
+  // components along B of order kappa * eps relative to the result.  For
+  // orthonormal B, ||w - B c||^2 = ||w||^2 - ||c||^2 needs no extra pass.
+  static constexpr double kReorthKeep = 1e-2;
+  int second_passes = 0;
+
+  // w -= B (B^H w) over vectors B_0..B_{k-1} (stride ldb), once or twice;
+  // returns the first-pass coefficients (host).
   std::vector<double> orthogonalize(const void* B, int64_t ldb, int k, void* w) {
     edhip_ctx* ctx = op->ctx;
     std::vector<double> c1(2 * (size_t)k, 0.0);
     if (k <= 0) return c1;
-    double* part = ctx->partial_buf((size_t)2 * k * 1024);
-    coef_d.ensure((size_t)2 * k * sizeof(double));
-    for (int pass = 0; pass < 2; ++pass) {
+    const bool adaptive = reorth == 0;
+    double* part = ctx->partial_buf(std::max<size_t>((size_t)2 * k * 1024, 2 * (size_t)vec_partial_blocks(n)));
+    coef_d.ensure((size_t)(2 * k + 2) * sizeof(double));
+    double* nn_d = coef_d.as<double>() + 2 * k;
+    std::vector<double> hc(2 * (size_t)k + 2, 0.0);
+    if (adaptive) HIP_OR_THROW(vec_dot(prec, n, w, w, part, nn_d, ctx->stream));
+    HIP_OR_THROW(vec_multi_dot(prec, n, k, B, ldb, w, part, coef_d.as<double>(), ctx->stream));
+    HIP_OR_THROW(hipMemcpyAsync(hc.data(), coef_d.p, hc.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_OR_THROW(vec_multi_axpy(prec, n, k, B, ldb, coef_d.as<double>(), w, ctx->stream));
+    ctx->sync();
+    std::copy(hc.begin(), hc.begin() + 2 * (size_t)k, c1.begin());
+    bool again = !adaptive;
+    if (adaptive) {
+      double cc = 0.0;
+      for (int i = 0; i < 2 * k; ++i) cc += hc[i] * hc[i];
+      const double ww = hc[2 * (size_t)k];
+      again = ww - cc < kReorthKeep * ww;
+    }
+    if (again) {
+      ++second_passes;
       HIP_OR_THROW(vec_multi_dot(prec, n, k, B, ldb, w, part, coef_d.as<double>(), ctx->stream));
       HIP_OR_THROW(vec_multi_axpy(prec, n, k, B, ldb, coef_d.as<double>(), w, ctx->stream));
-      if (pass == 0) {
-        HIP_OR_THROW(hipMemcpyAsync(c1.data(), coef_d.p, 2 * (size_t)k * sizeof(double), hipMemcpyDeviceToHost,
-                                    ctx->stream));
-      }
+      ctx->sync();
     }
-    ctx->sync();
     return c1;
   }
 
@@ -1229,7 +1270,7 @@ struct TRLanczos {
         matvec(vj, wv);
         ++iterations;
         if (n_locked) orthogonalize(locked, n, n_locked, wv);
-        std::vector<double> c = orthogonalize(V.p, ld, j + 1, wv);
+        std::vector<double> c = orthogonalize(Vp, ld, j + 1, wv);
         T[(size_t)j * mm + j] = c[2 * (size_t)j];
         std::complex<double> nn = dot(wv, wv);
         double beta = std::sqrt(std::max(nn.real(), 0.0));
@@ -1277,7 +1318,7 @@ struct TRLanczos {
       for (int a = 0; a < mm; ++a)
         for (int b = 0; b < mm; ++b) Tm[(size_t)a * mm + b] = T[(size_t)a * mm + b];
       sym_eig(mm, Tm, !arrow, w, Z);
-      int keep = std::min(mm - 1, std::max(n_ret + 2, std::min(mm / 3, n_ret + 16)));
+      int keep = keep_count(mm, n_ret);
       // new basis: V'_i = sum_j Z[j][i] V_j (i < keep), V'_keep = normalised residual
       std::vector<double> coef((size_t)mm * keep);
       for (int a = 0; a < mm; ++a)
@@ -1285,9 +1326,9 @@ struct TRLanczos {
       coef_d.ensure(coef.size() * sizeof(double));
       HIP_OR_THROW(hipMemcpyAsync(coef_d.p, coef.data(), coef.size() * sizeof(double), hipMemcpyHostToDevice,
                                   ctx->stream));
-      tmp.ensure((size_t)keep * ld * es());
-      HIP_OR_THROW(vec_combine(prec, prec, n, mm, V.p, ld, keep, coef_d.as<double>(), tmp.p, ld, ctx->stream));
-      HIP_OR_THROW(hipMemcpyAsync(V.p, tmp.p, (size_t)keep * ld * es(), hipMemcpyDeviceToDevice, ctx->stream));
+      if ((size_t)keep * ld * es() > tmp_bytes) fail(EDHIP_ERR_ARGUMENT, "restart scratch too small");
+      HIP_OR_THROW(vec_combine(prec, prec, n, mm, Vp, ld, keep, coef_d.as<double>(), tmpp, ld, ctx->stream));
+      HIP_OR_THROW(hipMemcpyAsync(Vp, tmpp, (size_t)keep * ld * es(), hipMemcpyDeviceToDevice, ctx->stream));
       // residual direction: w = vec(mm) (unnormalised), beta_last its norm
       scale(vec(mm), 1.0 / beta_last);
       HIP_OR_THROW(hipMemcpyAsync(vec(keep), vec(mm), (size_t)n * es(), hipMemcpyDeviceToDevice, ctx->stream));
@@ -1310,7 +1351,7 @@ struct TRLanczos {
       coef_d.ensure(coef.size() * sizeof(double));
       HIP_OR_THROW(hipMemcpyAsync(coef_d.p, coef.data(), coef.size() * sizeof(double), hipMemcpyHostToDevice,
                                   ctx->stream));
-      HIP_OR_THROW(vec_combine(prec, out_prec, n, sz, V.p, ld, nev, coef_d.as<double>(), out, n, ctx->stream));
+      HIP_OR_THROW(vec_combine(prec, out_prec, n, sz, Vp, ld, nev, coef_d.as<double>(), out, n, ctx->stream));
       ctx->sync();
     }
     (void)dim;
@@ -1377,6 +1418,14 @@ int edhip_synchronize(edhip_ctx* ctx) {
 int edhip_malloc(edhip_ctx* ctx, uint64_t bytes, void** out) {
   if (!ctx || !out) return EDHIP_ERR_ARGUMENT;
   return guarded(ctx, [&] { HIP_OR_THROW(hipMalloc(out, bytes ? bytes : 16)); });
+}
+
+int edhip_release_workspace(edhip_ctx* ctx) {
+  if (!ctx) return EDHIP_ERR_ARGUMENT;
+  return guarded(ctx, [&] {
+    ctx->sync();
+    ctx->krylov.release();
+  });
 }
 
 int edhip_free(edhip_ctx* ctx, void* p) {
@@ -2053,32 +2102,43 @@ int edhip_lanczos(edhip_op* op, const edhip_lanczos_opts* opts, double* evals, v
     edhip_lanczos_stats st{};
     std::vector<double> theta, res;
 
+    if (opts->reorth < 0 || opts->reorth > 1) fail(EDHIP_ERR_ARGUMENT, "reorth must be 0 or 1");
     TRLanczos L;
     L.op = op;
+    L.reorth = opts->reorth;
     L.prec = prec_store;
     L.set_size(n);
     L.m = (int)std::min<int64_t>(opts->max_basis, n);
-    L.V.alloc((size_t)(L.m + 1) * L.ld * elem_size(prec_store));
+    const bool mixed = opts->mode == EDHIP_LANCZOS_MIXED;
+    // One workspace: the FP64 Ritz vectors, then the main basis and its restart
+    // scratch, later reused by the refinement cycles' basis and scratch.
+    const size_t es_s = elem_size(prec_store), es64 = elem_size(prec64);
+    const int m_ref = (int)std::min<int64_t>(std::max(opts->refine_basis, 2), n);
+    const size_t ritz_b = align256((size_t)nev * n * es64);
+    const size_t main_v = align256((size_t)(L.m + 1) * L.ld * es_s);
+    const size_t main_t = align256((size_t)keep_count(L.m, nev + 1) * L.ld * es_s);
+    const size_t ref_v = mixed ? align256((size_t)(m_ref + 1) * L.ld * es64) : 0;
+    const size_t ref_t = mixed ? align256((size_t)keep_count(m_ref, 2) * L.ld * es64) : 0;
+    char* arena = ctx->krylov_arena(ritz_b + std::max(main_v + main_t, ref_v + ref_t));
+    void* ritz = arena;
+    L.Vp = arena + ritz_b;
+    L.tmpp = arena + ritz_b + main_v;
+    L.tmp_bytes = main_t;
     HIP_OR_THROW(vec_gaussian(prec_store, n, opts->seed, L.vec(0), ctx->stream));
     double nrm = std::sqrt(L.dot(L.vec(0), L.vec(0)).real());
     L.scale(L.vec(0), 1.0 / nrm);
 
-    // Ritz vectors of the main run, FP64 (or C128)
-    DevBuf ritz;
-    ritz.alloc((size_t)nev * n * elem_size(prec64));
-    const bool mixed = opts->mode == EDHIP_LANCZOS_MIXED;
     // The FP32 phase stops on the eigenvalue criterion (its residual cannot go
     // below single-precision rounding); the FP64 phase uses the caller's rule.
     L.criterion = mixed ? 1 : opts->criterion;
     const double tol_main = mixed ? std::max(opts->tol, 1e-8) : opts->tol;
-    bool conv = L.run(nev, tol_main, opts->max_restarts, opts->check_every, theta, res, ritz.p, prec64, nev + 1);
+    bool conv = L.run(nev, tol_main, opts->max_restarts, opts->check_every, theta, res, ritz, prec64, nev + 1);
     st.iterations = L.iterations;
     st.restarts = L.restarts;
     st.matvecs = L.matvecs;
+    st.second_passes = L.second_passes;
     st.t_matvec = L.t_matvec;
     for (int i = 0; i < nev && i < 16; ++i) st.unrefined[i] = theta[i];
-    L.V.release();
-    L.tmp.release();
 
     if (mixed) {
       Timer tr;
@@ -2090,18 +2150,21 @@ int edhip_lanczos(edhip_op* op, const edhip_lanczos_opts* opts, double* evals, v
       for (int i = 0; i < nev; ++i) {
         TRLanczos R;
         R.op = op;
+        R.reorth = opts->reorth;
         R.prec = prec64;
         R.set_size(n);
         R.m = (int)std::min<int64_t>(std::max(opts->refine_basis, 2), n - i);
         R.criterion = opts->criterion;
         R.gap_hint = (theta32.size() > (size_t)i + 1) ? (theta32[i + 1] - theta32[i]) : 0.0;
-        R.V.alloc((size_t)(R.m + 1) * R.ld * elem_size(prec64));
-        R.locked = ritz.p;
+        R.Vp = arena + ritz_b;
+        R.tmpp = arena + ritz_b + ref_v;
+        R.tmp_bytes = ref_t;
+        R.locked = ritz;
         R.n_locked = i;
-        void* start = (char*)ritz.p + (size_t)i * n * elem_size(prec64);
+        void* start = (char*)ritz + (size_t)i * n * elem_size(prec64);
         HIP_OR_THROW(hipMemcpyAsync(R.vec(0), start, (size_t)n * elem_size(prec64), hipMemcpyDeviceToDevice,
                                     ctx->stream));
-        if (i) R.orthogonalize(ritz.p, n, i, R.vec(0));
+        if (i) R.orthogonalize(ritz, n, i, R.vec(0));
         double nr = std::sqrt(R.dot(R.vec(0), R.vec(0)).real());
         R.scale(R.vec(0), 1.0 / nr);
         std::vector<double> th1, rs1;
@@ -2109,6 +2172,7 @@ int edhip_lanczos(edhip_op* op, const edhip_lanczos_opts* opts, double* evals, v
         theta[i] = th1[0];
         res[i] = rs1.empty() ? 0.0 : rs1[0];
         st.matvecs_fp64 += R.matvecs;
+        st.second_passes += R.second_passes;
         conv = conv && ok;
       }
       st.t_refine = tr.sec();
@@ -2118,7 +2182,7 @@ int edhip_lanczos(edhip_op* op, const edhip_lanczos_opts* opts, double* evals, v
       if (i < 16) st.residual[i] = res.size() > (size_t)i ? res[i] / std::max(1.0, std::fabs(theta[i])) : 0.0;
     }
     if (evecs && opts->want_vectors)
-      HIP_OR_THROW(hipMemcpyAsync(evecs, ritz.p, (size_t)nev * n * elem_size(prec64), hipMemcpyDeviceToDevice,
+      HIP_OR_THROW(hipMemcpyAsync(evecs, ritz, (size_t)nev * n * elem_size(prec64), hipMemcpyDeviceToDevice,
                                   ctx->stream));
     ctx->sync();
     st.converged = conv ? 1 : 0;
@@ -2172,9 +2236,9 @@ int edhip_minres(edhip_op* op, int prec, double shift, int n_deflate, const void
     H.op = op;
     H.prec = prec;
     H.set_size(n);
-    DevBuf buf;
-    buf.alloc(bytes * 7);
-    auto V = [&](int i) { return (void*)((char*)buf.p + (size_t)i * bytes); };
+    const size_t vb = align256(bytes);
+    char* buf = ctx->krylov_arena(vb * 7);
+    auto V = [&](int i) { return (void*)(buf + (size_t)i * vb); };
     void *r1 = V(0), *r2 = V(1), *y = V(2), *v = V(3), *w = V(4), *w1 = V(5), *w2 = V(6);
     auto project = [&](void* z) {
       if (n_deflate) H.orthogonalize(deflate, n, n_deflate, z);

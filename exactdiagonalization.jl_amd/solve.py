@@ -32,11 +32,13 @@ class EigsResult:
     t_total: float
     t_matvec: float
     t_refine: float
+    second_passes: int = 0
 
 
 def eigs(opr, nev: int = 1, which: str = "SR", *, mode: str = "fp64", tol: float = 1e-10, max_basis: int = 120,
          max_restarts: int = 50, seed: int = 20241202, return_vectors: bool = False, refine_basis: int = 12,
-         max_refine: int = 20, check_every: int = 5, criterion: str = "residual") -> EigsResult:
+         max_refine: int = 20, check_every: int = 5, criterion: str = "residual",
+         reorth: str = "adaptive") -> EigsResult:
     """Lowest ``nev`` eigenvalues of a Hermitian operator representation.
 
     mode="fp64": FP64 (ComplexF64) vectors.  mode="mixed": FP32 (ComplexF32)
@@ -47,6 +49,10 @@ def eigs(opr, nev: int = 1, which: str = "SR", *, mode: str = "fp64", tol: float
     criterion="residual": converged when ||H y - theta y|| <= tol*max(1,|theta|).
     criterion="eigenvalue": also when r^2/gap <= tol*max(1,|theta|) with theta
     stationary between checks (a bound on the eigenvalue error).
+
+    reorth="adaptive": one Gram-Schmidt pass against the Krylov basis per step,
+    a second when the first cancelled more than 99% of the vector's norm
+    squared; reorth="twice": always two passes.
     """
     if which != "SR":
         raise ArgumentError("only which='SR' (smallest real) is supported")
@@ -61,6 +67,9 @@ def eigs(opr, nev: int = 1, which: str = "SR", *, mode: str = "fp64", tol: float
     o.max_refine = int(max_refine)
     o.check_every = int(check_every)
     o.criterion = 1 if criterion == "eigenvalue" else 0
+    if reorth not in ("adaptive", "twice"):
+        raise ArgumentError("reorth must be 'adaptive' or 'twice'")
+    o.reorth = 0 if reorth == "adaptive" else 1
     o.seed = int(seed)
     o.tol = float(tol)
     st = _lib.LanczosStats()
@@ -87,12 +96,19 @@ def eigs(opr, nev: int = 1, which: str = "SR", *, mode: str = "fp64", tol: float
         matvecs=st.matvecs,
         matvecs_fp64=st.matvecs_fp64,
         converged=bool(st.converged),
+        second_passes=st.second_passes,
         residuals=np.array(st.residual[:nev]),
         unrefined=np.array(st.unrefined[:nev]),
         t_total=st.t_total,
         t_matvec=st.t_matvec,
         t_refine=st.t_refine,
     )
+
+
+def release_workspace(ctx=None):
+    """Free the device workspace that eigs and minres keep between calls."""
+    ctx = ctx or _lib.default_context()
+    ctx.check(_lib.lib().edhip_release_workspace(ctx.handle))
 
 
 def minres(opr, b, *, shift: float = 0.0, deflate=None, tol: float = 1e-10, max_iter: int = 2000):

@@ -62,6 +62,10 @@ int edhip_synchronize(edhip_ctx* ctx);
 /* Device memory of the context's device (plain hipMalloc / hipFree). */
 int edhip_malloc(edhip_ctx* ctx, uint64_t bytes, void** out);
 int edhip_free(edhip_ctx* ctx, void* ptr);
+/* The eigensolver and MINRES keep their vectors (Krylov basis, restart
+ * scratch, Ritz vectors) in one device workspace per context that grows to
+ * the largest request and is reused by later calls; this frees it. */
+int edhip_release_workspace(edhip_ctx* ctx);
 int edhip_memcpy(edhip_ctx* ctx, void* dst, const void* src, uint64_t bytes, int kind /* hipMemcpyKind */);
 
 /* ---- spaces ------------------------------------------------------------- */
@@ -252,7 +256,9 @@ typedef struct {
   int criterion;     /* 0: residual ||H y - theta y|| <= tol*max(1,|theta|);
                         1: also accept r^2/gap <= tol*max(1,|theta|) with theta
                         stationary between checks (eigenvalue accuracy) */
-  int pad_;
+  int reorth;        /* 0: one classical Gram-Schmidt pass against the basis,
+                        a second one when the first removed more than 99% of
+                        ||w||^2; 1: always two passes (CGS2) */
   uint64_t seed;     /* start vector: SplitMix64 + Box-Muller Gaussian */
   double tol;        /* see criterion */
 } edhip_lanczos_opts;
@@ -263,7 +269,7 @@ typedef struct {
   int matvecs;        /* matvecs in storage precision */
   int matvecs_fp64;   /* FP64 matvecs of the refinement */
   int converged;
-  int pad_;
+  int second_passes;  /* steps whose reorthogonalisation took a second pass */
   double residual[16];     /* final ||H y - theta y|| / max(1,|theta|) (FP64) */
   double unrefined[16];    /* mixed mode: Ritz values before refinement */
   double t_total;          /* seconds */

@@ -85,3 +85,45 @@ def test_full_size_configs_vs_arpack(ed, oracle, name, shape, k, nev):
     assert np.all(np.abs(r64.values - ref) < 1e-12 * np.abs(ref))
     rmx = ed.eigs(opr, nev, tol=1e-11, mode="mixed", max_basis=200, criterion="eigenvalue")
     assert np.all(np.abs(rmx.values - r64.values) < 1e-10 * np.abs(r64.values))
+
+
+@pytest.mark.parametrize("mode", ["fp64", "mixed"])
+def test_reorth_adaptive_matches_twice(ed, golden, mode):
+    """One Gram-Schmidt pass per step (second pass only on heavy cancellation)
+    gives the same four lowest levels as always-twice CGS2, within the
+    precision bar of the mode, with thick restarts in play; the eigenvectors
+    stay orthonormal."""
+    L = 16
+    hs, _ = ed.spin_half_system(L)
+    H = _heis(ed, hs, [(i, (i + 1) % L) for i in range(L)])
+    opr = ed.represent(ed.represent(ed.HilbertSpaceSector(hs, 0)), H)
+    ref = np.array(golden["eigenvalues_brute_force"]["chain16"])
+    kw = dict(tol=1e-12 if mode == "fp64" else 1e-11, mode=mode, max_basis=40, max_restarts=400)
+    a = ed.eigs(opr, 4, reorth="adaptive", return_vectors=True, **kw)
+    t = ed.eigs(opr, 4, reorth="twice", **kw)
+    bar = 1e-11 if mode == "fp64" else 1e-10
+    assert a.converged and t.converged and a.restarts > 0
+    assert np.all(np.abs(a.values - ref) < bar * np.abs(ref))
+    assert np.all(np.abs(t.values - ref) < bar * np.abs(ref))
+    assert a.second_passes < t.second_passes
+    V = a.vectors.cpu().numpy()
+    assert np.abs(V @ V.T - np.eye(4)).max() < 1e-9
+    with pytest.raises(ed.ArgumentError):
+        ed.eigs(opr, 1, reorth="never")
+
+
+def test_workspace_reuse_repeatable(ed):
+    """Calls that reuse the context's workspace (growing it, shrinking the
+    request, after a release) give bitwise the same results."""
+    L = 14
+    hs, _ = ed.spin_half_system(L)
+    H = _heis(ed, hs, [(i, (i + 1) % L) for i in range(L)])
+    opr = ed.represent(ed.symmetry_reduce(ed.represent(ed.HilbertSpaceSector(hs, 0)), ed.TranslationIrrep(L, 2)), H)
+    runs = []
+    for mode, basis in (("mixed", 30), ("fp64", 60), ("mixed", 30), ("fp64", 60)):
+        runs.append(ed.eigs(opr, 2, tol=1e-11, mode=mode, max_basis=basis, max_restarts=200).values)
+    ed.release_workspace(opr.ctx)
+    runs.append(ed.eigs(opr, 2, tol=1e-11, mode="fp64", max_basis=60, max_restarts=200).values)
+    assert np.array_equal(runs[0], runs[2])
+    assert np.array_equal(runs[1], runs[3]) and np.array_equal(runs[1], runs[4])
+    assert np.all(np.abs(runs[0] - runs[1]) < 1e-10 * np.abs(runs[1]))
