@@ -186,6 +186,34 @@ __device__ __forceinline__ int64_t index_of(const IndexDev& ix, const uint64_t* 
   return search_sorted(ix.keys, ix.prefix, ix.n_buckets, ix.shift, key);
 }
 
+// Two lookups with their loads issued together.
+__device__ __forceinline__ void index_of2(const IndexDev& ix, const uint64_t* W_lds, uint64_t k1, uint64_t k2,
+                                          int64_t& i1, int64_t& i2) {
+  if (ix.bm_bits) {
+    const bool in1 = (k1 >> ix.bm_bits) == 0, in2 = (k2 >> ix.bm_bits) == 0;
+    const uint64_t a1 = in1 ? k1 : 0, a2 = in2 ? k2 : 0;
+    const uint32_t j1 = __umulhi((uint32_t)(a1 >> 7), 0xAAAAAAABu) >> 1;
+    const uint32_t j2 = __umulhi((uint32_t)(a2 >> 7), 0xAAAAAAABu) >> 1;
+    const uint32_t w1 = (uint32_t)(a1 - (uint64_t)j1 * kRankBlockWords);
+    const uint32_t w2 = (uint32_t)(a2 - (uint64_t)j2 * kRankBlockWords);
+    const uint64_t* b1 = ix.bitmap + (uint64_t)j1 * 8u;
+    const uint64_t* b2 = ix.bitmap + (uint64_t)j2 * 8u;
+    const uint64_t word1 = b1[w1 >> 6], word2 = b2[w2 >> 6];
+    const ulonglong2 t1 = *(const ulonglong2*)(b1 + 6);
+    const ulonglong2 t2 = *(const ulonglong2*)(b2 + 6);
+    const uint32_t s1 = (w1 >> 6) ? (uint32_t)(t1.y >> (9u * ((w1 >> 6) - 1u))) & 511u : 0u;
+    const uint32_t s2 = (w2 >> 6) ? (uint32_t)(t2.y >> (9u * ((w2 >> 6) - 1u))) & 511u : 0u;
+    const uint32_t bit1 = w1 & 63u, bit2 = w2 & 63u;
+    const int64_t r1 = (int64_t)(t1.x + s1 + (uint64_t)__builtin_popcountll(word1 & ((1ull << bit1) - 1ull)));
+    const int64_t r2 = (int64_t)(t2.x + s2 + (uint64_t)__builtin_popcountll(word2 & ((1ull << bit2) - 1ull)));
+    i1 = (in1 && ((word1 >> bit1) & 1ull)) ? r1 : -1;
+    i2 = (in2 && ((word2 >> bit2) & 1ull)) ? r2 : -1;
+    return;
+  }
+  i1 = index_of(ix, W_lds, k1);
+  i2 = index_of(ix, W_lds, k2);
+}
+
 // ----------------------------------------------------------------------------
 // Symmetry group description on the device.
 // ----------------------------------------------------------------------------
