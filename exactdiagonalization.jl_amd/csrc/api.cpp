@@ -721,6 +721,8 @@ struct edhip_op {
   bool complex_amp = false;
   // device terms
   DevBuf off_row, off_col, diag, all_row, all_col;
+  DevBuf mv_row, mv_col;  // paired copies of off_row / off_col for the matvec
+  int n_mv = 0;
   int n_off = 0, n_diag = 0;
   std::vector<int64_t> diag_ids, off_ids;  // original term index of each device term
   OpDev dev_row{}, dev_col{};
@@ -789,9 +791,17 @@ OpDev make_opdev(edhip_op* op, bool transpose) {
   d.n_diag = op->n_diag;
   d.off = transpose ? op->off_col.as<OffTerm>() : op->off_row.as<OffTerm>();
   d.diag = op->diag.as<DiagTerm>();
-  const uint32_t op_bytes = align_up((uint32_t)(op->n_off * sizeof(OffTerm)), 16) +
-                            (d.ix.rank_mode ? (uint32_t)((sp->n_bits + 1) * (sp->n_bits + 1) * 8) : 0u);
-  if (d.g.pair_table && align_up(d.lay.total, 16) + op_bytes > 64 * 1024) {
+  // the matvec stages 36 bytes per padded term, the other kernels 40 per term
+  const uint32_t w_bytes = d.ix.rank_mode ? (uint32_t)((sp->n_bits + 1) * (sp->n_bits + 1) * 8) : 0u;
+  const uint32_t plain_bytes = align_up((uint32_t)(op->n_off * sizeof(OffTerm)), 16);
+  const uint32_t paired_bytes = std::max(plain_bytes, align_up((uint32_t)(op->n_mv * 36), 16));
+  const uint32_t group_bytes = align_up(d.lay.total, 16);
+  d.mv_off = transpose ? op->mv_col.as<OffTerm>() : op->mv_row.as<OffTerm>();
+  d.n_mv = op->n_mv;
+  if (group_bytes + paired_bytes + w_bytes > 64 * 1024 && group_bytes + plain_bytes + w_bytes <= 64 * 1024)
+    d.n_mv = 0;  // no room for the paired copy: match term by term
+  const uint32_t op_bytes = (d.n_mv ? paired_bytes : plain_bytes) + w_bytes;
+  if (d.g.pair_table && group_bytes + op_bytes > 64 * 1024) {
     // not enough LDS next to this operator's terms: use the per-row table path
     d.g.pair_table = 0;
     d.lay.bytes_pair = 0;
@@ -799,7 +809,7 @@ OpDev make_opdev(edhip_op* op, bool transpose) {
   }
   uint32_t off = align_up(d.lay.total, 16);
   d.off_lds = off;
-  off += (uint32_t)(op->n_off * sizeof(OffTerm));
+  off += d.n_mv ? paired_bytes : plain_bytes;
   off = align_up(off, 16);
   d.w_lds = off;
   d.w_bytes = d.ix.rank_mode ? (uint32_t)((sp->n_bits + 1) * (sp->n_bits + 1) * 8) : 0u;
@@ -1851,6 +1861,24 @@ int edhip_op_create(edhip_ctx* ctx, const edhip_space* space, int64_t n_terms, c
     op->n_diag = (int)dg.size();
     upload(op->off_row, offr, ctx->stream);
     upload(op->off_col, offc, ctx->stream);
+    // runs of equal masks padded to even lengths (both forms share the masks,
+    // so they pad alike); the padding term keeps the mask, matches nothing
+    // (a match outside the mask) and flips nothing
+    auto pad_runs = [](const std::vector<OffTerm>& v) {
+      std::vector<OffTerm> out;
+      for (size_t i = 0; i < v.size();) {
+        size_t j = i;
+        while (j < v.size() && v[j].mask == v[i].mask) ++j;
+        out.insert(out.end(), v.begin() + i, v.begin() + j);
+        if ((j - i) & 1) out.push_back(OffTerm{v[i].mask, ~v[i].mask, ~v[i].mask, 0.0, 0.0});
+        i = j;
+      }
+      return out;
+    };
+    const std::vector<OffTerm> mvr = pad_runs(offr), mvc = pad_runs(offc);
+    op->n_mv = (int)mvr.size();
+    upload(op->mv_row, mvr, ctx->stream);
+    upload(op->mv_col, mvc, ctx->stream);
     upload(op->diag, dg, ctx->stream);
     upload(op->all_row, allr, ctx->stream);
     upload(op->all_col, allc, ctx->stream);

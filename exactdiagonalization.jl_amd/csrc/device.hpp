@@ -338,7 +338,7 @@ __device__ __forceinline__ void row_minrot(const GroupDev& g, const GroupLds& L,
 // position pos = a + Lx * bb of a translation T(pos) reaching it (among ties
 // the one with the largest invt[pos]) and cnt = #{T : T(b) = s}.
 // invt: inverse indices of the elements T(pos) p of the coset being scanned.
-template <int LXC = 0, int LYC = 0>
+template <int LXC = 0, int LYC = 0, bool NEED_H = true>
 __device__ __forceinline__ void canonicalize_torus_pairs_t(const GroupDev& g, const GroupLds& L, const int32_t* invt,
                                                            uint64_t b, uint64_t& s, int& pos, int& cnt) {
   const int Lx = LXC ? LXC : g.Lx;
@@ -377,16 +377,16 @@ __device__ __forceinline__ void canonicalize_torus_pairs_t(const GroupDev& g, co
     const uint64_t t = ((b << sh) | (b >> (NB - sh))) & FULL;
     const uint64_t img = ((t << a) & L.rot_hi[a]) | ((t >> (Lx - a)) & L.rot_lo[a]);
     const bool lt = img < s, eq = img == s;
-    if (eq) {  // a tie (s has a nontrivial stabiliser): keep the largest inverse index
+    if (NEED_H && eq) {  // a tie (s has a nontrivial stabiliser): keep the largest inverse index
       const int p_new = a + Lx * (Ly - 1 - (bit >> 3));
       const int p_cur = (win & 7) + Lx * (Ly - 1 - (win >> 3));
       if (invt[p_new] > invt[p_cur]) win = bit;
     }
     cnt = lt ? 1 : cnt + (eq ? 1 : 0);
-    win = lt ? bit : win;
+    if (NEED_H) win = lt ? bit : win;
     s = lt ? img : s;
   }
-  pos = (win & 7) + Lx * (Ly - 1 - (win >> 3));
+  pos = NEED_H ? (win & 7) + Lx * (Ly - 1 - (win >> 3)) : 0;
 }
 
 
@@ -456,7 +456,7 @@ __device__ __forceinline__ uint32_t torus_pair_min(const GroupDev& g, const Grou
 // p(b); s, h (largest inverse index among ties) and cnt = |Stab(s)|.  With
 // the row-pair table and several cosets, a first pass finds the cosets whose
 // images reach the smallest (top, second) row pair; only those are scanned.
-template <int LXC = 0, int LYC = 0, bool PAIRS = true>
+template <int LXC = 0, int LYC = 0, bool PAIRS = true, bool NEED_H = true>
 __device__ __forceinline__ void canonicalize_torus_cosets(const GroupDev& g, const GroupLds& L, uint64_t b,
                                                           uint64_t& s, int& h, int& cnt) {
   const int nT = (LXC ? LXC : g.Lx) * (LYC ? LYC : g.Ly);
@@ -476,49 +476,51 @@ __device__ __forceinline__ void canonicalize_torus_cosets(const GroupDev& g, con
       sel &= sel - 1;
       uint64_t s2;
       int pos2, c2;
-      canonicalize_torus_pairs_t<LXC, LYC>(g, L, L.coset_inv + p * nT, apply_coset(g, L, p, b), s2, pos2, c2);
-      const int h2 = L.coset_elem[p * nT + pos2];
+      canonicalize_torus_pairs_t<LXC, LYC, NEED_H>(g, L, L.coset_inv + p * nT, apply_coset(g, L, p, b), s2, pos2, c2);
+      const int h2 = NEED_H ? L.coset_elem[p * nT + pos2] : 0;
       if (s2 < s) {
         s = s2;
         h = h2;
         cnt = c2;
       } else if (s2 == s) {
         cnt += c2;
-        if (L.inv[h2] > L.inv[h]) h = h2;
+        if (NEED_H && L.inv[h2] > L.inv[h]) h = h2;
       }
     }
     return;
   }
   int pos;
   if constexpr (PAIRS)
-    canonicalize_torus_pairs_t<LXC, LYC>(g, L, L.coset_inv, b, s, pos, cnt);
+    canonicalize_torus_pairs_t<LXC, LYC, NEED_H>(g, L, L.coset_inv, b, s, pos, cnt);
   else
     canonicalize_torus_rows(g, L, L.coset_inv, b, s, pos, cnt);
-  h = L.coset_elem[pos];
+  h = NEED_H ? L.coset_elem[pos] : 0;
   for (int p = 1; p < g.n_cosets; ++p) {
     const uint64_t w = apply_coset(g, L, p, b);
     uint64_t s2;
     int pos2, c2;
     if constexpr (PAIRS)
-      canonicalize_torus_pairs_t<LXC, LYC>(g, L, L.coset_inv + p * nT, w, s2, pos2, c2);
+      canonicalize_torus_pairs_t<LXC, LYC, NEED_H>(g, L, L.coset_inv + p * nT, w, s2, pos2, c2);
     else
       canonicalize_torus_rows(g, L, L.coset_inv + p * nT, w, s2, pos2, c2);
-    const int h2 = L.coset_elem[p * nT + pos2];
+    const int h2 = NEED_H ? L.coset_elem[p * nT + pos2] : 0;
     if (s2 < s) {
       s = s2;
       h = h2;
       cnt = c2;
     } else if (s2 == s) {
       cnt += c2;
-      if (L.inv[h2] > L.inv[h]) h = h2;
+      if (NEED_H && L.inv[h2] > L.inv[h]) h = h2;
     }
   }
 }
 
-template <int LXC = 0, int LYC = 0>
+// NEED_H = false: h is not tracked (returned as 0), for callers that only use
+// s and cnt (every character 1).
+template <int LXC = 0, int LYC = 0, bool NEED_H = true>
 __device__ __forceinline__ void canonicalize_torus_pairs(const GroupDev& g, const GroupLds& L, uint64_t b, uint64_t& s,
                                                          int& h, int& cnt) {
-  canonicalize_torus_cosets<LXC, LYC, true>(g, L, b, s, h, cnt);
+  canonicalize_torus_cosets<LXC, LYC, true, NEED_H>(g, L, b, s, h, cnt);
 }
 
 // Canonical form s = min_h h(b), an element h with h(b) = s (among ties, the

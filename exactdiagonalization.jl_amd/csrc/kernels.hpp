@@ -94,6 +94,12 @@ struct OpDev {
   int n_off;             // number of off-diagonal terms (in the row or column sense)
   int n_diag;
   const OffTerm* off;    // row form: match = R, image = C ; column form: match = C, image = R
+  // The matvec's copy of off: every run of equal masks padded to an even
+  // length (padding terms carry the run's mask, a match outside it and no
+  // flip), so terms 2q and 2q + 1 share a mask and are matched together.
+  const OffTerm* mv_off;
+  int n_mv;              // padded length (0: the matvec matches off term by term)
+  int pad_mv_;
   const DiagTerm* diag;
   const double* diag_re; // per-row diagonal element (summed diagonal terms), or nullptr
   const double* diag_im; // its imaginary part, or nullptr when every diagonal amplitude is real

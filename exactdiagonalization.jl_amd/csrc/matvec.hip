@@ -20,6 +20,8 @@ constexpr int kBlock = 256;
 
 template <int MODE>
 struct Canon {
+  // NEED_H = false: h is left 0 where the canonicaliser can skip tracking it
+  template <bool NEED_H = true>
   __device__ __forceinline__ static void run(const GroupDev& g, const GroupLds& L, uint64_t b, uint64_t& s, int& h,
                                              int& cnt) {
     if constexpr (MODE == kGroupNone) {
@@ -27,15 +29,15 @@ struct Canon {
       h = 0;
       cnt = 1;
     } else if constexpr (MODE == kGroupTorusPairs) {
-      canonicalize_torus_pairs(g, L, b, s, h, cnt);
+      canonicalize_torus_pairs<0, 0, NEED_H>(g, L, b, s, h, cnt);
     } else if constexpr (MODE == kTorusPairs6x6) {
-      canonicalize_torus_pairs<6, 6>(g, L, b, s, h, cnt);
+      canonicalize_torus_pairs<6, 6, NEED_H>(g, L, b, s, h, cnt);
     } else if constexpr (MODE == kTorusPairs4x6) {
-      canonicalize_torus_pairs<4, 6>(g, L, b, s, h, cnt);
+      canonicalize_torus_pairs<4, 6, NEED_H>(g, L, b, s, h, cnt);
     } else if constexpr (MODE == kTorusPairs6x4) {
-      canonicalize_torus_pairs<6, 4>(g, L, b, s, h, cnt);
+      canonicalize_torus_pairs<6, 4, NEED_H>(g, L, b, s, h, cnt);
     } else if constexpr (MODE == kTorusPairs4x4) {
-      canonicalize_torus_pairs<4, 4>(g, L, b, s, h, cnt);
+      canonicalize_torus_pairs<4, 4, NEED_H>(g, L, b, s, h, cnt);
     } else {
       canonicalize(g, L, b, s, h, cnt);
     }
@@ -66,19 +68,32 @@ __device__ __forceinline__ Staged stage_op(const OpDev& op, unsigned char* lds) 
 // amplitude) pairs read per matched term, where flip = match ^ image gives the
 // image of a matching row b as b ^ flip; imaginary parts apart.
 struct MvTerms {
-  const uint64_t* mm;  // 2 per term: mask, match
+  const uint64_t* mm;  // term by term: mask, match; paired (n_mv): mask, match 2q, match 2q+1
   const uint64_t* fa;  // 2 per term: flip, bits of Re(amplitude)
   const double* ai;    // Im(amplitude)
 };
 
-__device__ __forceinline__ MvTerms stage_mv_terms(const OpDev& op, unsigned char* lds) {
-  uint64_t* mm = (uint64_t*)(lds + op.off_lds);  // same n_off * 40 bytes as the OffTerm list
-  uint64_t* fa = mm + 2 * op.n_off;
-  double* ai = (double*)(fa + 2 * op.n_off);
-  for (int t = threadIdx.x; t < op.n_off; t += blockDim.x) {
-    const OffTerm o = op.off[t];
-    mm[2 * t] = o.mask;
-    mm[2 * t + 1] = o.match;
+// paired: the padded copy (op.n_mv > 0) with one mask per pair of terms;
+// else the terms of op.off one by one, indexed as in op.off.
+__device__ __forceinline__ MvTerms stage_mv_terms(const OpDev& op, unsigned char* lds, bool paired) {
+  const int n = paired ? op.n_mv : op.n_off;
+  const OffTerm* src = paired ? op.mv_off : op.off;
+  uint64_t* mm = (uint64_t*)(lds + op.off_lds);  // at most 40 bytes per term, as reserved
+  uint64_t* fa = mm + (paired ? 3 * (n / 2) : 2 * n);
+  double* ai = (double*)(fa + 2 * n);
+  for (int t = threadIdx.x; t < n; t += blockDim.x) {
+    const OffTerm o = src[t];
+    if (paired) {
+      if ((t & 1) == 0) {
+        mm[3 * (t >> 1)] = o.mask;
+        mm[3 * (t >> 1) + 1] = o.match;
+      } else {
+        mm[3 * (t >> 1) + 2] = o.match;
+      }
+    } else {
+      mm[2 * t] = o.mask;
+      mm[2 * t + 1] = o.match;
+    }
     fa[2 * t] = o.match ^ o.image;
     fa[2 * t + 1] = (uint64_t)__double_as_longlong(o.are);
     ai[t] = o.aim;
@@ -87,7 +102,7 @@ __device__ __forceinline__ MvTerms stage_mv_terms(const OpDev& op, unsigned char
 }
 
 // Weight of an off-diagonal element: a * phase^(conj if TRANS) * sqrt(N_own) / sqrt(N_img)
-template <int MODE, bool TRANS>
+template <int MODE, bool TRANS, bool UNIT = false>
 __device__ __forceinline__ void element_weight_a(const OpDev& op, const GroupLds& L, double are, double aim, int h,
                                                  int cnt, double sqrt_own, double& wr, double& wi) {
   if constexpr (MODE == kGroupNone) {
@@ -95,7 +110,7 @@ __device__ __forceinline__ void element_weight_a(const OpDev& op, const GroupLds
     wi = aim;
   } else {
     const double sc = sqrt_own * L.sqcnt[cnt];
-    if (op.g.unit_phases) {
+    if (UNIT || op.g.unit_phases) {
       wr = are * sc;
       wi = aim * sc;
     } else {
@@ -146,12 +161,14 @@ __global__ __launch_bounds__(kBlock) void k_diag_values(OpDev op, double* __rest
 // images share cache lines of the index and of x).
 constexpr int kMvBlock = 512;
 
-template <typename V, int MODE, bool TRANS>
+// UNIT: every character is 1 (op.g.unit_phases): the weight needs no phase,
+// so the canonicaliser does not track the element reaching the image.
+template <typename V, int MODE, bool TRANS, bool UNIT>
 __global__ __launch_bounds__(kMvBlock) void k_matvec(OpDev op, const V* __restrict__ x, V* __restrict__ y, int accumulate) {
   extern __shared__ unsigned char lds[];
   GroupLds L{};
   if constexpr (MODE != kGroupNone) L = stage_group(op.g, op.lay, lds);
-  const MvTerms T = stage_mv_terms(op, lds);
+  const MvTerms T = stage_mv_terms(op, lds, op.n_mv != 0);
   lds_copy(lds + op.w_lds, op.ix.W, op.w_bytes);
   const uint64_t* W = (const uint64_t*)(lds + op.w_lds);
   __syncthreads();
@@ -174,12 +191,21 @@ __global__ __launch_bounds__(kMvBlock) void k_matvec(OpDev op, const V* __restri
   double sqrt_own = 1.0;
   if constexpr (MODE != kGroupNone) sqrt_own = sqrt((double)op.orbit[row]);
 
-  for (int seg = 0; seg < op.n_off; seg += 64) {
-    const int nseg = (op.n_off - seg) < 64 ? (op.n_off - seg) : 64;
+  const int n_terms = op.n_mv ? op.n_mv : op.n_off;
+  for (int seg = 0; seg < n_terms; seg += 64) {
+    const int nseg = (n_terms - seg) < 64 ? (n_terms - seg) : 64;
     uint64_t bits = 0;
-    for (int j = 0; j < nseg; ++j) {
-      const uint64_t* m = T.mm + 2 * (seg + j);
-      bits |= (uint64_t)((b & m[0]) == m[1]) << j;
+    if (op.n_mv) {  // two terms with one mask per test (nseg is even)
+      for (int j = 0; j < nseg; j += 2) {
+        const uint64_t* m = T.mm + 3 * ((seg + j) >> 1);
+        const uint64_t xm = b & m[0];
+        bits |= (uint64_t)((uint32_t)(xm == m[1]) | ((uint32_t)(xm == m[2]) << 1)) << j;
+      }
+    } else {
+      for (int j = 0; j < nseg; ++j) {
+        const uint64_t* m = T.mm + 2 * (seg + j);
+        bits |= (uint64_t)((b & m[0]) == m[1]) << j;
+      }
     }
     // two images per step: both canonical forms first, then both lookups and
     // both x gathers issued before either is used (twice the loads in flight;
@@ -204,7 +230,7 @@ __global__ __launch_bounds__(kMvBlock) void k_matvec(OpDev op, const V* __restri
 #pragma unroll
       for (int k = 0; k < NW; ++k) {
         fa[k] = *(const ulonglong2*)(T.fa + 2 * t[k]);
-        Canon<MODE>::run(op.g, L, b ^ fa[k].x, sk[k], hk[k], ck[k]);
+        Canon<MODE>::template run<!UNIT>(op.g, L, b ^ fa[k].x, sk[k], hk[k], ck[k]);
       }
       int64_t ik[NW];
 #pragma unroll
@@ -225,7 +251,7 @@ __global__ __launch_bounds__(kMvBlock) void k_matvec(OpDev op, const V* __restri
       for (int k = 0; k < NW; ++k) {
         if (ik[k] >= 0) {
           double wr, wi;
-          element_weight_a<MODE, TRANS>(op, L, __longlong_as_double((long long)fa[k].y),
+          element_weight_a<MODE, TRANS, UNIT>(op, L, __longlong_as_double((long long)fa[k].y),
                                         VT<V>::cx ? T.ai[t[k]] : 0.0, hk[k], ck[k], sqrt_own, wr, wi);
           vfma(acc, (R)wr, (R)wi, xv[k]);
         }
@@ -238,12 +264,17 @@ __global__ __launch_bounds__(kMvBlock) void k_matvec(OpDev op, const V* __restri
 template <typename V, int MODE>
 hipError_t launch_mv_t(const OpDev& op, int transpose, int accumulate, const void* x, void* y, hipStream_t st) {
   dim3 grid((unsigned)((op.dim + kMvBlock - 1) / kMvBlock));
-  if (transpose)
-    hipLaunchKernelGGL((k_matvec<V, MODE, true>), grid, dim3(kMvBlock), op.lds_total, st, op, (const V*)x, (V*)y,
-                       accumulate);
-  else
-    hipLaunchKernelGGL((k_matvec<V, MODE, false>), grid, dim3(kMvBlock), op.lds_total, st, op, (const V*)x, (V*)y,
-                       accumulate);
+  // unit-phase kernels only where the canonicaliser can drop the element (row-pair tori)
+  const bool unit = MODE >= kGroupTorusPairs && op.g.unit_phases;
+#define EDHIP_MV(T, U)                                                                                                 \
+  hipLaunchKernelGGL((k_matvec<V, MODE, T, U>), grid, dim3(kMvBlock), op.lds_total, st, op, (const V*)x, (V*)y,     \
+                     accumulate)
+  if (transpose) {
+    if (unit) EDHIP_MV(true, MODE >= kGroupTorusPairs); else EDHIP_MV(true, false);
+  } else {
+    if (unit) EDHIP_MV(false, MODE >= kGroupTorusPairs); else EDHIP_MV(false, false);
+  }
+#undef EDHIP_MV
   return hipGetLastError();
 }
 
@@ -493,7 +524,7 @@ __global__ __launch_bounds__(kBlock) void k_term_expect(OpDev op, const V* __res
   extern __shared__ unsigned char lds[];
   GroupLds L{};
   if constexpr (MODE != kGroupNone) L = stage_group(op.g, op.lay, lds);
-  const MvTerms T = stage_mv_terms(op, lds);
+  const MvTerms T = stage_mv_terms(op, lds, false);
   lds_copy(lds + op.w_lds, op.ix.W, op.w_bytes);
   const uint64_t* W = (const uint64_t*)(lds + op.w_lds);
   double* bins = (double*)(lds + op.lds_total);  // [wave][nw][NC]
