@@ -190,6 +190,10 @@ struct GroupHost {
   GroupLdsLayout lay{};
 };
 
+// Generic groups whose 4-bit permutation tables exceed this stay in global
+// memory (read through L1/L2) instead of LDS.
+constexpr size_t kNibLdsMax = 40 * 1024;
+
 struct edhip_space {
   edhip_ctx* ctx = nullptr;
   int n_bits = 0;
@@ -691,8 +695,12 @@ void build_group(GroupHost& gh, int nb, double tol, hipStream_t st) {
         }
     upload(gh.nib, tab, st);
     d.nib = gh.nib.as<uint64_t>();
-    L.bytes_nib = (uint32_t)(tab.size() * 8);
-    place(L.off_nib, L.bytes_nib);
+    if (tab.size() * 8 <= kNibLdsMax) {
+      L.bytes_nib = (uint32_t)(tab.size() * 8);
+      place(L.off_nib, L.bytes_nib);
+    } else {
+      d.nib_global = 1;  // large groups read their tables through the caches
+    }
   }
   // the row-pair table goes last, so that an operator whose terms leave no
   // room for it can drop it (make_opdev) without moving the other tables

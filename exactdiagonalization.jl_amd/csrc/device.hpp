@@ -238,7 +238,7 @@ struct GroupDev {
   int unit_phases;      // 1 if every phase is exactly 1 (all characters 1)
   int has_xor;          // some element XORs a mask into the permuted word (global spin flip)
   int n_cosets;         // torus: G / (Lx Ly) cosets T p of the translations (1: translations only)
-  int pad4_;
+  int nib_global;       // generic: the permutation tables are read from nib (global), not LDS
   uint64_t row_mask;    // torus: (1 << Lx) - 1
   uint64_t full_mask;   // (1 << nb) - 1
   const uint64_t* nib;  // generic: G * n_nib * 16 images of 4-bit chunks
@@ -526,6 +526,17 @@ __device__ __forceinline__ void canonicalize_torus_pairs(const GroupDev& g, cons
 // Canonical form s = min_h h(b), an element h with h(b) = s (among ties, the
 // one whose inverse has the largest index: the reference's last writer of the
 // orbit dictionary), and cnt = #{h : h(b) = s} = |Stab(s)|.
+// Calls f(tables) with the generic group's 4-bit permutation tables: the LDS
+// copy, or the global array for groups too large for LDS (two separate
+// inlined calls, so that each reads through its own kind of load).
+template <typename F>
+__device__ __forceinline__ void with_nib_tables(const GroupDev& g, const GroupLds& L, F&& f) {
+  if (g.nib_global)
+    f(g.nib);
+  else
+    f(L.nib);
+}
+
 __device__ __forceinline__ void canonicalize(const GroupDev& g, const GroupLds& L, uint64_t b, uint64_t& s, int& h,
                                              int& cnt) {
   if (g.mode == kGroupGeneric) {
@@ -533,18 +544,20 @@ __device__ __forceinline__ void canonicalize(const GroupDev& g, const GroupLds& 
     h = 0;
     cnt = 1;
     const int stride = g.n_nib << 4;
-    for (int e = 1; e < g.G; ++e) {
-      uint64_t img = perm_apply_nib(L.nib + e * stride, g.n_nib, b);
-      if (g.has_xor) img ^= L.xr[e];
-      if (img < s) {
-        s = img;
-        h = e;
-        cnt = 1;
-      } else if (img == s) {
-        ++cnt;
-        if (L.inv[e] > L.inv[h]) h = e;
+    with_nib_tables(g, L, [&](const uint64_t* nib) {
+      for (int e = 1; e < g.G; ++e) {
+        uint64_t img = perm_apply_nib(nib + e * stride, g.n_nib, b);
+        if (g.has_xor) img ^= L.xr[e];
+        if (img < s) {
+          s = img;
+          h = e;
+          cnt = 1;
+        } else if (img == s) {
+          ++cnt;
+          if (L.inv[e] > L.inv[h]) h = e;
+        }
       }
-    }
+    });
     return;
   }
   if (g.mode == kGroupTorus) {
@@ -602,16 +615,20 @@ __device__ __forceinline__ bool torus_scan(const GroupDev& g, const GroupLds& L,
 __device__ __forceinline__ bool is_representative(const GroupDev& g, const GroupLds& L, uint64_t b, int& orbit) {
   if (g.mode == kGroupGeneric) {
     int stab = 1;
+    bool ok = true;
     const int stride = g.n_nib << 4;
-    for (int e = 1; e < g.G; ++e) {
-      uint64_t img = perm_apply_nib(L.nib + e * stride, g.n_nib, b);
-      if (g.has_xor) img ^= L.xr[e];
-      if (img < b) return false;
-      if (img == b) {
-        if (!L.trivial[e]) return false;
-        ++stab;
+    with_nib_tables(g, L, [&](const uint64_t* nib) {
+      for (int e = 1; e < g.G; ++e) {
+        uint64_t img = perm_apply_nib(nib + e * stride, g.n_nib, b);
+        if (g.has_xor) img ^= L.xr[e];
+        if (img < b || (img == b && !L.trivial[e])) {
+          ok = false;
+          return;
+        }
+        if (img == b) ++stab;
       }
-    }
+    });
+    if (!ok) return false;
     orbit = g.G / stab;
     return true;
   }

@@ -249,7 +249,9 @@ __global__ __launch_bounds__(kBlock) void k_vec_reduce(const uint64_t* __restric
     }
   } else {
     const int stride = g.n_nib << 4;
-    for (int e = 0; e < g.G; ++e) add(e, perm_apply_nib(L.nib + e * stride, g.n_nib, s) ^ (g.has_xor ? L.xr[e] : 0ull));
+    with_nib_tables(g, L, [&](const uint64_t* nib) {
+      for (int e = 0; e < g.G; ++e) add(e, perm_apply_nib(nib + e * stride, g.n_nib, s) ^ (g.has_xor ? L.xr[e] : 0ull));
+    });
   }
   small[r] = acc;
 }
