@@ -369,10 +369,26 @@ __device__ __forceinline__ void canonicalize_torus_pairs_t(const GroupDev& g, co
   s = ~0ull;
   cnt = 0;
   int win = 0;  // bit 8y + a of the candidate giving s
+  const uint32_t rmask = (1u << Lx) - 1u;
+  bool first = true;
   while (cand) {
     const int bit = __builtin_ctzll(cand);
     cand &= cand - 1;
     const int a = bit & 7;
+    // Every candidate shares the top two rows of its image; after the first,
+    // one whose third row (row y - 2, rotated by a) exceeds that of s cannot
+    // reach s and is skipped before its full image is formed (most lanes
+    // have a single candidate: the loop runs as long as the lane with the
+    // most candidates, and the skipped ones cost a few operations).
+    bool full = true;
+    if (Ly >= 3 && !first) {
+      const int y3 = ((bit >> 3) + Ly - 2) % Ly;
+      const uint32_t r3 = (uint32_t)(b >> (y3 * Lx)) & rmask;
+      const uint32_t t3 = ((r3 << a) | (r3 >> (Lx - a))) & rmask;
+      full = t3 <= ((uint32_t)(s >> ((Ly - 3) * Lx)) & rmask);
+    }
+    first = false;
+    if (!full) continue;
     const int sh = (Ly - 1 - (bit >> 3)) * Lx;  // row y moves to the top
     const uint64_t t = ((b << sh) | (b >> (NB - sh))) & FULL;
     const uint64_t img = ((t << a) & L.rot_hi[a]) | ((t >> (Lx - a)) & L.rot_lo[a]);
