@@ -2168,7 +2168,11 @@ int edhip_lanczos(edhip_op* op, const edhip_lanczos_opts* opts, double* evals, v
     // below single-precision rounding); the FP64 phase uses the caller's rule.
     L.criterion = mixed ? 1 : opts->criterion;
     const double tol_main = mixed ? std::max(opts->tol, 1e-8) : opts->tol;
-    bool conv = L.run(nev, tol_main, opts->max_restarts, opts->check_every, theta, res, ritz, prec64, nev + 1);
+    // Large bases check convergence at every step: the host eigensolve of the
+    // small projected matrix costs far less than the product it may save.
+    const bool large = n > (int64_t)1 << 22;
+    const int check_main = large ? 1 : opts->check_every;
+    bool conv = L.run(nev, tol_main, opts->max_restarts, check_main, theta, res, ritz, prec64, nev + 1);
     st.iterations = L.iterations;
     st.restarts = L.restarts;
     st.matvecs = L.matvecs;

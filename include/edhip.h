@@ -252,7 +252,8 @@ typedef struct {
   int want_vectors;  /* 1: write the eigenvectors (nev x dim, FP64/C128) */
   int refine_basis;  /* mixed mode: Krylov size of one FP64 refinement cycle */
   int max_refine;    /* mixed mode: refinement cycles per eigenvalue */
-  int check_every;   /* Ritz-value convergence check period (iterations) */
+  int check_every;   /* Ritz-value convergence check period (iterations);
+                        bases over 2^22 elements check every iteration */
   int criterion;     /* 0: residual ||H y - theta y|| <= tol*max(1,|theta|);
                         1: also accept r^2/gap <= tol*max(1,|theta|) with theta
                         stationary between checks (eigenvalue accuracy) */
