@@ -190,6 +190,12 @@ struct GroupHost {
   GroupLdsLayout lay{};
 };
 
+// Top bits of the words coded in the spread of x (a table of 2^12 u16 codes,
+// 8 KiB, staged in LDS by the matvec), and the largest spread, in elements
+// per basis word, the matvec builds (6x6 k=(0,0): 18.2).
+constexpr int kXTopBits = 12;
+constexpr uint64_t kXMapRatio = 32;
+
 // Generic groups whose 4-bit permutation tables exceed this stay in global
 // memory (read through L1/L2) instead of LDS.
 constexpr size_t kNibLdsMax = 40 * 1024;
@@ -281,6 +287,76 @@ struct edhip_space {
   mutable DevBuf bitmap;
   mutable int bm_bits = 0;
   mutable bool bm_tried = false;
+
+  // The input vector spread over word positions (XMapDev), one array per
+  // element size (4, 8, 16 bytes), built on the first product that can use
+  // it; positions without a basis word stay 0.
+  struct XMapHost {
+    DevBuf values;
+    XMapDev dev{};
+    bool tried = false, ok = false;
+  };
+  mutable XMapHost xmap[3];
+  mutable DevBuf xtop;  // code of each value of the top bits of the words
+  mutable int xtop_bits = -1, xtop_shift = 0, xtop_K = 0;
+  mutable uint64_t xtop_codes = 0;
+
+  void release_xmaps() const {
+    for (XMapHost& X : xmap) {
+      X.values.release();
+      X.tried = X.ok = false;
+    }
+  }
+
+  const XMapHost* ensure_xmap(size_t es) const {
+    const int slot = es == 4 ? 0 : es == 8 ? 1 : es == 16 ? 2 : -1;
+    if (slot < 0 || (is_sector && !group)) return nullptr;  // full sectors: rank arithmetic
+    XMapHost& X = xmap[slot];
+    if (X.tried) return X.ok ? &X : nullptr;
+    X.tried = true;
+    build_index();
+    if (dim == 0) return nullptr;
+    try {
+      if (xtop_bits < 0) {
+        uint64_t max_key = 0;
+        HIP_OR_THROW(hipMemcpyAsync(&max_key, keys.as<uint64_t>() + (dim - 1), 8, hipMemcpyDeviceToHost, ctx->stream));
+        ctx->sync();
+        const int K = std::max(4, bit_length(max_key));
+        const int t = std::min(kXTopBits, K);
+        const int sh = K - t;
+        DevBuf flags;
+        flags.alloc(((size_t)1 << t) * 4);
+        HIP_OR_THROW(hipMemsetAsync(flags.p, 0, ((size_t)1 << t) * 4, ctx->stream));
+        HIP_OR_THROW(launch_mark_tops(keys.as<uint64_t>(), dim, sh, flags.as<uint32_t>(), ctx->stream));
+        std::vector<uint32_t> fl((size_t)1 << t);
+        HIP_OR_THROW(hipMemcpyAsync(fl.data(), flags.p, fl.size() * 4, hipMemcpyDeviceToHost, ctx->stream));
+        ctx->sync();
+        std::vector<uint16_t> codes(fl.size(), 0xffff);
+        uint64_t nc = 0;
+        for (size_t v = 0; v < fl.size(); ++v)
+          if (fl[v]) codes[v] = (uint16_t)nc++;
+        upload(xtop, codes, ctx->stream);
+        xtop_bits = t;
+        xtop_shift = sh;
+        xtop_K = K;
+        xtop_codes = nc;
+      }
+      const uint64_t n_pos = xtop_codes << xtop_shift;
+      // the spread may take up to kXMapRatio times the memory of the vector
+      if (n_pos > (uint64_t)kXMapRatio * (uint64_t)dim && n_pos > (1ull << 20)) return nullptr;
+      X.values.alloc(n_pos * es);
+      HIP_OR_THROW(hipMemsetAsync(X.values.p, 0, n_pos * es, ctx->stream));
+      ctx->sync();
+      X.dev = XMapDev{X.values.p, xtop.as<uint16_t>(), xtop_K, xtop_bits, xtop_shift, 0};
+      X.ok = true;
+      return &X;
+    } catch (const Error& e) {
+      if (e.code != EDHIP_ERR_MEMORY) throw;
+      (void)hipGetLastError();  // no room: the products use the rank index
+      X.values.release();
+      return nullptr;
+    }
+  }
 
   bool ensure_bitmap() const {
     if (bm_tried) return bm_bits > 0;
@@ -823,6 +899,9 @@ OpDev make_opdev(edhip_op* op, bool transpose) {
   d.w_bytes = d.ix.rank_mode ? (uint32_t)((sp->n_bits + 1) * (sp->n_bits + 1) * 8) : 0u;
   off += d.w_bytes;
   d.lds_total = align_up(off, 16);
+  // the matvec alone stages the code table of the spread x after the rest
+  d.xm = XMapDev{};
+  d.codes_lds = d.lds_total + (2u << kXTopBits) <= 64 * 1024 ? d.lds_total : 0;
   if (d.lds_total > 64 * 1024)
     fail(EDHIP_ERR_ARGUMENT, "operator and symmetry tables exceed the 64 KiB LDS budget of the matvec kernels (" +
                                  std::to_string(d.lds_total) + " bytes)");
@@ -982,13 +1061,26 @@ void op_matvec(edhip_op* op, int prec, int transpose, int accumulate, const void
   const int idx_lookup = (op->strat_lookup == 2) ? 0 : op->strat_lookup;  // the hash applies to the pair kernel only
   if (!fast) {
     OpDev d = transpose ? op->dev_col : op->dev_row;
-    d.ix = sp->index_dev(idx_lookup);
+    d.ix = sp->index_dev(idx_lookup == 4 ? 0 : idx_lookup);
     if (ensure_diag(op)) {
       d.diag_re = op->diag_re.as<double>();
       d.diag_im = op->diag_im.as<double>();
     }
     op->used_kernel = 1;
     op->used_lookup = rank_mode ? 1 : (d.ix.bm_bits ? 3 : 1);
+    // lookup 4, and auto for 4-byte elements: x spread over the word positions
+    // when it fits (6x6 J1-J2: FP32 118 -> 97 ms; FP64 gains nothing, its
+    // spread being twice as large, so auto keeps the rank index there)
+    const bool want_spread = idx_lookup == 4 || (idx_lookup == 0 && elem_size(prec) == 4);
+    if (!rank_mode && want_spread && d.codes_lds) {
+      const edhip_space::XMapHost* X = sp->ensure_xmap(elem_size(prec));
+      if (X) {
+        HIP_OR_THROW(launch_xmap_fill(prec, X->dev, sp->keys.as<uint64_t>(), sp->dim, x, st));
+        d.xm = X->dev;
+        op->used_lookup = 4;
+      }
+    }
+    if (idx_lookup == 4 && !d.xm.values) fail(EDHIP_ERR_ARGUMENT, "x cannot be spread over the words of this space");
     HIP_OR_THROW(launch_matvec(d, prec, transpose ? 1 : 0, accumulate ? 1 : 0, x, y, st));
     return;
   }
@@ -1005,7 +1097,7 @@ void op_matvec(edhip_op* op, int prec, int transpose, int accumulate, const void
     HIP_OR_THROW(launch_matvec_pairs(F, prec, transpose, accumulate, x, y, st));
   } else {
     FastOpDev F = make_fast(op, transpose, nullptr);
-    F.base.ix = sp->index_dev(idx_lookup);
+    F.base.ix = sp->index_dev(idx_lookup == 4 ? 0 : idx_lookup);
     op->used_lookup = rank_mode ? 1 : (F.base.ix.bm_bits ? 3 : 1);
     HIP_OR_THROW(launch_matvec_pairs(F, prec, transpose, accumulate, x, y, st));
   }
@@ -1159,6 +1251,20 @@ inline int keep_count(int mm, int n_ret) {
 }
 
 inline size_t align256(size_t b) { return (b + 255) / 256 * 256; }
+
+// The context's workspace; when it does not fit, the spreads of x that the
+// operator's space keeps (caches the next product rebuilds if memory allows)
+// are freed and the allocation is tried once more.
+char* krylov_arena_for(edhip_op* op, size_t bytes) {
+  try {
+    return op->ctx->krylov_arena(bytes);
+  } catch (const Error& e) {
+    if (e.code != EDHIP_ERR_MEMORY) throw;
+    (void)hipGetLastError();
+    op->space->release_xmaps();
+    return op->ctx->krylov_arena(bytes);
+  }
+}
 
 struct TRLanczos {
   edhip_op* op;
@@ -1344,9 +1450,18 @@ struct TRLanczos {
       coef_d.ensure(coef.size() * sizeof(double));
       HIP_OR_THROW(hipMemcpyAsync(coef_d.p, coef.data(), coef.size() * sizeof(double), hipMemcpyHostToDevice,
                                   ctx->stream));
-      if ((size_t)keep * ld * es() > tmp_bytes) fail(EDHIP_ERR_ARGUMENT, "restart scratch too small");
-      HIP_OR_THROW(vec_combine(prec, prec, n, mm, Vp, ld, keep, coef_d.as<double>(), tmpp, ld, ctx->stream));
-      HIP_OR_THROW(hipMemcpyAsync(Vp, tmpp, (size_t)keep * ld * es(), hipMemcpyDeviceToDevice, ctx->stream));
+      // V[:, :keep] = V Z[:, :keep] in place, a chunk of rows at a time through
+      // the scratch (each row of the result needs only that row of V)
+      int64_t chunk = (int64_t)std::min<size_t>((size_t)n, tmp_bytes / ((size_t)keep * es()));
+      if (chunk < n) chunk &= ~(int64_t)3;  // chunks start on 16-byte boundaries
+      if (chunk < 1) fail(EDHIP_ERR_ARGUMENT, "restart scratch too small");
+      for (int64_t r0 = 0; r0 < n; r0 += chunk) {
+        const int64_t len = std::min<int64_t>(chunk, n - r0);
+        char* vrow = (char*)Vp + (size_t)r0 * es();
+        HIP_OR_THROW(vec_combine(prec, prec, len, mm, vrow, ld, keep, coef_d.as<double>(), tmpp, len, ctx->stream));
+        HIP_OR_THROW(hipMemcpy2DAsync(vrow, (size_t)ld * es(), tmpp, (size_t)len * es(), (size_t)len * es(), keep,
+                                      hipMemcpyDeviceToDevice, ctx->stream));
+      }
       // residual direction: w = vec(mm) (unnormalised), beta_last its norm
       scale(vec(mm), 1.0 / beta_last);
       HIP_OR_THROW(hipMemcpyAsync(vec(keep), vec(mm), (size_t)n * es(), hipMemcpyDeviceToDevice, ctx->stream));
@@ -1949,7 +2064,7 @@ int edhip_apply_host(edhip_op* op, int prec, int transpose, int accumulate, int6
 }
 
 int edhip_op_set_strategy(edhip_op* op, int kernel, int lookup) {
-  if (!op || kernel < 0 || kernel > 2 || lookup < 0 || lookup > 3) return EDHIP_ERR_ARGUMENT;
+  if (!op || kernel < 0 || kernel > 2 || lookup < 0 || lookup > 4) return EDHIP_ERR_ARGUMENT;
   return guarded(op->ctx, [&] {
     if (kernel == 2 && !fast_path_usable(op))
       fail(EDHIP_ERR_ARGUMENT, "the pair-group kernel does not cover this operator or symmetry group");
@@ -2152,10 +2267,12 @@ int edhip_lanczos(edhip_op* op, const edhip_lanczos_opts* opts, double* evals, v
     const int m_ref = (int)std::min<int64_t>(std::max(opts->refine_basis, 2), n);
     const size_t ritz_b = align256((size_t)nev * n * es64);
     const size_t main_v = align256((size_t)(L.m + 1) * L.ld * es_s);
-    const size_t main_t = align256((size_t)keep_count(L.m, nev + 1) * L.ld * es_s);
+    // restart scratch: the kept vectors for a chunk of at most 2^24 rows
+    const int64_t rows_t = std::min<int64_t>(L.ld, (int64_t)1 << 24);
+    const size_t main_t = align256((size_t)keep_count(L.m, nev + 1) * rows_t * es_s);
     const size_t ref_v = mixed ? align256((size_t)(m_ref + 1) * L.ld * es64) : 0;
-    const size_t ref_t = mixed ? align256((size_t)keep_count(m_ref, 2) * L.ld * es64) : 0;
-    char* arena = ctx->krylov_arena(ritz_b + std::max(main_v + main_t, ref_v + ref_t));
+    const size_t ref_t = mixed ? align256((size_t)keep_count(m_ref, 2) * rows_t * es64) : 0;
+    char* arena = krylov_arena_for(op, ritz_b + std::max(main_v + main_t, ref_v + ref_t));
     void* ritz = arena;
     L.Vp = arena + ritz_b;
     L.tmpp = arena + ritz_b + main_v;
@@ -2277,7 +2394,7 @@ int edhip_minres(edhip_op* op, int prec, double shift, int n_deflate, const void
     H.prec = prec;
     H.set_size(n);
     const size_t vb = align256(bytes);
-    char* buf = ctx->krylov_arena(vb * 7);
+    char* buf = krylov_arena_for(op, vb * 7);
     auto V = [&](int i) { return (void*)(buf + (size_t)i * vb); };
     void *r1 = V(0), *r2 = V(1), *y = V(2), *v = V(3), *w = V(4), *w1 = V(5), *w2 = V(6);
     auto project = [&](void* z) {

@@ -785,6 +785,30 @@ __device__ __forceinline__ uint32_t hash_word(uint64_t s) {
   return h;
 }
 
+// The input vector spread over word positions (the matvec's lookup): a
+// word's position is its bits below top_shift under the code of its top bits
+// (topcode: the rank of the top value among those of the basis words, 0xffff
+// for none; the codes keep the order).  values[position] holds the element
+// of x of the basis word there, 0 at every other position, so that a
+// canonical image finds its x element with one load and no rank; an image
+// outside the basis adds w * 0.  Refreshed before every product.
+struct XMapDev {
+  void* values;             // null: not used
+  const uint16_t* topcode;  // 2^top_bits codes (global memory; the matvec copies it to LDS)
+  int bm_bits;              // words must be below 2^bm_bits
+  int top_bits, top_shift;
+  int pad_;
+};
+
+template <typename V>
+__device__ __forceinline__ bool xmap_fetch(const XMapDev& X, const uint16_t* codes, uint64_t key, V& xv) {
+  if ((key >> X.bm_bits) != 0) return false;
+  const uint32_t c = codes[key >> X.top_shift];
+  if (c == 0xffffu) return false;
+  xv = ((const V*)X.values)[((uint64_t)c << X.top_shift) | (key & ((1ull << X.top_shift) - 1ull))];
+  return true;
+}
+
 struct HashDev {
   unsigned char* slots;  // n_slots * slot_bytes; key at offset 0, vector element at offset xoff
   uint32_t mask;         // n_slots - 1 (power of two)

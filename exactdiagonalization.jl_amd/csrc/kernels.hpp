@@ -22,6 +22,11 @@ constexpr size_t kQnStageBytes = 48 * 1024;
 // words[i] = i-th word of a multi-QN sector in ascending order, i < count
 hipError_t launch_enumerate_qn(uint64_t count, int n_sites, const QnSite* d_sites, int n_entries, const uint64_t* d_cnt,
                                const int32_t* d_nxt, uint64_t* d_out, hipStream_t st);
+// flags[key >> shift] = 1 for every key
+hipError_t launch_mark_tops(const uint64_t* d_keys, int64_t dim, int shift, uint32_t* d_flags, hipStream_t st);
+// x spread over the word positions of X (before each product).
+hipError_t launch_xmap_fill(int prec, const XMapDev& X, const uint64_t* d_keys, int64_t dim, const void* x,
+                            hipStream_t st);
 // prefix[j] = lower_bound(keys, j << shift), j <= n_buckets
 hipError_t launch_build_prefix(const uint64_t* d_keys, int64_t dim, int shift, uint64_t n_buckets, uint32_t* d_prefix,
                                hipStream_t st);
@@ -106,7 +111,9 @@ struct OpDev {
   uint32_t off_lds;      // LDS offset of the staged off-diagonal terms
   uint32_t w_lds;        // LDS offset of the staged rank table (rank mode)
   uint32_t w_bytes;
-  uint32_t lds_total;
+  XMapDev xm;            // matvec: x read from its spread over word positions (xm.values != null)
+  uint32_t codes_lds;    // LDS offset of xm's code table, after lds_total (0: no room)
+  uint32_t lds_total;    // LDS of the kernels (the matvec adds xm's code table)
 };
 
 enum Prec : int { kF64 = 0, kC128 = 1, kF32 = 2, kC64 = 3 };

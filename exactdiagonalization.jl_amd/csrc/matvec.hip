@@ -171,6 +171,9 @@ __global__ __launch_bounds__(kMvBlock) void k_matvec(OpDev op, const V* __restri
   const MvTerms T = stage_mv_terms(op, lds, op.n_mv != 0);
   lds_copy(lds + op.w_lds, op.ix.W, op.w_bytes);
   const uint64_t* W = (const uint64_t*)(lds + op.w_lds);
+  // code table of the spread x (the host sets xm only with room for it)
+  uint16_t* codes = (uint16_t*)(lds + op.codes_lds);
+  if (op.xm.values) lds_copy((unsigned char*)codes, op.xm.topcode, 2u << op.xm.top_bits);
   __syncthreads();
   const int64_t row = (int64_t)blockIdx.x * kMvBlock + threadIdx.x;
   if (row >= op.dim) return;
@@ -233,19 +236,19 @@ __global__ __launch_bounds__(kMvBlock) void k_matvec(OpDev op, const V* __restri
         Canon<MODE>::template run<!UNIT>(op.g, L, b ^ fa[k].x, sk[k], hk[k], ck[k]);
       }
       int64_t ik[NW];
-#pragma unroll
-      for (int k = 0; k < NW; k += 2) {
-        if (k + 1 < NW) {
-          index_of2(op.ix, W, sk[k], sk[k + 1], ik[k], ik[k + 1]);
-        } else {
-          ik[k] = index_of(op.ix, W, sk[k]);
-        }
-      }
       V xv[NW];
+      if (op.xm.values) {  // x of each image with one load at its word position
+        const bool f0 = xmap_fetch(op.xm, codes, sk[0], xv[0]);
+        const bool f1 = xmap_fetch(op.xm, codes, sk[1], xv[1]);
+        ik[0] = f0 ? 0 : -1;
+        ik[1] = (ok[1] && f1) ? 0 : -1;
+      } else {
+        index_of2(op.ix, W, sk[0], sk[1], ik[0], ik[1]);
 #pragma unroll
-      for (int k = 0; k < NW; ++k) {
-        ik[k] = ok[k] ? ik[k] : -1;
-        xv[k] = x[ik[k] >= 0 ? ik[k] : 0];
+        for (int k = 0; k < NW; ++k) {
+          ik[k] = ok[k] ? ik[k] : -1;
+          xv[k] = x[ik[k] >= 0 ? ik[k] : 0];
+        }
       }
 #pragma unroll
       for (int k = 0; k < NW; ++k) {
@@ -266,9 +269,9 @@ hipError_t launch_mv_t(const OpDev& op, int transpose, int accumulate, const voi
   dim3 grid((unsigned)((op.dim + kMvBlock - 1) / kMvBlock));
   // unit-phase kernels only where the canonicaliser can drop the element (row-pair tori)
   const bool unit = MODE >= kGroupTorusPairs && op.g.unit_phases;
+  const uint32_t lds = op.xm.values ? op.codes_lds + (2u << op.xm.top_bits) : op.lds_total;
 #define EDHIP_MV(T, U)                                                                                                 \
-  hipLaunchKernelGGL((k_matvec<V, MODE, T, U>), grid, dim3(kMvBlock), op.lds_total, st, op, (const V*)x, (V*)y,     \
-                     accumulate)
+  hipLaunchKernelGGL((k_matvec<V, MODE, T, U>), grid, dim3(kMvBlock), lds, st, op, (const V*)x, (V*)y, accumulate)
   if (transpose) {
     if (unit) EDHIP_MV(true, MODE >= kGroupTorusPairs); else EDHIP_MV(true, false);
   } else {

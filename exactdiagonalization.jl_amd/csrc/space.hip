@@ -303,6 +303,25 @@ __global__ __launch_bounds__(kBlock) void k_bitmap_set_base(uint64_t* __restrict
   blocks[j * 8 + 6] = base[j];
 }
 
+// flags[key >> shift] = 1 for the top value of every key (idempotent stores)
+__global__ __launch_bounds__(kBlock) void k_mark_tops(const uint64_t* __restrict__ keys, int64_t dim, int shift,
+                                                      uint32_t* __restrict__ flags) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= dim) return;
+  flags[keys[i] >> shift] = 1u;
+}
+
+// Before a product: x spread over the word positions (XMapDev).
+template <typename V>
+__global__ __launch_bounds__(kBlock) void k_xmap_fill(XMapDev X, const uint64_t* __restrict__ keys, int64_t dim,
+                                                      const V* __restrict__ x) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= dim) return;
+  const uint64_t k = keys[i];
+  const uint64_t pos = ((uint64_t)X.topcode[k >> X.top_shift] << X.top_shift) | (k & ((1ull << X.top_shift) - 1ull));
+  ((V*)X.values)[pos] = x[i];
+}
+
 __global__ void k_u32_to_u64(const uint32_t* __restrict__ in, uint64_t* __restrict__ out, uint64_t n) {
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) out[i] = in[i];
@@ -395,6 +414,35 @@ hipError_t launch_bitmap_build(const uint64_t* d_keys, int64_t dim, uint64_t* d_
   e = hipcub::DeviceScan::ExclusiveSum(d_tmp, tmp_bytes, d_totals, d_base, (int64_t)n_blocks, st);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_bitmap_set_base, dim3(grid_for(n_blocks)), dim3(kBlock), 0, st, d_blocks, n_blocks, d_base);
+  return hipGetLastError();
+}
+
+hipError_t launch_mark_tops(const uint64_t* d_keys, int64_t dim, int shift, uint32_t* d_flags, hipStream_t st) {
+  if (dim == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_mark_tops, dim3(grid_for((uint64_t)dim)), dim3(kBlock), 0, st, d_keys, dim, shift, d_flags);
+  return hipGetLastError();
+}
+
+hipError_t launch_xmap_fill(int prec, const XMapDev& X, const uint64_t* d_keys, int64_t dim, const void* x,
+                            hipStream_t st) {
+  if (dim == 0) return hipSuccess;
+  dim3 grid(grid_for((uint64_t)dim));
+  switch (prec) {
+    case kF64:
+      hipLaunchKernelGGL(k_xmap_fill<double>, grid, dim3(kBlock), 0, st, X, d_keys, dim, (const double*)x);
+      break;
+    case kC128:
+      hipLaunchKernelGGL(k_xmap_fill<Cx<double>>, grid, dim3(kBlock), 0, st, X, d_keys, dim, (const Cx<double>*)x);
+      break;
+    case kF32:
+      hipLaunchKernelGGL(k_xmap_fill<float>, grid, dim3(kBlock), 0, st, X, d_keys, dim, (const float*)x);
+      break;
+    case kC64:
+      hipLaunchKernelGGL(k_xmap_fill<Cx<float>>, grid, dim3(kBlock), 0, st, X, d_keys, dim, (const Cx<float>*)x);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 

@@ -191,11 +191,17 @@ int edhip_apply_host(edhip_op* op, int prec, int transpose, int accumulate, int6
  *   kernel: 0 auto (term list), 1 term-list kernel, 2 bond-pair kernel
  *           (operators whose terms act on one or two bits, plain or
  *           torus-translation spaces)
- *   lookup: 0 auto (bitmap when available, else sorted search),
+ *   lookup: 0 auto (term-list kernel: FP32 / 4-byte elements x spread
+ *             over word positions when it fits; else bitmap, else sorted
+ *             search),
  *           1 sorted search behind the prefix table,
  *           2 open-addressing table of the basis with the input vector
  *             embedded (bond-pair kernel only; built on first use),
- *           3 bitmap rank index (words of at most 36 bits)
+ *           3 bitmap rank index (words of at most 36 bits),
+ *           4 x spread over the positions of the words (term-list kernel,
+ *             reduced spaces): one load per image, no rank; the spread (at
+ *             most 32 elements per basis word) is allocated per element size
+ *             on first use and refreshed before every product
  *   a full popcount sector always uses rank arithmetic.
  * edhip_op_last_strategy reports what the last product used. */
 int edhip_op_set_strategy(edhip_op* op, int kernel, int lookup);

@@ -31,7 +31,8 @@ def _torus_system(ed, oracle, Lx, Ly, k, j2=0.5, field=0.0):
     return ed.represent(rh, H), oracle.Opr(red, ed.terms_arrays(H))
 
 
-STRATS = [("terms", "search"), ("terms", "bitmap"), ("pairs", "search"), ("pairs", "bitmap"), ("pairs", "hash")]
+STRATS = [("terms", "search"), ("terms", "bitmap"), ("terms", "xspread"), ("pairs", "search"), ("pairs", "bitmap"),
+          ("pairs", "hash")]
 
 
 @pytest.mark.parametrize("Lx,Ly,k", [(4, 4, (2, 0)), (4, 4, (1, 2)), (4, 6, (2, 0)), (4, 6, (1, 3)), (6, 4, (3, 0)),
@@ -134,13 +135,22 @@ def test_torus6x6_properties(ed):
     vH = torch.empty_like(v)
     ed.mul_(vH, v, opr)
     assert (torch.linalg.norm(vH - Hv) / torch.linalg.norm(Hv)).item() < 1e-13
-    opr.set_strategy("pairs", "bitmap")
-    Hv2 = torch.empty_like(v)
-    ed.mul_(Hv2, opr, v)
+    assert opr.last_strategy() == ("terms", "bitmap")  # FP64: auto keeps the rank index
+    for strat in (("pairs", "bitmap"), ("terms", "xspread")):
+        opr.set_strategy(*strat)
+        Hv2 = torch.empty_like(v)
+        ed.mul_(Hv2, opr, v)
+        assert opr.last_strategy() == strat
+        assert (torch.linalg.norm(Hv2 - Hv) / torch.linalg.norm(Hv)).item() < 1e-13
     opr.set_strategy()
-    assert (torch.linalg.norm(Hv2 - Hv) / torch.linalg.norm(Hv)).item() < 1e-13
     del Hu, vH, Hv2, u
     v32 = v.float()
     Hv32 = torch.empty_like(v32)
     ed.mul_(Hv32, opr, v32)
+    assert opr.last_strategy() == ("terms", "xspread")  # FP32: x spread over the word positions
     assert (torch.linalg.norm(Hv32.double() - Hv) / torch.linalg.norm(Hv)).item() < 1e-5
+    opr.set_strategy("terms", "bitmap")
+    Hv32b = torch.empty_like(v32)
+    ed.mul_(Hv32b, opr, v32)
+    opr.set_strategy()
+    assert torch.equal(Hv32, Hv32b)  # same products in the same order: bitwise equal

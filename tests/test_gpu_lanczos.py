@@ -127,3 +127,18 @@ def test_workspace_reuse_repeatable(ed):
     assert np.array_equal(runs[0], runs[2])
     assert np.array_equal(runs[1], runs[3]) and np.array_equal(runs[1], runs[4])
     assert np.all(np.abs(runs[0] - runs[1]) < 1e-10 * np.abs(runs[1]))
+
+
+def test_restart_in_row_chunks_space_group_6x6(ed):
+    """6x6 J1-J2, translations x C4v, A1 (31,566,122 states): a 12-vector
+    basis forces thick restarts whose combination runs in two row chunks of
+    the restart scratch; E0 = -18.13714754 as in the translation sector."""
+    hs, _ = ed.spin_half_system(36)
+    j1, j2 = ed.square_bonds(6, 6)
+    H = _heis(ed, hs, j1 + j2, [1.0] * len(j1) + [0.5] * len(j2))
+    r = ed.symmetry_reduce(ed.represent(ed.HilbertSpaceSector(hs, 0)), ed.SquareSpaceIrrep(6, (0, 0), "A1"))
+    assert r.dimension == 31566122
+    opr = ed.represent(r, H)
+    res = ed.eigs(opr, 1, tol=1e-10, max_basis=12, max_restarts=400, criterion="eigenvalue")
+    assert res.converged and res.restarts > 0
+    assert abs(res.values[0] - (-18.13714754)) < 1e-8
