@@ -154,6 +154,8 @@ SIGNATURES = {
     "edhip_vec_convert": (_i32, [_vp, _i32, _i32, _i64, _vp, _vp]),
     "edhip_event_record": (_i32, [_vp, _i32]),
     "edhip_event_elapsed": (_i32, [_vp, _i32, _i32, _P(_dbl)]),
+    "edhip_set_kernel_timing": (_i32, [_vp, _i32]),
+    "edhip_op_kernel_time": (_i32, [_vp, _i32, _P(_dbl), _P(_i64)]),
 }
 
 _lib = None

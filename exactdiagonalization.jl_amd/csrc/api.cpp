@@ -107,6 +107,7 @@ struct edhip_ctx {
   // allocations of hundreds of GB after frees cost seconds per call.
   DevBuf krylov;
   hipEvent_t ev[16] = {};
+  bool kernel_timing = false;  // time every matvec kernel launch (edhip_set_kernel_timing)
 
   char* krylov_arena(size_t bytes) {
     krylov.ensure(bytes);
@@ -827,6 +828,8 @@ struct edhip_op {
   int strat_kernel = 0;  // 0 auto, 1 term list, 2 pair groups
   int strat_lookup = 0;  // 0 auto, 1 sorted search (or rank), 2 hash table
   int used_kernel = 0, used_lookup = 0;
+  double kt_ms = 0.0;  // summed durations of the timed matvec kernel launches
+  int64_t kt_n = 0;
 
   bool is_complex() const { return complex_amp || (space->group && space->group->complex_phase); }
 };
@@ -1081,7 +1084,16 @@ void op_matvec(edhip_op* op, int prec, int transpose, int accumulate, const void
       }
     }
     if (idx_lookup == 4 && !d.xm.values) fail(EDHIP_ERR_ARGUMENT, "x cannot be spread over the words of this space");
+    if (op->ctx->kernel_timing) HIP_OR_THROW(hipEventRecord(op->ctx->ev[14], st));
     HIP_OR_THROW(launch_matvec(d, prec, transpose ? 1 : 0, accumulate ? 1 : 0, x, y, st));
+    if (op->ctx->kernel_timing) {
+      HIP_OR_THROW(hipEventRecord(op->ctx->ev[15], st));
+      HIP_OR_THROW(hipEventSynchronize(op->ctx->ev[15]));
+      float f = 0.f;
+      HIP_OR_THROW(hipEventElapsedTime(&f, op->ctx->ev[14], op->ctx->ev[15]));
+      op->kt_ms += f;
+      ++op->kt_n;
+    }
     return;
   }
   bool use_hash = false;
@@ -2504,6 +2516,23 @@ int edhip_vec_convert(edhip_ctx* ctx, int pi, int po, int64_t n, const void* x, 
 int edhip_event_record(edhip_ctx* ctx, int slot) {
   if (!ctx || slot < 0 || slot >= 16) return EDHIP_ERR_ARGUMENT;
   return guarded(ctx, [&] { HIP_OR_THROW(hipEventRecord(ctx->ev[slot], ctx->stream)); });
+}
+
+int edhip_set_kernel_timing(edhip_ctx* ctx, int on) {
+  if (!ctx) return EDHIP_ERR_ARGUMENT;
+  ctx->kernel_timing = on != 0;
+  return EDHIP_OK;
+}
+
+int edhip_op_kernel_time(edhip_op* op, int reset, double* ms_total, int64_t* launches) {
+  if (!op || !ms_total || !launches) return EDHIP_ERR_ARGUMENT;
+  *ms_total = op->kt_ms;
+  *launches = op->kt_n;
+  if (reset) {
+    op->kt_ms = 0.0;
+    op->kt_n = 0;
+  }
+  return EDHIP_OK;
 }
 
 int edhip_event_elapsed(edhip_ctx* ctx, int s0, int s1, double* ms) {

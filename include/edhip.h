@@ -317,8 +317,16 @@ int edhip_vec_gaussian(edhip_ctx* ctx, int prec, int64_t n, uint64_t seed, void*
 int edhip_vec_convert(edhip_ctx* ctx, int prec_in, int prec_out, int64_t n, const void* x, void* y);
 
 /* Timing on the context's stream (hipEvent pairs). */
-int edhip_event_record(edhip_ctx* ctx, int slot);           /* slot 0..15 */
+int edhip_event_record(edhip_ctx* ctx, int slot); /* slot 0..15 (14, 15 taken while kernel timing is on) */
 int edhip_event_elapsed(edhip_ctx* ctx, int slot0, int slot1, double* ms);
+/* Measurement (no counterpart in the reference): with timing on, every
+ * term-list matvec kernel launch of the context is bracketed by events
+ * (slots 14 and 15) and waited for, and its duration is added to its
+ * operator's total, which edhip_op_kernel_time reports (and clears when
+ * reset != 0).  The steps before the kernel (e.g. spreading x) are not
+ * included. */
+int edhip_set_kernel_timing(edhip_ctx* ctx, int on);
+int edhip_op_kernel_time(edhip_op* op, int reset, double* ms_total, int64_t* launches);
 
 #ifdef __cplusplus
 }
