@@ -288,9 +288,29 @@ struct GroupLds {
   const int32_t* coset_inv;
 };
 
+// Image of b under a bit permutation given as n_nib tables of 16 entries (the
+// image of each value of 4-bit chunk k at tab[16 k + v]).  The reads go out
+// four at a time, so that a chunk's read does not wait for the previous one;
+// in the last, partial group the chunks past n_nib read entry 0, the image of
+// no bits (0).
 __device__ __forceinline__ uint64_t perm_apply_nib(const uint64_t* tab, int n_nib, uint64_t b) {
   uint64_t out = 0;
-  for (int k = 0; k < n_nib; ++k) out |= tab[(k << 4) + (int)((b >> (4 * k)) & 15ull)];
+  int k0 = 0;
+  for (; k0 + 4 <= n_nib; k0 += 4) {
+    uint64_t v[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) v[q] = tab[((k0 + q) << 4) + (int)((b >> (4 * (k0 + q))) & 15ull)];
+    out |= (v[0] | v[1]) | (v[2] | v[3]);
+  }
+  if (k0 < n_nib) {  // 1 to 3 chunks left
+    uint64_t v[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const int k = k0 + q;
+      v[q] = tab[k < n_nib ? (k << 4) + (int)((b >> ((4 * k) & 63)) & 15ull) : 0];
+    }
+    out |= v[0] | v[1] | v[2];
+  }
   return out;
 }
 
@@ -443,12 +463,35 @@ __device__ __forceinline__ void canonicalize_torus_rows(const GroupDev& g, const
   }
 }
 
-// p(b) for the coset representative p
+// perm_apply_nib with the number of chunks fixed: all NNIB table reads are
+// issued before the first is used (the loop form waits for each in turn).
+template <int NNIB>
+__device__ __forceinline__ uint64_t perm_apply_nib_t(const uint64_t* tab, uint64_t b) {
+  uint64_t out = 0;
+#pragma unroll
+  for (int k = 0; k < NNIB; ++k) out |= tab[(k << 4) + (int)((b >> (4 * k)) & 15ull)];
+  return out;
+}
+
+// p(b) for the coset representative p.  NB > 0: words of NB bits, known at
+// compile time (the chunk count g.n_nib is then (NB + 3) / 4).  UNIFORM: p is
+// the same in every lane of the wave, so its kind and table are read once
+// into scalar registers and the branches on them are taken by the whole wave.
+template <int NB = 0, bool UNIFORM = false>
 __device__ __forceinline__ uint64_t apply_coset(const GroupDev& g, const GroupLds& L, int p, uint64_t b) {
-  const int kind = L.coset_kind[p];
+  int kind = L.coset_kind[p];
+  if (UNIFORM) kind = __builtin_amdgcn_readfirstlane(kind);
   if (kind == 0) return b;
   uint64_t w = b;
-  if (kind == 2) w = perm_apply_nib(L.coset_nib + L.coset_pidx[p] * (g.n_nib << 4), g.n_nib, b);
+  if (kind == 2) {
+    int pidx = L.coset_pidx[p];
+    if (UNIFORM) pidx = __builtin_amdgcn_readfirstlane(pidx);
+    const uint64_t* tab = L.coset_nib + pidx * (g.n_nib << 4);
+    if constexpr (NB > 0)
+      w = perm_apply_nib_t<(NB + 3) / 4>(tab, b);
+    else
+      w = perm_apply_nib(tab, g.n_nib, b);
+  }
   return w ^ L.coset_xr[p];
 }
 
@@ -476,11 +519,12 @@ template <int LXC = 0, int LYC = 0, bool PAIRS = true, bool NEED_H = true>
 __device__ __forceinline__ void canonicalize_torus_cosets(const GroupDev& g, const GroupLds& L, uint64_t b,
                                                           uint64_t& s, int& h, int& cnt) {
   const int nT = (LXC ? LXC : g.Lx) * (LYC ? LYC : g.Ly);
+  constexpr int NBC = LXC * LYC;  // word bits when the shape is fixed, else 0
   if (PAIRS && g.n_cosets > 1 && g.n_cosets <= 64) {
     uint32_t best = 0xffffffffu;
     uint64_t sel = 0;  // cosets reaching best
     for (int p = 0; p < g.n_cosets; ++p) {
-      const uint32_t v = torus_pair_min<LXC, LYC>(g, L, apply_coset(g, L, p, b));
+      const uint32_t v = torus_pair_min<LXC, LYC>(g, L, apply_coset<NBC, true>(g, L, p, b));
       sel = v < best ? (1ull << p) : (v == best ? sel | (1ull << p) : sel);
       best = v < best ? v : best;
     }
@@ -492,7 +536,7 @@ __device__ __forceinline__ void canonicalize_torus_cosets(const GroupDev& g, con
       sel &= sel - 1;
       uint64_t s2;
       int pos2, c2;
-      canonicalize_torus_pairs_t<LXC, LYC, NEED_H>(g, L, L.coset_inv + p * nT, apply_coset(g, L, p, b), s2, pos2, c2);
+      canonicalize_torus_pairs_t<LXC, LYC, NEED_H>(g, L, L.coset_inv + p * nT, apply_coset<NBC>(g, L, p, b), s2, pos2, c2);
       const int h2 = NEED_H ? L.coset_elem[p * nT + pos2] : 0;
       if (s2 < s) {
         s = s2;
@@ -512,7 +556,7 @@ __device__ __forceinline__ void canonicalize_torus_cosets(const GroupDev& g, con
     canonicalize_torus_rows(g, L, L.coset_inv, b, s, pos, cnt);
   h = NEED_H ? L.coset_elem[pos] : 0;
   for (int p = 1; p < g.n_cosets; ++p) {
-    const uint64_t w = apply_coset(g, L, p, b);
+    const uint64_t w = apply_coset<NBC, true>(g, L, p, b);
     uint64_t s2;
     int pos2, c2;
     if constexpr (PAIRS)

@@ -28,6 +28,23 @@ def test_full_space_and_multi_sector(ed, golden):
     assert ed.represent(ed.HilbertSpaceSector(hs, 0)).basis_list.tolist() == ka["basis"]
 
 
+def test_empty_space_operator(ed):
+    """An operator on an empty sector: zero-length products, a 0 x 0 sparse
+    and dense matrix (no kernel is launched on an empty grid)."""
+    hs, _ = ed.spin_half_system(4)
+    hsr = ed.represent(ed.HilbertSpaceSector(hs, 1))
+    assert hsr.dimension == 0
+    H = ed.simplify(sum((ed.heisenberg_bond(hs, i, (i + 1) % 4, 1.0) for i in range(4)), ed.NullOperator()))
+    opr = ed.represent(hsr, H)
+    assert opr.dimension == 0
+    out = np.zeros(0)
+    ed.mul_(out, opr, np.zeros(0))
+    ed.mul_(out, np.zeros(0), opr)
+    assert out.shape == (0,)
+    assert opr.sparse().shape == (0, 0)
+    assert opr.Matrix().shape == (0, 0)
+
+
 def test_explicit_list_and_lookup(ed, oracle):
     hs, _ = ed.spin_half_system(20)
     rng = np.random.default_rng(11)
