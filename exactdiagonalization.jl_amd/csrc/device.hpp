@@ -844,13 +844,22 @@ struct XMapDev {
   int pad_;
 };
 
+// x elements of two canonical images k0, k1 (f0, f1: the word has a
+// position), without branches: both code reads are issued, then both loads;
+// an image without a position loads position 0 and is dropped by the caller.
 template <typename V>
-__device__ __forceinline__ bool xmap_fetch(const XMapDev& X, const uint16_t* codes, uint64_t key, V& xv) {
-  if ((key >> X.bm_bits) != 0) return false;
-  const uint32_t c = codes[key >> X.top_shift];
-  if (c == 0xffffu) return false;
-  xv = ((const V*)X.values)[((uint64_t)c << X.top_shift) | (key & ((1ull << X.top_shift) - 1ull))];
-  return true;
+__device__ __forceinline__ void xmap_fetch2(const XMapDev& X, const uint16_t* codes, uint64_t k0, uint64_t k1, V& x0,
+                                            V& x1, bool& f0, bool& f1) {
+  const bool in0 = (k0 >> X.bm_bits) == 0, in1 = (k1 >> X.bm_bits) == 0;
+  const uint32_t c0 = codes[in0 ? (uint32_t)(k0 >> X.top_shift) : 0u];
+  const uint32_t c1 = codes[in1 ? (uint32_t)(k1 >> X.top_shift) : 0u];
+  f0 = in0 && c0 != 0xffffu;
+  f1 = in1 && c1 != 0xffffu;
+  const uint64_t low = (1ull << X.top_shift) - 1ull;
+  const uint64_t p0 = f0 ? (((uint64_t)c0 << X.top_shift) | (k0 & low)) : 0ull;
+  const uint64_t p1 = f1 ? (((uint64_t)c1 << X.top_shift) | (k1 & low)) : 0ull;
+  x0 = ((const V*)X.values)[p0];
+  x1 = ((const V*)X.values)[p1];
 }
 
 struct HashDev {

@@ -238,8 +238,8 @@ __global__ __launch_bounds__(kMvBlock) void k_matvec(OpDev op, const V* __restri
       int64_t ik[NW];
       V xv[NW];
       if (op.xm.values) {  // x of each image with one load at its word position
-        const bool f0 = xmap_fetch(op.xm, codes, sk[0], xv[0]);
-        const bool f1 = xmap_fetch(op.xm, codes, sk[1], xv[1]);
+        bool f0, f1;
+        xmap_fetch2(op.xm, codes, sk[0], sk[1], xv[0], xv[1], f0, f1);
         ik[0] = f0 ? 0 : -1;
         ik[1] = (ok[1] && f1) ? 0 : -1;
       } else {
