@@ -196,6 +196,9 @@ struct GroupHost {
 // per basis word, the matvec builds (6x6 k=(0,0): 18.2).
 constexpr int kXTopBits = 12;
 constexpr uint64_t kXMapRatio = 32;
+// Spreads of 8- and 16-byte elements are used by default only up to this
+// size (about what the last-level cache holds)
+constexpr uint64_t kXMapCachedBytes = 256ull << 20;
 
 // Generic groups whose 4-bit permutation tables exceed this stay in global
 // memory (read through L1/L2) instead of LDS.
@@ -309,40 +312,47 @@ struct edhip_space {
     }
   }
 
+  // Code table of the top bits of the words (built once); returns the number
+  // of spread positions, 0 when the space has no words.
+  uint64_t ensure_xtop() const {
+    if (dim == 0) return 0;
+    if (xtop_bits < 0) {
+      materialize();
+      uint64_t max_key = 0;
+      HIP_OR_THROW(hipMemcpyAsync(&max_key, keys.as<uint64_t>() + (dim - 1), 8, hipMemcpyDeviceToHost, ctx->stream));
+      ctx->sync();
+      const int K = std::max(4, bit_length(max_key));
+      const int t = std::min(kXTopBits, K);
+      const int sh = K - t;
+      DevBuf flags;
+      flags.alloc(((size_t)1 << t) * 4);
+      HIP_OR_THROW(hipMemsetAsync(flags.p, 0, ((size_t)1 << t) * 4, ctx->stream));
+      HIP_OR_THROW(launch_mark_tops(keys.as<uint64_t>(), dim, sh, flags.as<uint32_t>(), ctx->stream));
+      std::vector<uint32_t> fl((size_t)1 << t);
+      HIP_OR_THROW(hipMemcpyAsync(fl.data(), flags.p, fl.size() * 4, hipMemcpyDeviceToHost, ctx->stream));
+      ctx->sync();
+      std::vector<uint16_t> codes(fl.size(), 0xffff);
+      uint64_t nc = 0;
+      for (size_t v = 0; v < fl.size(); ++v)
+        if (fl[v]) codes[v] = (uint16_t)nc++;
+      upload(xtop, codes, ctx->stream);
+      xtop_bits = t;
+      xtop_shift = sh;
+      xtop_K = K;
+      xtop_codes = nc;
+    }
+    return xtop_codes << xtop_shift;
+  }
+
   const XMapHost* ensure_xmap(size_t es) const {
     const int slot = es == 4 ? 0 : es == 8 ? 1 : es == 16 ? 2 : -1;
-    if (slot < 0 || (is_sector && !group)) return nullptr;  // full sectors: rank arithmetic
+    if (slot < 0) return nullptr;
     XMapHost& X = xmap[slot];
     if (X.tried) return X.ok ? &X : nullptr;
     X.tried = true;
-    build_index();
     if (dim == 0) return nullptr;
     try {
-      if (xtop_bits < 0) {
-        uint64_t max_key = 0;
-        HIP_OR_THROW(hipMemcpyAsync(&max_key, keys.as<uint64_t>() + (dim - 1), 8, hipMemcpyDeviceToHost, ctx->stream));
-        ctx->sync();
-        const int K = std::max(4, bit_length(max_key));
-        const int t = std::min(kXTopBits, K);
-        const int sh = K - t;
-        DevBuf flags;
-        flags.alloc(((size_t)1 << t) * 4);
-        HIP_OR_THROW(hipMemsetAsync(flags.p, 0, ((size_t)1 << t) * 4, ctx->stream));
-        HIP_OR_THROW(launch_mark_tops(keys.as<uint64_t>(), dim, sh, flags.as<uint32_t>(), ctx->stream));
-        std::vector<uint32_t> fl((size_t)1 << t);
-        HIP_OR_THROW(hipMemcpyAsync(fl.data(), flags.p, fl.size() * 4, hipMemcpyDeviceToHost, ctx->stream));
-        ctx->sync();
-        std::vector<uint16_t> codes(fl.size(), 0xffff);
-        uint64_t nc = 0;
-        for (size_t v = 0; v < fl.size(); ++v)
-          if (fl[v]) codes[v] = (uint16_t)nc++;
-        upload(xtop, codes, ctx->stream);
-        xtop_bits = t;
-        xtop_shift = sh;
-        xtop_K = K;
-        xtop_codes = nc;
-      }
-      const uint64_t n_pos = xtop_codes << xtop_shift;
+      const uint64_t n_pos = ensure_xtop();
       // the spread may take up to kXMapRatio times the memory of the vector
       if (n_pos > (uint64_t)kXMapRatio * (uint64_t)dim && n_pos > (1ull << 20)) return nullptr;
       X.values.alloc(n_pos * es);
@@ -1051,6 +1061,28 @@ bool ensure_diag(edhip_op* op) {
   return true;
 }
 
+// Points d at the spread of x over the word positions, filled with x, when
+// the operator's lookup strategy calls for it: lookup 4, and auto for 4-byte
+// elements (6x6 J1-J2 FP32: 118 -> 97 ms); for wider ones on full sectors,
+// whose other lookup is rank arithmetic, a chain of table reads per image
+// (4x7 J1-J2: FP32 14.9 -> 4.6 ms, FP64 14.9 -> 8.1 ms), and where the spread
+// stays cache-sized; a large FP64 spread is no faster than the blocked rank
+// index (6x6: 127 vs 128 ms).  Returns whether it did.
+bool attach_spread(edhip_op* op, int prec, OpDev& d, const void* x, hipStream_t st) {
+  const edhip_space* sp = op->space;
+  const bool rank_mode = sp->is_sector && !sp->group;
+  const int idx_lookup = (op->strat_lookup == 2) ? 0 : op->strat_lookup;
+  const size_t es = elem_size(prec);
+  const bool want = idx_lookup == 4 ||
+                    (idx_lookup == 0 && (es == 4 || rank_mode || sp->ensure_xtop() * es <= kXMapCachedBytes));
+  if (!want || !d.codes_lds) return false;
+  const edhip_space::XMapHost* X = sp->ensure_xmap(es);
+  if (!X) return false;
+  HIP_OR_THROW(launch_xmap_fill(prec, X->dev, sp->keys.as<uint64_t>(), sp->dim, x, st));
+  d.xm = X->dev;
+  return true;
+}
+
 void op_matvec(edhip_op* op, int prec, int transpose, int accumulate, const void* x, void* y) {
   hipStream_t st = op->ctx->stream;
   const edhip_space* sp = op->space;
@@ -1071,18 +1103,7 @@ void op_matvec(edhip_op* op, int prec, int transpose, int accumulate, const void
     }
     op->used_kernel = 1;
     op->used_lookup = rank_mode ? 1 : (d.ix.bm_bits ? 3 : 1);
-    // lookup 4, and auto for 4-byte elements: x spread over the word positions
-    // when it fits (6x6 J1-J2: FP32 118 -> 97 ms; FP64 gains nothing, its
-    // spread being twice as large, so auto keeps the rank index there)
-    const bool want_spread = idx_lookup == 4 || (idx_lookup == 0 && elem_size(prec) == 4);
-    if (!rank_mode && want_spread && d.codes_lds) {
-      const edhip_space::XMapHost* X = sp->ensure_xmap(elem_size(prec));
-      if (X) {
-        HIP_OR_THROW(launch_xmap_fill(prec, X->dev, sp->keys.as<uint64_t>(), sp->dim, x, st));
-        d.xm = X->dev;
-        op->used_lookup = 4;
-      }
-    }
+    if (attach_spread(op, prec, d, x, st)) op->used_lookup = 4;
     if (idx_lookup == 4 && !d.xm.values) fail(EDHIP_ERR_ARGUMENT, "x cannot be spread over the words of this space");
     if (op->ctx->kernel_timing) HIP_OR_THROW(hipEventRecord(op->ctx->ev[14], st));
     HIP_OR_THROW(launch_matvec(d, prec, transpose ? 1 : 0, accumulate ? 1 : 0, x, y, st));
@@ -2371,10 +2392,15 @@ int edhip_term_expect(edhip_op* op, int prec, const void* phi, const void* psi, 
     int64_t nb = term_expect_blocks(op->dev_row);
     op->ex_partial.ensure((size_t)term_expect_scratch_rows(op->dev_row) * nt * 2 * 8);
     op->ex_out.ensure((size_t)nt * 2 * 8);
-    if (term_expect_window(op->dev_row, prec_is_complex(prec) ? 2 : 1) < 1)
+    const int nc = prec_is_complex(prec) ? 2 : 1;
+    if (term_expect_window(op->dev_row, nc) < 1)
       fail(EDHIP_ERR_ARGUMENT, "operator and symmetry tables leave no LDS room for the fused expectation kernel");
-    HIP_OR_THROW(launch_term_expect(op->dev_row, prec, phi, psi, op->ex_partial.as<double>(), nb,
-                                    op->ex_out.as<double>(), op->ctx->stream));
+    // psi of the images from its spread, as the products read x, where its
+    // code table leaves LDS room for the bins
+    OpDev d = op->dev_row;
+    if (attach_spread(op, prec, d, psi, op->ctx->stream) && term_expect_window(d, nc) < 1) d.xm = XMapDev{};
+    HIP_OR_THROW(launch_term_expect(d, prec, phi, psi, op->ex_partial.as<double>(), nb, op->ex_out.as<double>(),
+                                    op->ctx->stream));
     std::vector<double> h((size_t)nt * 2);
     HIP_OR_THROW(hipMemcpyAsync(h.data(), op->ex_out.p, h.size() * 8, hipMemcpyDeviceToHost, op->ctx->stream));
     op->ctx->sync();

@@ -844,6 +844,16 @@ struct XMapDev {
   int pad_;
 };
 
+// x element of one canonical image k (false: the word has no position).
+template <typename V>
+__device__ __forceinline__ bool xmap_fetch1(const XMapDev& X, const uint16_t* codes, uint64_t k, V& x) {
+  const bool in = (k >> X.bm_bits) == 0;
+  const uint32_t c = codes[in ? (uint32_t)(k >> X.top_shift) : 0u];
+  const bool f = in && c != 0xffffu;
+  x = ((const V*)X.values)[f ? (((uint64_t)c << X.top_shift) | (k & ((1ull << X.top_shift) - 1ull))) : 0ull];
+  return f;
+}
+
 // x elements of two canonical images k0, k1 (f0, f1: the word has a
 // position), without branches: both code reads are issued, then both loads;
 // an image without a position loads position 0 and is dropped by the caller.

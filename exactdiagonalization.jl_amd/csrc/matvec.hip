@@ -509,6 +509,12 @@ __device__ __forceinline__ void load_c(const V* p, int64_t i, double& re, double
   }
 }
 
+// LDS offset of the term pass's per-wave bins: after the operator's tables,
+// and after the spread's code table when psi is read from the spread.
+__host__ __device__ __forceinline__ uint32_t expect_bins_offset(const OpDev& op) {
+  return op.xm.values ? op.codes_lds + (2u << op.xm.top_bits) : op.lds_total;
+}
+
 constexpr int kExpectBlocksMax = 2048;
 constexpr int kExpectWaves = kBlock / 64;
 
@@ -530,7 +536,11 @@ __global__ __launch_bounds__(kBlock) void k_term_expect(OpDev op, const V* __res
   const MvTerms T = stage_mv_terms(op, lds, false);
   lds_copy(lds + op.w_lds, op.ix.W, op.w_bytes);
   const uint64_t* W = (const uint64_t*)(lds + op.w_lds);
-  double* bins = (double*)(lds + op.lds_total);  // [wave][nw][NC]
+  // psi of the images from its spread over the word positions (op.xm set):
+  // the code table after the operator's tables, then the bins
+  uint16_t* codes = (uint16_t*)(lds + op.codes_lds);
+  if (op.xm.values) lds_copy((unsigned char*)codes, op.xm.topcode, 2u << op.xm.top_bits);
+  double* bins = (double*)(lds + expect_bins_offset(op));  // [wave][nw][NC]
   for (int i = threadIdx.x; i < kExpectWaves * nw * NC; i += kBlock) bins[i] = 0.0;
   __syncthreads();
   double* mine = bins + (threadIdx.x >> 6) * nw * NC;
@@ -574,16 +584,22 @@ __global__ __launch_bounds__(kBlock) void k_term_expect(OpDev op, const V* __res
           uint64_t s;
           int h, cnt;
           Canon<MODE>::run(op.g, L, img, s, h, cnt);
-          const int64_t idx = index_of(op.ix, W, s);
-          if (idx < 0) continue;
+          double sr, si;
+          if (op.xm.values) {
+            V xv;
+            if (!xmap_fetch1(op.xm, codes, s, xv)) continue;
+            load_c(&xv, 0, sr, si);
+          } else {
+            const int64_t idx = index_of(op.ix, W, s);
+            if (idx < 0) continue;
+            load_c(psi, idx, sr, si);
+          }
           double wr = 1.0, wi = 0.0;
           if constexpr (MODE != kGroupNone) {
             const double sc = sqrt_own * L.sqcnt[cnt];
             wr = L.ph_re[h] * sc;
             if constexpr (CX) wi = L.ph_im[h] * sc;
           }
-          double sr, si;
-          load_c(psi, idx, sr, si);
           const double qr = wr * sr - wi * si;  // w * psi_s
           const int t = n_diag + j - t0;
           if constexpr (CX) {
@@ -637,7 +653,7 @@ hipError_t launch_expect_t(const OpDev& op, const void* phi, const void* psi, do
   if (nw_max <= 0) return hipErrorInvalidValue;
   for (int t0 = 0; t0 < n_terms; t0 += nw_max) {
     const int nw = n_terms - t0 < nw_max ? n_terms - t0 : nw_max;
-    const uint32_t lds = op.lds_total + (uint32_t)(kExpectWaves * nw * (VT<V>::cx ? 2 : 1) * 8);
+    const uint32_t lds = expect_bins_offset(op) + (uint32_t)(kExpectWaves * nw * (VT<V>::cx ? 2 : 1) * 8);
     hipLaunchKernelGGL((k_term_expect<V, MODE>), dim3((unsigned)n_blocks), dim3(kBlock), lds, st, op, (const V*)phi,
                        (const V*)psi, t0, nw, d_partial);
     hipError_t e = hipGetLastError();
@@ -782,7 +798,7 @@ hipError_t launch_sparse_compact(int64_t dim, int slot, const int64_t* d_colptr,
 // terms per pass: the per-wave bins (nc doubles per term) fill the LDS left
 // over by the operator and symmetry tables (64 KiB per block)
 int term_expect_window(const OpDev& op, int nc) {
-  const int64_t room = 64 * 1024 - (int64_t)op.lds_total;
+  const int64_t room = 64 * 1024 - (int64_t)expect_bins_offset(op);
   const int64_t per_term = (int64_t)kExpectWaves * nc * 8;
   const int64_t nw = room / per_term;
   const int n_terms = op.n_diag + op.n_off;
