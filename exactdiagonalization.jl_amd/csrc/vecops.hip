@@ -71,14 +71,23 @@ __global__ __launch_bounds__(kBlock) void k_dot(int64_t n, const V* __restrict__
   }
 }
 
-// out[j] = sum_b partial[b*width + j], fixed order
+// out[j] = sum_b partial[b*width + j] in a fixed order: block j; thread t
+// adds the partials b = t, t + kBlock, ... in turn, then a fixed tree in LDS.
+// (One thread per column, adding all nb partials in a chain of dependent
+// loads, took ~150 us per reduction.)
 __global__ __launch_bounds__(kBlock) void k_sum_partials(const double* __restrict__ partial, int nb, int width,
                                                          double* __restrict__ out) {
-  int j = blockIdx.x * kBlock + threadIdx.x;
-  if (j >= width) return;
+  __shared__ double sh[kBlock];
+  const int j = blockIdx.x;
   double s = 0.0;
-  for (int b = 0; b < nb; ++b) s += partial[(int64_t)b * width + j];
-  out[j] = s;
+  for (int b = threadIdx.x; b < nb; b += kBlock) s += partial[(int64_t)b * width + j];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = kBlock / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[j] = sh[0];
 }
 
 // Tile of kTile vectors: partial[(blk)*(2k) + 2j..] = sum_i conj(V_j,i) w_i
@@ -363,7 +372,7 @@ template <typename V>
 struct DotOp {
   static hipError_t run(int64_t n, const void* x, const void* y, double* partial, double* out, hipStream_t s) {
     hipLaunchKernelGGL(k_dot<V>, dim3(kRedBlocks), dim3(kBlock), 0, s, n, (const V*)x, (const V*)y, partial);
-    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(kBlock), 0, s, partial, kRedBlocks, 2, out);
+    hipLaunchKernelGGL(k_sum_partials, dim3(2), dim3(kBlock), 0, s, partial, kRedBlocks, 2, out);
     return hipGetLastError();
   }
 };
@@ -374,6 +383,7 @@ template <typename V>
 struct MultiDotOp {
   static hipError_t run(int64_t n, int k, const void* Vm, int64_t ldv, const void* w, double* partial, double* out,
                         hipStream_t s) {
+    if (k <= 0) return hipSuccess;
     if (aligned16(Vm) && aligned16(w) && ((ldv * (int64_t)sizeof(V)) & 15) == 0) {
       constexpr int KT = VT<V>::cx ? 8 : 16;
       const int nb = kRedBlocks;
@@ -381,7 +391,7 @@ struct MultiDotOp {
       hipLaunchKernelGGL((k_multi_dot16<V, KT>), grid, dim3(kBlock), 0, s, n, k, (const V*)Vm, ldv, (const V*)w,
                          partial);
       const int width = 2 * k;
-      hipLaunchKernelGGL(k_sum_partials, dim3((width + kBlock - 1) / kBlock), dim3(kBlock), 0, s, partial, nb, width,
+      hipLaunchKernelGGL(k_sum_partials, dim3(width), dim3(kBlock), 0, s, partial, nb, width,
                          out);
       return hipGetLastError();
     }
@@ -389,7 +399,7 @@ struct MultiDotOp {
     dim3 grid(nb, (k + kTile - 1) / kTile);
     hipLaunchKernelGGL(k_multi_dot<V>, grid, dim3(kBlock), 0, s, n, k, (const V*)Vm, ldv, (const V*)w, partial);
     int width = 2 * k;
-    hipLaunchKernelGGL(k_sum_partials, dim3((width + kBlock - 1) / kBlock), dim3(kBlock), 0, s, partial, nb, width, out);
+    hipLaunchKernelGGL(k_sum_partials, dim3(width), dim3(kBlock), 0, s, partial, nb, width, out);
     return hipGetLastError();
   }
 };
