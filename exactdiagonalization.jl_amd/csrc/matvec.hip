@@ -645,13 +645,69 @@ __global__ __launch_bounds__(kBlock) void k_reduce_partials(const double* __rest
   out[i] = s;
 }
 
+// Diagonal terms of the term pass: g_t = sum_r [b_r & mask_t == match_t]
+// conj(phi_r) psi_r, one thread per term (tiles of kBlock terms).  The block
+// stages a chunk of its rows (word and conj(phi_r) psi_r) in LDS and every
+// thread walks the chunk in order, adding into registers: no atomics, since
+// each term has its own lane (with LDS atomics, one add per matched term and
+// row, the diagonal terms took half of the pass).  Rows are taken
+// grid-stride in chunks of kBlock; partial[block][2 t] as the term pass.
+template <typename V>
+__global__ __launch_bounds__(kBlock) void k_diag_expect(OpDev op, const V* __restrict__ phi,
+                                                        const V* __restrict__ psi, double* __restrict__ partial,
+                                                        int width) {
+  constexpr bool CX = VT<V>::cx;
+  __shared__ uint64_t rb[kBlock];
+  __shared__ double rv[kBlock], ri[kBlock];
+  for (int tt = 0; tt < op.n_diag; tt += kBlock) {
+    const int t = tt + (int)threadIdx.x;
+    uint64_t m = 0, c = 1;  // no term: (b & 0) != 1, never matches
+    if (t < op.n_diag) {
+      m = op.diag[t].mask;
+      c = op.diag[t].match;
+    }
+    double ar = 0.0, ai = 0.0;
+    for (int64_t r0 = (int64_t)blockIdx.x * kBlock; r0 < op.dim; r0 += (int64_t)gridDim.x * kBlock) {
+      const int64_t r = r0 + threadIdx.x;
+      __syncthreads();  // the previous chunk is done with
+      if (r < op.dim) {
+        double fr, fi, pr, pi;
+        load_c(phi, r, fr, fi);
+        load_c(psi, r, pr, pi);
+        rb[threadIdx.x] = op.rows[r];
+        rv[threadIdx.x] = fr * pr + fi * pi;  // conj(phi_r) psi_r
+        ri[threadIdx.x] = fr * pi - fi * pr;
+      }
+      __syncthreads();
+      const int nr = op.dim - r0 < kBlock ? (int)(op.dim - r0) : kBlock;
+      for (int j = 0; j < nr; ++j) {
+        if ((rb[j] & m) == c) {
+          ar += rv[j];
+          if constexpr (CX) ai += ri[j];
+        }
+      }
+    }
+    if (t < op.n_diag) {
+      partial[(int64_t)blockIdx.x * width + 2 * t] = ar;
+      partial[(int64_t)blockIdx.x * width + 2 * t + 1] = ai;
+    }
+  }
+}
+
 template <typename V, int MODE>
 hipError_t launch_expect_t(const OpDev& op, const void* phi, const void* psi, double* d_partial, int64_t n_blocks,
                            double* d_out, hipStream_t st) {
   const int n_terms = op.n_diag + op.n_off;
   const int nw_max = term_expect_window(op, VT<V>::cx ? 2 : 1);
   if (nw_max <= 0) return hipErrorInvalidValue;
-  for (int t0 = 0; t0 < n_terms; t0 += nw_max) {
+  // diagonal terms in their own kernel, then windows of the off-diagonal ones
+  if (op.n_diag > 0) {
+    hipLaunchKernelGGL(k_diag_expect<V>, dim3((unsigned)n_blocks), dim3(kBlock), 0, st, op, (const V*)phi,
+                       (const V*)psi, d_partial, n_terms * 2);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  for (int t0 = op.n_diag; t0 < n_terms; t0 += nw_max) {
     const int nw = n_terms - t0 < nw_max ? n_terms - t0 : nw_max;
     const uint32_t lds = expect_bins_offset(op) + (uint32_t)(kExpectWaves * nw * (VT<V>::cx ? 2 : 1) * 8);
     hipLaunchKernelGGL((k_term_expect<V, MODE>), dim3((unsigned)n_blocks), dim3(kBlock), lds, st, op, (const V*)phi,
