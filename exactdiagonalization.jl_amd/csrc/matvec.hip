@@ -519,8 +519,9 @@ constexpr int kExpectBlocksMax = 2048;
 constexpr int kExpectWaves = kBlock / 64;
 
 // Per-term bilinear forms g_t = sum_r conj(phi_r) sum_{images of t} w psi_s
-// (unit amplitudes), for the terms [t0, t0 + nw) of the combined numbering
-// (diagonal terms first, then off-diagonal ones).  One lane per row, as in the
+// (unit amplitudes), for the off-diagonal terms [t0, t0 + nw) of the combined
+// numbering (diagonal terms first, t0 >= n_diag; the diagonal ones are
+// k_diag_expect's).  One lane per row, as in the
 // matvec: a lane walks its row's matched terms and adds each contribution to
 // its wave's bin of that term in LDS (float64 LDS atomics; a wave only adds
 // to its own bins, in program order).  Blocks walk the rows grid-stride; the
@@ -545,28 +546,12 @@ __global__ __launch_bounds__(kBlock) void k_term_expect(OpDev op, const V* __res
   __syncthreads();
   double* mine = bins + (threadIdx.x >> 6) * nw * NC;
   const int n_diag = op.n_diag, n_terms = op.n_diag + op.n_off;
-  const int d_hi = t0 + nw < n_diag ? t0 + nw : n_diag;              // diagonal terms [t0, d_hi)
-  const int o_lo = (t0 > n_diag ? t0 : n_diag) - n_diag;              // off-diagonal terms [o_lo, o_hi)
+  const int o_lo = t0 - n_diag;                                        // off-diagonal terms [o_lo, o_hi)
   const int o_hi = (t0 + nw < n_terms ? t0 + nw : n_terms) - n_diag;
   for (int64_t row = (int64_t)blockIdx.x * kBlock + threadIdx.x; row < op.dim; row += (int64_t)gridDim.x * kBlock) {
     const uint64_t b = op.rows[row];
-    double fr, fi, pr, pi;
+    double fr, fi;
     load_c(phi, row, fr, fi);
-    load_c(psi, row, pr, pi);
-    for (int seg = t0; seg < d_hi; seg += 64) {
-      const int n = d_hi - seg < 64 ? d_hi - seg : 64;
-      uint64_t bits = 0;
-      for (int j = 0; j < n; ++j) {
-        const DiagTerm d = op.diag[seg + j];
-        bits |= (uint64_t)((b & d.mask) == d.match) << j;
-      }
-      while (bits) {
-        const int t = seg + __builtin_ctzll(bits);
-        bits &= bits - 1;
-        atomicAdd(mine + (t - t0) * NC, fr * pr + fi * pi);  // conj(phi_r) psi_r
-        if constexpr (CX) atomicAdd(mine + (t - t0) * NC + 1, fr * pi - fi * pr);
-      }
-    }
     if (o_lo < o_hi) {
       double sqrt_own = 1.0;
       if constexpr (MODE != kGroupNone) sqrt_own = sqrt((double)op.orbit[row]);
