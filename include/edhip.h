@@ -191,18 +191,21 @@ int edhip_apply_host(edhip_op* op, int prec, int transpose, int accumulate, int6
  *   kernel: 0 auto (term list), 1 term-list kernel, 2 bond-pair kernel
  *           (operators whose terms act on one or two bits, plain or
  *           torus-translation spaces)
- *   lookup: 0 auto (term-list kernel: FP32 / 4-byte elements x spread
- *             over word positions when it fits; else bitmap, else sorted
- *             search),
- *           1 sorted search behind the prefix table,
+ *   lookup: 0 auto (term-list kernel: x spread over word positions when
+ *             it fits, for 4-byte elements, for full popcount sectors and
+ *             where the spread is at most 256 MB; else bitmap, else sorted
+ *             search, or rank arithmetic for a full popcount sector),
+ *           1 sorted search behind the prefix table (rank arithmetic for a
+ *             full popcount sector),
  *           2 open-addressing table of the basis with the input vector
  *             embedded (bond-pair kernel only; built on first use),
  *           3 bitmap rank index (words of at most 36 bits),
- *           4 x spread over the positions of the words (term-list kernel,
- *             reduced spaces): one load per image, no rank; the spread (at
- *             most 32 elements per basis word) is allocated per element size
- *             on first use and refreshed before every product
- *   a full popcount sector always uses rank arithmetic.
+ *           4 x spread over the positions of the words (term-list kernel):
+ *             one load per image, no rank; the spread (at most 32 elements
+ *             per basis word) is allocated per element size on first use
+ *             and refreshed before every product (and before the term pass
+ *             of edhip_term_expect, which reads psi through it);
+ *   on a full popcount sector lookups 1 to 3 use rank arithmetic.
  * edhip_op_last_strategy reports what the last product used. */
 int edhip_op_set_strategy(edhip_op* op, int kernel, int lookup);
 int edhip_op_last_strategy(const edhip_op* op, int* kernel, int* lookup);
