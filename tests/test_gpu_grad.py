@@ -157,3 +157,50 @@ def test_term_expectations_bitwise_repeatable(ed):
     x = model.expectations(phi, psi)
     y = model.expectations(phi, psi)
     assert np.array_equal(x, y)
+
+
+@pytest.mark.parametrize("k,dtype", [((1, 0), "complex128"), ((1, 0), "complex64"), ((0, 0), "float64"),
+                                     ((2, 2), "float32")])
+@pytest.mark.parametrize("lookup", ["auto", "bitmap"])
+def test_term_expectations_reduced_vs_oracle(ed, oracle, k, dtype, lookup):
+    """The per-term pass on a reduced space (diagonal-term kernel and
+    off-diagonal pass; psi of the images read through the spread of x or the
+    bitmap rank index): 4x4 J1-J2 torus with a longitudinal and a transverse
+    field, translation sector k, every term's <phi|T_t|psi> (unit amplitude)
+    against the oracle's reduced representation of that term alone."""
+    import torch
+
+    Lx = Ly = 4
+    n = Lx * Ly
+    hs, s = ed.spin_half_system(n)
+    j1, j2 = ed.square_bonds(Lx, Ly)
+    H = ed.simplify(sum([ed.heisenberg_bond(hs, i, j, 1.0) for i, j in j1]
+                        + [ed.heisenberg_bond(hs, i, j, 0.5) for i, j in j2]
+                        + [0.3 * s(i, "z") + 0.15 * s(i, "x") for i in range(n)], ed.NullOperator()))
+    irrep = ed.TranslationIrrep((Lx, Ly), k)
+    rh = ed.symmetry_reduce(ed.represent(ed.HilbertSpaceSector(hs, 0)), irrep)
+    perms = np.stack([ed.bit_permutation(hs, p) for p, _ in irrep.symops_and_amplitudes()])
+    red = oracle.symmetry_reduce(oracle.sector_basis(n, 0), n, perms, [a for _, a in irrep.symops_and_amplitudes()])
+    opr = ed.represent(rh, H)
+    opr.set_strategy("terms", lookup)
+    D = opr.dimension
+    rng = np.random.default_rng(5)
+    phi = rng.standard_normal(D) + 1j * rng.standard_normal(D)
+    psi = rng.standard_normal(D) + 1j * rng.standard_normal(D)
+    if not dtype.startswith("complex"):
+        phi, psi = phi.real.copy(), psi.real.copy()
+    tdt = getattr(torch, dtype)
+    phi_d = torch.tensor(phi, dtype=tdt, device="cuda")
+    psi_d = torch.tensor(psi, dtype=tdt, device="cuda")
+    got = ed.term_expectations(opr, phi_d, psi_d)
+    # the values the device saw (rounded to the storage type)
+    phi_c = phi_d.cpu().numpy().astype(np.complex128)
+    psi_c = psi_d.cpu().numpy().astype(np.complex128)
+    mask, row, col, _ = ed.terms_arrays(H)
+    tol = 1e-12 if dtype in ("complex128", "float64") else 1e-5
+    scale = np.linalg.norm(phi_c) * np.linalg.norm(psi_c)
+    assert got.size == mask.size
+    for t in range(mask.size):
+        o = oracle.Opr(red, (mask[t:t + 1], row[t:t + 1], col[t:t + 1], np.ones(1, complex)))
+        ref = np.vdot(phi_c, o.matvec(psi_c))
+        assert abs(got[t] - ref) <= tol * scale, (t, got[t], ref)
