@@ -556,12 +556,14 @@ void build_group(GroupHost& gh, int nb, double tol, hipStream_t st) {
       best_map = g_of_ab;
     }
   }
-  // cosets T p, representatives in element order (the identity first)
+  // cosets T p: representatives without an XOR word in element order (the
+  // identity first), each followed by the cosets of its products with the
+  // group's XOR words (same permutation, so that a canonicaliser applying the
+  // cosets in order permutes once for both), then any coset left
   std::vector<int32_t> coset_rep, coset_elem;
   if (bestLx > 0) {
     std::vector<char> covered(G, 0);
-    for (int g = 0; g < G; ++g) {
-      if (covered[g]) continue;
+    auto add_coset = [&](int g) {
       coset_rep.push_back(g);
       for (int q = 0; q < nb; ++q) {
         const int t = best_map[q];
@@ -571,7 +573,21 @@ void build_group(GroupHost& gh, int nb, double tol, hipStream_t st) {
         covered[it->second] = 1;
         coset_elem.push_back(it->second);
       }
+    };
+    std::vector<uint64_t> xwords;
+    for (int g = 0; g < G; ++g)
+      if (gh.xr[g] && std::find(xwords.begin(), xwords.end(), gh.xr[g]) == xwords.end()) xwords.push_back(gh.xr[g]);
+    for (int g = 0; g < G; ++g) {
+      if (covered[g] || gh.xr[g]) continue;
+      add_coset(g);
+      const std::vector<int32_t> pg(gh.perm.begin() + (size_t)g * nb, gh.perm.begin() + (size_t)(g + 1) * nb);
+      for (uint64_t x : xwords) {
+        auto it = index.find(std::make_pair(pg, x));
+        if (it != index.end() && !covered[it->second]) add_coset(it->second);
+      }
     }
+    for (int g = 0; g < G; ++g)
+      if (!covered[g]) add_coset(g);
   }
   gh.mode = (bestLx > 0) ? kGroupTorus : kGroupGeneric;
   if (G == 1) gh.mode = kGroupGeneric;

@@ -523,8 +523,27 @@ __device__ __forceinline__ void canonicalize_torus_cosets(const GroupDev& g, con
   if (PAIRS && g.n_cosets > 1 && g.n_cosets <= 64) {
     uint32_t best = 0xffffffffu;
     uint64_t sel = 0;  // cosets reaching best
+    // p(b) of the last permutation applied: cosets that share a permutation
+    // (a representative and its products with XOR words) follow each other
+    int last = -1;
+    uint64_t pb = 0;
     for (int p = 0; p < g.n_cosets; ++p) {
-      const uint32_t v = torus_pair_min<LXC, LYC>(g, L, apply_coset<NBC, true>(g, L, p, b));
+      const int kind = __builtin_amdgcn_readfirstlane(L.coset_kind[p]);
+      uint64_t w = b;
+      if (kind == 2) {
+        const int pidx = __builtin_amdgcn_readfirstlane(L.coset_pidx[p]);
+        if (pidx != last) {
+          const uint64_t* tab = L.coset_nib + pidx * (g.n_nib << 4);
+          if constexpr (NBC > 0)
+            pb = perm_apply_nib_t<(NBC + 3) / 4>(tab, b);
+          else
+            pb = perm_apply_nib(tab, g.n_nib, b);
+          last = pidx;
+        }
+        w = pb;
+      }
+      if (kind != 0) w ^= L.coset_xr[p];
+      const uint32_t v = torus_pair_min<LXC, LYC>(g, L, w);
       sel = v < best ? (1ull << p) : (v == best ? sel | (1ull << p) : sel);
       best = v < best ? v : best;
     }
