@@ -718,6 +718,27 @@ void build_group(GroupHost& gh, int nb, double tol, hipStream_t st) {
     d.coset_kind = gh.coset_kind_d.as<int32_t>();
     d.coset_elem = gh.coset_elem_d.as<int32_t>();
     d.coset_inv = gh.coset_inv_d.as<int32_t>();
+    // the same, packed into kernel arguments when small enough
+    d.coset_regs = 0;
+    uint64_t xw = 0;
+    bool one_x = true;
+    for (int c = 0; c < nc; ++c)
+      if (cxr[c]) {
+        if (xw && cxr[c] != xw) one_x = false;
+        xw = cxr[c];
+      }
+    if (nc <= 16 && pidx.size() <= 16 && one_x) {
+      d.coset_regs = 1;
+      d.ck_bits = 0;
+      d.cp_bits = 0;
+      d.cx_bits = 0;
+      d.cxword = xw;
+      for (int c = 0; c < nc; ++c) {
+        d.ck_bits |= (uint32_t)ckind[c] << (2 * c);
+        d.cp_bits |= (uint64_t)cpidx[c] << (4 * c);
+        if (cxr[c]) d.cx_bits |= 1u << c;
+      }
+    }
     L.bytes_cnib = (uint32_t)(cnib.size() * 8);
     L.bytes_cxr = (uint32_t)(nc * 8);
     L.bytes_ckind = (uint32_t)(nc * 4);

@@ -239,6 +239,15 @@ struct GroupDev {
   int has_xor;          // some element XORs a mask into the permuted word (global spin flip)
   int n_cosets;         // torus: G / (Lx Ly) cosets T p of the translations (1: translations only)
   int nib_global;       // generic: the permutation tables are read from nib (global), not LDS
+  // torus cosets in kernel arguments (coset_regs = 1: at most 16 cosets, 16
+  // permutations and one XOR word): 2 bits of kind and 4 bits of permutation
+  // per coset, bit p of cx_bits when coset p XORs cxword
+  int coset_regs;
+  uint32_t ck_bits;
+  uint64_t cp_bits;
+  uint32_t cx_bits;
+  uint32_t pad_cx_;
+  uint64_t cxword;
   uint64_t row_mask;    // torus: (1 << Lx) - 1
   uint64_t full_mask;   // (1 << nb) - 1
   const uint64_t* nib;  // generic: G * n_nib * 16 images of 4-bit chunks
@@ -528,10 +537,14 @@ __device__ __forceinline__ void canonicalize_torus_cosets(const GroupDev& g, con
     int last = -1;
     uint64_t pb = 0;
     for (int p = 0; p < g.n_cosets; ++p) {
-      const int kind = __builtin_amdgcn_readfirstlane(L.coset_kind[p]);
+      // the coset's kind and permutation: from the kernel arguments, or read
+      // once per wave from LDS
+      const int kind = g.coset_regs ? (int)((g.ck_bits >> (2 * p)) & 3u)
+                                    : __builtin_amdgcn_readfirstlane(L.coset_kind[p]);
       uint64_t w = b;
       if (kind == 2) {
-        const int pidx = __builtin_amdgcn_readfirstlane(L.coset_pidx[p]);
+        const int pidx = g.coset_regs ? (int)((g.cp_bits >> (4 * p)) & 15u)
+                                      : __builtin_amdgcn_readfirstlane(L.coset_pidx[p]);
         if (pidx != last) {
           const uint64_t* tab = L.coset_nib + pidx * (g.n_nib << 4);
           if constexpr (NBC > 0)
@@ -542,7 +555,10 @@ __device__ __forceinline__ void canonicalize_torus_cosets(const GroupDev& g, con
         }
         w = pb;
       }
-      if (kind != 0) w ^= L.coset_xr[p];
+      if (g.coset_regs)
+        w ^= ((g.cx_bits >> p) & 1u) ? g.cxword : 0ull;
+      else if (kind != 0)
+        w ^= L.coset_xr[p];
       const uint32_t v = torus_pair_min<LXC, LYC>(g, L, w);
       sel = v < best ? (1ull << p) : (v == best ? sel | (1ull << p) : sel);
       best = v < best ? v : best;
