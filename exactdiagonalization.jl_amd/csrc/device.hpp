@@ -488,6 +488,18 @@ __device__ __forceinline__ uint64_t perm_apply_nib_t(const uint64_t* tab, uint64
 // into scalar registers and the branches on them are taken by the whole wave.
 template <int NB = 0, bool UNIFORM = false>
 __device__ __forceinline__ uint64_t apply_coset(const GroupDev& g, const GroupLds& L, int p, uint64_t b) {
+  if (g.coset_regs) {  // kind, permutation and XOR flag from the kernel arguments
+    const int kind = (int)((g.ck_bits >> (2 * p)) & 3u);
+    uint64_t w = b;
+    if (kind == 2) {
+      const uint64_t* tab = L.coset_nib + (int)((g.cp_bits >> (4 * p)) & 15u) * (g.n_nib << 4);
+      if constexpr (NB > 0)
+        w = perm_apply_nib_t<(NB + 3) / 4>(tab, b);
+      else
+        w = perm_apply_nib(tab, g.n_nib, b);
+    }
+    return w ^ (((g.cx_bits >> p) & 1u) ? g.cxword : 0ull);
+  }
   int kind = L.coset_kind[p];
   if (UNIFORM) kind = __builtin_amdgcn_readfirstlane(kind);
   if (kind == 0) return b;
