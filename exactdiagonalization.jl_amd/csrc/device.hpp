@@ -225,6 +225,12 @@ constexpr int kTorusPairs6x6 = 4;
 constexpr int kTorusPairs4x6 = 5;
 constexpr int kTorusPairs6x4 = 6;
 constexpr int kTorusPairs4x4 = 7;
+// the same shapes for the translations alone (one coset): kernels without
+// the coset code, which the larger space-group kernels carry
+constexpr int kTransPairs6x6 = 8;
+constexpr int kTransPairs4x6 = 9;
+constexpr int kTransPairs6x4 = 10;
+constexpr int kTransPairs4x4 = 11;
 
 constexpr int kMaxLy = 64;
 

@@ -38,6 +38,14 @@ struct Canon {
       canonicalize_torus_pairs<6, 4, NEED_H>(g, L, b, s, h, cnt);
     } else if constexpr (MODE == kTorusPairs4x4) {
       canonicalize_torus_pairs<4, 4, NEED_H>(g, L, b, s, h, cnt);
+    } else if constexpr (MODE >= kTransPairs6x6 && MODE <= kTransPairs4x4) {
+      // translations only: the identity coset, whose element table maps the
+      // translation found to the group element
+      constexpr int LX = (MODE == kTransPairs6x6 || MODE == kTransPairs6x4) ? 6 : 4;
+      constexpr int LY = (MODE == kTransPairs6x6 || MODE == kTransPairs4x6) ? 6 : 4;
+      int pos;
+      canonicalize_torus_pairs_t<LX, LY, NEED_H>(g, L, L.coset_inv, b, s, pos, cnt);
+      h = NEED_H ? L.coset_elem[pos] : 0;
     } else {
       canonicalize(g, L, b, s, h, cnt);
     }
@@ -291,6 +299,12 @@ hipError_t launch_mv_v(const OpDev& op, int transpose, int accumulate, const voi
     default:
       if (op.g.pair_table) {
         const int Lx = op.g.Lx, Ly = op.g.Ly;
+        if (op.g.n_cosets == 1) {
+          if (Lx == 6 && Ly == 6) return launch_mv_t<V, kTransPairs6x6>(op, transpose, accumulate, x, y, st);
+          if (Lx == 4 && Ly == 6) return launch_mv_t<V, kTransPairs4x6>(op, transpose, accumulate, x, y, st);
+          if (Lx == 6 && Ly == 4) return launch_mv_t<V, kTransPairs6x4>(op, transpose, accumulate, x, y, st);
+          if (Lx == 4 && Ly == 4) return launch_mv_t<V, kTransPairs4x4>(op, transpose, accumulate, x, y, st);
+        }
         if (Lx == 6 && Ly == 6) return launch_mv_t<V, kTorusPairs6x6>(op, transpose, accumulate, x, y, st);
         if (Lx == 4 && Ly == 6) return launch_mv_t<V, kTorusPairs4x6>(op, transpose, accumulate, x, y, st);
         if (Lx == 6 && Ly == 4) return launch_mv_t<V, kTorusPairs6x4>(op, transpose, accumulate, x, y, st);
