@@ -79,6 +79,7 @@ class LanczosOpts(ctypes.Structure):
         ("reorth", ctypes.c_int),
         ("seed", ctypes.c_uint64),
         ("tol", ctypes.c_double),
+        ("tol_fp32", ctypes.c_double),
     ]
 
 
@@ -95,6 +96,7 @@ class LanczosStats(ctypes.Structure):
         ("t_total", ctypes.c_double),
         ("t_matvec", ctypes.c_double),
         ("t_refine", ctypes.c_double),
+        ("t_alloc", ctypes.c_double),
     ]
 
 
@@ -111,6 +113,7 @@ SIGNATURES = {
     "edhip_destroy": (None, [_vp]),
     "edhip_last_error": (ctypes.c_char_p, [_vp]),
     "edhip_set_stream": (_i32, [_vp, _vp]),
+    "edhip_get_stream": (_i32, [_vp, _P(_vp)]),
     "edhip_synchronize": (_i32, [_vp]),
     "edhip_malloc": (_i32, [_vp, _u64, _P(_vp)]),
     "edhip_free": (_i32, [_vp, _vp]),
@@ -209,6 +212,33 @@ class Context:
 
     def set_stream(self, stream_ptr):
         self.check(lib().edhip_set_stream(self.handle, ctypes.c_void_p(stream_ptr)))
+        self._stream = None
+
+    def stream_ptr(self) -> int:
+        s = ctypes.c_void_p()
+        self.check(lib().edhip_get_stream(self.handle, ctypes.byref(s)))
+        return s.value or 0
+
+    def order_after(self, torch_stream):
+        """Run the context's next calls after the work queued so far on
+        ``torch_stream``, with no host synchronisation: a non-default torch
+        stream becomes the context's stream (one queue); for the default
+        (null) stream, which the library cannot adopt, the context's own
+        stream waits on an event of it.  Returns the torch stream the calls
+        run on."""
+        import torch
+
+        ptr = torch_stream.cuda_stream
+        if ptr:
+            if getattr(self, "_stream", None) != ptr:
+                self.set_stream(ptr)
+                self._stream = ptr
+            return torch_stream
+        if getattr(self, "_stream", None) is not None:
+            self.set_stream(0)
+        own = torch.cuda.ExternalStream(self.stream_ptr(), device=torch_stream.device)
+        own.wait_stream(torch_stream)
+        return own
 
     def __del__(self):
         try:

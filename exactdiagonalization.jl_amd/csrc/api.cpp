@@ -1227,7 +1227,9 @@ void householder_tridiag(int n, std::vector<double>& A, std::vector<double>& Q, 
 }
 
 // Implicit QL with Wilkinson-type shifts on a symmetric tridiagonal matrix
-// (d diagonal, e[i] coupling i and i+1).  Z (row-major n x n) accumulates the
+// (d diagonal, e[i] coupling i and i+1): the published tql2 procedure of
+// EISPACK (Bowdler, Martin, Reinsch and Wilkinson, "The QR and QL
+// algorithms for symmetric matrices", Numer. Math. 11 (1968) 293-306).  Z (row-major n x n) accumulates the
 // rotations: on entry the basis the tridiagonal matrix is expressed in, on
 // exit the eigenvectors as columns.  Eigenvalues are sorted ascending.
 bool tridiag_ql(int n, std::vector<double>& d, std::vector<double>& e, std::vector<double>& Z) {
@@ -1436,11 +1438,43 @@ struct TRLanczos {
   //                Hermitian bound |theta - lambda| <= r^2 / gap on the value.
   int criterion = 0;
   double gap_hint = 0.0;
+  // stationary: criterion 1 also asks theta to be unchanged since the previous
+  // check (the Lanczos residual estimate of an FP32 basis is not a true
+  // residual); false where beta is the true FP64 residual ||H y - theta y||,
+  // which bounds the eigenvalue error by itself
+  bool stationary = true;
+  bool accept_invariant = false;
+
+  // Random directions after an invariant subspace (beta ~ 0): the Krylov
+  // space of the start vector misses every eigenvector orthogonal to it
+  // (degenerate copies included), so, as ARPACK's getv0 does, the iteration
+  // goes on from a random unit vector orthogonal to the locked vectors and to
+  // the basis, with a zero coupling.
+  uint64_t seed = 0;
+  int injections = 0;
+
+  void inject(int nb, void* w) {
+    ++injections;
+    HIP_OR_THROW(vec_gaussian(prec, n, seed ^ (0x9E3779B97F4A7C15ull * (uint64_t)injections), w, op->ctx->stream));
+    for (int pass = 0; pass < 2; ++pass) {
+      if (n_locked) orthogonalize(locked, n, n_locked, w);
+      orthogonalize(Vp, ld, nb, w);
+    }
+    const double nn = std::sqrt(std::max(dot(w, w).real(), 0.0));
+    if (!(nn > 0.0)) fail(EDHIP_ERR_NOCONV, "no direction left orthogonal to the Krylov basis");
+    scale(w, 1.0 / nn);
+  }
 
   // Runs the iteration from the unit vector in vec(0).  Returns the lowest
   // n_ret >= nev Ritz values (convergence is required of the lowest nev);
   // Ritz vectors of the lowest nev (if out != nullptr) are written in
   // precision out_prec with stride n.  res: residual norms of the lowest nev.
+  // Convergence is accepted at a step whose beta is not tiny, once the basis
+  // spans the whole (deflated) space, or at the end of a full basis holding
+  // two or more closed Krylov blocks; after a random direction was injected
+  // it is tested only when the basis is full, as ARPACK tests at the end of
+  // an Arnoldi cycle, so that the eigenvalues of every Krylov block built so
+  // far (degenerate copies among them) are in the projected matrix.
   bool run(int nev, double tol, int max_restarts, int check_every, std::vector<double>& theta, std::vector<double>& res,
            void* out, int out_prec, int n_ret = 0) {
     edhip_ctx* ctx = op->ctx;
@@ -1456,8 +1490,15 @@ struct TRLanczos {
     int j = 0;
     double beta_last = 0.0;
     int size = 0;  // current basis size used for the final Ritz step
+    bool arrow = false;  // T is arrowhead + tridiagonal after a thick restart, else tridiagonal
+    auto ritz = [&](int sz) {  // eigenpairs of the leading sz x sz block of T
+      std::vector<double> Tj((size_t)sz * sz);
+      for (int a = 0; a < sz; ++a)
+        for (int b = 0; b < sz; ++b) Tj[(size_t)a * sz + b] = T[(size_t)a * mm + b];
+      sym_eig(sz, Tj, !arrow, w, Z);
+    };
     for (;;) {
-      bool arrow = (k > 0);
+      arrow = (k > 0);
       for (j = k; j < mm; ++j) {
         void* vj = vec(j);
         void* wv = vec(j + 1);
@@ -1468,50 +1509,63 @@ struct TRLanczos {
         T[(size_t)j * mm + j] = c[2 * (size_t)j];
         std::complex<double> nn = dot(wv, wv);
         double beta = std::sqrt(std::max(nn.real(), 0.0));
-        beta_last = beta;
         size = j + 1;
-        // convergence check on the leading (j+1) x (j+1) block
-        bool last = (j + 1 == mm);
-        bool tiny = beta <= 1e-13 * std::max(1.0, std::fabs(T[(size_t)j * mm + j]));
-        bool check = last || tiny || ((j + 1 - k) % std::max(1, check_every) == 0 && j + 1 >= nev);
+        const bool last = (j + 1 == mm);
+        const bool exhausted = (int64_t)(j + 1) >= dim - n_locked;
+        const bool tiny = beta <= 1e-13 * std::max(1.0, std::fabs(T[(size_t)j * mm + j]));
+        // settle: a refinement that started on an eigenvector (to working
+        // precision) takes it; everything else continues past an invariant
+        // subspace from a random direction (unit, orthogonal), zero coupling
+        const bool settle = tiny && !exhausted && accept_invariant && j + 1 >= nev;
+        const bool invariant = tiny && !exhausted && !settle;
+        if (invariant) {
+          inject(j + 1, wv);
+          beta = 0.0;
+        }
+        beta_last = beta;
+        const bool periodic = (j + 1 - k) % std::max(1, check_every) == 0 && j + 1 >= nev;
+        // at the end of the basis also after an invariant step once two or more
+        // Krylov blocks are complete (an operator whose blocks all close at
+        // once, e.g. zero, has only such steps)
+        const bool check = exhausted || settle || (last && (!invariant || injections >= 2)) ||
+                           (!invariant && injections == 0 && periodic);
         if (check) {
-          std::vector<double> Tj((size_t)(j + 1) * (j + 1));
-          for (int a = 0; a <= j; ++a)
-            for (int b = 0; b <= j; ++b) Tj[(size_t)a * (j + 1) + b] = T[(size_t)a * mm + b];
-          sym_eig(j + 1, Tj, !arrow, w, Z);
+          ritz(j + 1);
           bool ok = (j + 1 >= nev);
           res.assign(nev, 0.0);
           for (int i = 0; i < nev && i <= j; ++i) {
             res[i] = std::fabs(beta * Z[(size_t)j * (j + 1) + i]);
             const double scale = tol * std::max(1.0, std::fabs(w[i]));
             bool ok_i = res[i] <= scale;
-            if (!ok_i && criterion == 1 && prev.size() > (size_t)i) {
+            if (!ok_i && criterion == 1 && (prev.size() > (size_t)i || !stationary)) {
               double gap = (i + 1 <= j) ? (w[i + 1] - w[i]) : gap_hint;
               if (gap_hint > 0.0 && (i + 1 > j || gap > gap_hint)) gap = gap_hint;
               gap = std::max(gap, 1e-12);
-              ok_i = (res[i] * res[i] / gap <= scale) && (std::fabs(w[i] - prev[i]) <= scale);
+              ok_i = (res[i] * res[i] / gap <= scale) && (!stationary || std::fabs(w[i] - prev[i]) <= scale);
             }
             if (!ok_i) ok = false;
           }
           prev.assign(w.begin(), w.begin() + std::min<size_t>(w.size(), (size_t)nev));
-          if (ok || tiny) {
-            converged = ok || tiny;
+          if (ok) {
+            converged = true;
             break;
+          }
+          if (exhausted) break;  // the whole space is in T: nothing left to add
+          if (settle) {
+            inject(j + 1, wv);
+            beta = beta_last = 0.0;
           }
         }
         if (j + 1 < mm) {
           T[(size_t)(j + 1) * mm + j] = T[(size_t)j * mm + j + 1] = beta;
-          scale(wv, 1.0 / beta);
+          if (beta > 0.0) scale(wv, 1.0 / beta);
         }
       }
-      if (converged) break;
+      if (converged || size >= dim - n_locked) break;
       // thick restart
       if (restarts >= max_restarts) break;
       ++restarts;
-      std::vector<double> Tm((size_t)mm * mm);
-      for (int a = 0; a < mm; ++a)
-        for (int b = 0; b < mm; ++b) Tm[(size_t)a * mm + b] = T[(size_t)a * mm + b];
-      sym_eig(mm, Tm, !arrow, w, Z);
+      ritz(mm);
       int keep = keep_count(mm, n_ret);
       // new basis: V'_i = sum_j Z[j][i] V_j (i < keep), V'_keep = normalised residual
       std::vector<double> coef((size_t)mm * keep);
@@ -1532,8 +1586,9 @@ struct TRLanczos {
         HIP_OR_THROW(hipMemcpy2DAsync(vrow, (size_t)ld * es(), tmpp, (size_t)len * es(), (size_t)len * es(), keep,
                                       hipMemcpyDeviceToDevice, ctx->stream));
       }
-      // residual direction: w = vec(mm) (unnormalised), beta_last its norm
-      scale(vec(mm), 1.0 / beta_last);
+      // residual direction: vec(mm), unnormalised with norm beta_last, or a
+      // unit random direction injected after an invariant subspace (beta_last 0)
+      if (beta_last > 0.0) scale(vec(mm), 1.0 / beta_last);
       HIP_OR_THROW(hipMemcpyAsync(vec(keep), vec(mm), (size_t)n * es(), hipMemcpyDeviceToDevice, ctx->stream));
       std::fill(T.begin(), T.end(), 0.0);
       for (int i = 0; i < keep; ++i) {
@@ -1542,15 +1597,22 @@ struct TRLanczos {
         T[(size_t)i * mm + keep] = T[(size_t)keep * mm + i] = s;
       }
       k = keep;
+      size = keep;
       ctx->sync();
     }
     // Ritz values / vectors from the current (size x size) block
-    theta.assign(w.begin(), w.begin() + std::min<size_t>(n_ret, w.size()));
+    if (!converged) {
+      ritz(size);
+      res.assign(nev, 0.0);
+      for (int i = 0; i < nev && i < size; ++i) res[i] = std::fabs(beta_last * Z[(size_t)(size - 1) * size + i]);
+    }
+    const int nv = std::min(nev, size);
+    theta.assign(w.begin(), w.begin() + std::min<size_t>((size_t)n_ret, w.size()));
     if (out) {
       int sz = size;
-      std::vector<double> coef((size_t)sz * nev);
+      std::vector<double> coef((size_t)sz * nev, 0.0);
       for (int a = 0; a < sz; ++a)
-        for (int i = 0; i < nev; ++i) coef[(size_t)a * nev + i] = Z[(size_t)a * sz + i];
+        for (int i = 0; i < nv; ++i) coef[(size_t)a * nev + i] = Z[(size_t)a * sz + i];
       coef_d.ensure(coef.size() * sizeof(double));
       HIP_OR_THROW(hipMemcpyAsync(coef_d.p, coef.data(), coef.size() * sizeof(double), hipMemcpyHostToDevice,
                                   ctx->stream));
@@ -1611,6 +1673,12 @@ int edhip_set_stream(edhip_ctx* ctx, void* s) {
     ctx->sync();
     ctx->stream = s ? (hipStream_t)s : ctx->own;
   });
+}
+
+int edhip_get_stream(const edhip_ctx* ctx, void** s) {
+  if (!ctx || !s) return EDHIP_ERR_ARGUMENT;
+  *s = (void*)ctx->stream;
+  return EDHIP_OK;
 }
 
 int edhip_synchronize(edhip_ctx* ctx) {
@@ -2310,7 +2378,7 @@ void edhip_lanczos_default_opts(edhip_lanczos_opts* o) {
 int edhip_lanczos(edhip_op* op, const edhip_lanczos_opts* opts, double* evals, void* evecs, edhip_lanczos_stats* stats) {
   if (!op || !opts || !evals) return EDHIP_ERR_ARGUMENT;
   return guarded(op->ctx, [&] {
-    Timer total;
+    Timer talloc;
     edhip_ctx* ctx = op->ctx;
     const int64_t n = op->space->dim;
     const bool cx = op->is_complex();
@@ -2327,6 +2395,7 @@ int edhip_lanczos(edhip_op* op, const edhip_lanczos_opts* opts, double* evals, v
     TRLanczos L;
     L.op = op;
     L.reorth = opts->reorth;
+    L.seed = opts->seed;
     L.prec = prec_store;
     L.set_size(n);
     L.m = (int)std::min<int64_t>(opts->max_basis, n);
@@ -2343,6 +2412,11 @@ int edhip_lanczos(edhip_op* op, const edhip_lanczos_opts* opts, double* evals, v
     const size_t ref_v = mixed ? align256((size_t)(m_ref + 1) * L.ld * es64) : 0;
     const size_t ref_t = mixed ? align256((size_t)keep_count(m_ref, 2) * rows_t * es64) : 0;
     char* arena = krylov_arena_for(op, ritz_b + std::max(main_v + main_t, ref_v + ref_t));
+    ctx->sync();
+    st.t_alloc = talloc.sec();
+    // the time to E0 starts once the workspace is there (a first call
+    // allocates it, later calls of the same size reuse it)
+    Timer total;
     void* ritz = arena;
     L.Vp = arena + ritz_b;
     L.tmpp = arena + ritz_b + main_v;
@@ -2352,9 +2426,11 @@ int edhip_lanczos(edhip_op* op, const edhip_lanczos_opts* opts, double* evals, v
     L.scale(L.vec(0), 1.0 / nrm);
 
     // The FP32 phase stops on the eigenvalue criterion (its residual cannot go
-    // below single-precision rounding); the FP64 phase uses the caller's rule.
+    // below single-precision rounding) at tol_fp32 (default max(tol, 1e-8));
+    // the FP64 phase uses the caller's rule.
     L.criterion = mixed ? 1 : opts->criterion;
-    const double tol_main = mixed ? std::max(opts->tol, 1e-8) : opts->tol;
+    if (opts->tol_fp32 < 0.0) fail(EDHIP_ERR_ARGUMENT, "tol_fp32 must be >= 0");
+    const double tol_main = mixed ? std::max(opts->tol, opts->tol_fp32 > 0.0 ? opts->tol_fp32 : 1e-8) : opts->tol;
     // Large bases check convergence at every step: the host eigensolve of the
     // small projected matrix costs far less than the product it may save.
     const bool large = n > (int64_t)1 << 22;
@@ -2370,18 +2446,26 @@ int edhip_lanczos(edhip_op* op, const edhip_lanczos_opts* opts, double* evals, v
     if (mixed) {
       Timer tr;
       conv = true;
-      // FP64 refinement with locking: eigenpair i is refined by short FP64
-      // thick-restart cycles started from its mixed-precision Ritz vector,
-      // orthogonal to the already refined ones.
+      // FP64 residual correction with locking: eigenpair i restarts an FP64
+      // Lanczos from its mixed-precision Ritz vector y, orthogonal to the
+      // already refined ones.  Its first step is the FP64 residual: theta =
+      // y^T H y (Rayleigh quotient) and beta = ||H y - theta y||, which
+      // bounds the eigenvalue error by beta^2 / gap; when that bound (or
+      // the residual rule) holds, one FP64 product settles the eigenvalue.
+      // Otherwise the FP64 basis grows from [y, r] (Rayleigh-Ritz on it at
+      // every step) and thick-restarts until it holds.
       const std::vector<double> theta32 = theta;
       for (int i = 0; i < nev; ++i) {
         TRLanczos R;
         R.op = op;
         R.reorth = opts->reorth;
+        R.seed = opts->seed + 0x632BE59BD9B4E019ull * (uint64_t)(i + 1);
         R.prec = prec64;
         R.set_size(n);
         R.m = (int)std::min<int64_t>(std::max(opts->refine_basis, 2), n - i);
         R.criterion = opts->criterion;
+        R.stationary = false;
+        R.accept_invariant = true;
         R.gap_hint = (theta32.size() > (size_t)i + 1) ? (theta32[i + 1] - theta32[i]) : 0.0;
         R.Vp = arena + ritz_b;
         R.tmpp = arena + ritz_b + ref_v;
@@ -2395,7 +2479,7 @@ int edhip_lanczos(edhip_op* op, const edhip_lanczos_opts* opts, double* evals, v
         double nr = std::sqrt(R.dot(R.vec(0), R.vec(0)).real());
         R.scale(R.vec(0), 1.0 / nr);
         std::vector<double> th1, rs1;
-        bool ok = R.run(1, opts->tol, std::max(opts->max_refine - 1, 0), 2, th1, rs1, start, prec64, 2);
+        bool ok = R.run(1, opts->tol, std::max(opts->max_refine - 1, 0), 1, th1, rs1, start, prec64, 2);
         theta[i] = th1[0];
         res[i] = rs1.empty() ? 0.0 : rs1[0];
         st.matvecs_fp64 += R.matvecs;

@@ -22,12 +22,70 @@ from typing import Callable, Sequence
 import numpy as np
 
 from . import solve
+from ._lib import ArgumentError
 from .operators import NullOperator, PureOperator, SumOperator, simplify, terms_arrays
-from .representation import represent
+from .representation import (
+    DirectProductOperation,
+    GlobalBitFlip,
+    ReducedHilbertSpaceRepresentation,
+    SitePermutation,
+    _perm_and_flip,
+    represent,
+)
 
 
 def _key(t: PureOperator):
     return (t.bitmask, t.bitrow, t.bitcol)
+
+
+def _term_map(op, perm_bits=None, xor=0):
+    """{(mask, row, col): amplitude} of a simplified operator, after the bit
+    permutation perm_bits (bit i -> bit perm_bits[i]) and the global flip
+    (xor != 0: every bit of the mask inverted in row and column, the action
+    of GlobalBitFlip, bitflipsymmetry.jl:23-35, on a PureOperator)."""
+
+    def pb(w):
+        if perm_bits is None:
+            return w
+        out, i = 0, 0
+        while w:
+            if w & 1:
+                out |= 1 << int(perm_bits[i])
+            w >>= 1
+            i += 1
+        return out
+
+    out = {}
+    for t in ([] if isinstance(op, NullOperator) else op.terms):
+        m, r, c = t.bitmask, t.bitrow, t.bitcol
+        if xor:
+            r, c = m & ~r, m & ~c
+        out[(pb(m), pb(r), pb(c))] = complex(t.amplitude)
+    return out
+
+
+def _check_invariant(space, operators, tol=1e-12):
+    """On a symmetry-reduced space the representation applies an operator to
+    the representative of each row only (reduced_operator_representation.jl:
+    57-85), which is the operator's action on the symmetric states only when
+    it commutes with every group element; a coupling operator that does not
+    would give silently wrong <psi|O_k|psi>.  Raises ArgumentError."""
+    if not isinstance(space, ReducedHilbertSpaceRepresentation):
+        return
+    hs = space.hilbert_space
+    elems = []
+    for g, _ in space.symops_and_amplitudes:
+        if not isinstance(g, (SitePermutation, GlobalBitFlip, DirectProductOperation)):
+            raise ArgumentError(f"unsupported symmetry operation {g!r}")
+        elems.append(_perm_and_flip(hs, g))
+    for k, o in enumerate(operators):
+        base = _term_map(o)
+        for perm, xor in elems:
+            img = _term_map(o, perm, xor)
+            if img.keys() != base.keys() or any(abs(img[q] - base[q]) > tol for q in base):
+                raise ArgumentError(
+                    f"operator {k} is not invariant under the symmetry group of the reduced space: its "
+                    "expectation in the reduced basis is not that of the operator")
 
 
 class LinearCouplings:
@@ -36,6 +94,7 @@ class LinearCouplings:
     def __init__(self, space, operators: Sequence):
         self.space = space
         self.operators = [simplify(o) for o in operators]
+        _check_invariant(space, self.operators)
         keys = {}
         for o in self.operators:
             for t in ([] if isinstance(o, NullOperator) else o.terms):
@@ -87,6 +146,7 @@ def correlator_and_grad(model: LinearCouplings, c, dc_dtheta, observable_op, *, 
     """<O> in the ground state and d<O>/dtheta via the MINRES adjoint."""
     import torch
 
+    _check_invariant(model.space, [simplify(observable_op)])
     H, r = ground_state(model, c, **kw)
     E0 = float(r.values[0])
     psi = r.vectors[0].contiguous()
@@ -138,6 +198,7 @@ def _torch_function():
         @staticmethod
         def forward(ctx, theta, model, coupling_fn, observable_op, opts=None):
             opts = opts or {}
+            _check_invariant(model.space, [simplify(observable_op)])
             th = theta.detach().cpu().numpy()
             c, _ = coupling_fn(th)
             H, r = ground_state(model, c, **opts)

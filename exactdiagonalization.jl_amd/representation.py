@@ -493,15 +493,19 @@ def _is_torch(x):
 
 
 def _device_call(ctx, fn, *args):
-    """Run a library call on GPU tensors.  The library works on its own
-    non-blocking stream: wait for the caller's current stream before the call
-    (its producers of the inputs) and for the library's stream after it (so
-    that the outputs are complete for whatever reads them next)."""
+    """Run a library call on GPU tensors in stream order, without host
+    synchronisation: the call is queued after the caller's current torch
+    stream (the producers of its inputs), and that stream waits for it (its
+    outputs are complete for whatever the caller queues next).  The memory of
+    the inputs stays valid meanwhile: later work on the caller's stream,
+    which may reuse it once it is freed, is queued after the call."""
     import torch
 
-    torch.cuda.current_stream(ctx.device).synchronize()
+    cur = torch.cuda.current_stream(ctx.device)
+    run = ctx.order_after(cur)
     ctx.check(fn(*args))
-    ctx.synchronize()
+    if run is not cur:
+        cur.wait_stream(run)
 
 
 def _torch_prec(t):

@@ -58,6 +58,11 @@ void edhip_destroy(edhip_ctx* ctx);
 const char* edhip_last_error(const edhip_ctx* ctx);
 /* Use an external hipStream_t (NULL restores the context's own stream). */
 int edhip_set_stream(edhip_ctx* ctx, void* hip_stream);
+/* The hipStream_t the context's calls run on (its own non-blocking stream,
+ * or the one set by edhip_set_stream), so that a caller can order its work
+ * against the library's with events instead of host synchronisation.  No
+ * counterpart in the reference (Julia has no streams at this boundary). */
+int edhip_get_stream(const edhip_ctx* ctx, void** hip_stream);
 int edhip_synchronize(edhip_ctx* ctx);
 /* Device memory of the context's device (plain hipMalloc / hipFree). */
 int edhip_malloc(edhip_ctx* ctx, uint64_t bytes, void** out);
@@ -187,10 +192,15 @@ int edhip_apply_host(edhip_op* op, int prec, int transpose, int accumulate, int6
                      int64_t n_y, void* y_host);
 
 /* Kernel and lookup used by apply/mul/eigensolver calls (performance knob;
- * every choice gives the same matrix):
+ * every choice gives the same matrix).  The automatic choice (0, 0) is the
+ * product path; the bond-pair kernel (kernel 2) and the open-addressing
+ * table (lookup 2) are never chosen automatically, measured slower on every
+ * workload tried (e.g. 514 vs 97 ms per 6x6 product for the table), and are
+ * kept only as alternative implementations the parity tests cross-check
+ * against (test-only).
  *   kernel: 0 auto (term list), 1 term-list kernel, 2 bond-pair kernel
  *           (operators whose terms act on one or two bits, plain or
- *           torus-translation spaces)
+ *           torus-translation spaces; test-only)
  *   lookup: 0 auto (term-list kernel: x spread over word positions when
  *             it fits, for 4-byte elements, for full popcount sectors and
  *             where the spread is at most 256 MB; else bitmap, else sorted
@@ -198,7 +208,8 @@ int edhip_apply_host(edhip_op* op, int prec, int transpose, int accumulate, int6
  *           1 sorted search behind the prefix table (rank arithmetic for a
  *             full popcount sector),
  *           2 open-addressing table of the basis with the input vector
- *             embedded (bond-pair kernel only; built on first use),
+ *             embedded (bond-pair kernel only; built on first use;
+ *             test-only),
  *           3 bitmap rank index (words of at most 36 bits),
  *           4 x spread over the positions of the words (term-list kernel):
  *             one load per image, no rank; the spread (at most 32 elements
@@ -271,6 +282,8 @@ typedef struct {
                         ||w||^2; 1: always two passes (CGS2) */
   uint64_t seed;     /* start vector: SplitMix64 + Box-Muller Gaussian */
   double tol;        /* see criterion */
+  double tol_fp32;   /* mixed mode: eigenvalue-criterion tolerance of the FP32
+                        phase, max(tol, tol_fp32); 0 selects max(tol, 1e-8) */
 } edhip_lanczos_opts;
 
 typedef struct {
@@ -282,9 +295,12 @@ typedef struct {
   int second_passes;  /* steps whose reorthogonalisation took a second pass */
   double residual[16];     /* final ||H y - theta y|| / max(1,|theta|) (FP64) */
   double unrefined[16];    /* mixed mode: Ritz values before refinement */
-  double t_total;          /* seconds */
+  double t_total;          /* seconds from the start vector to the eigenvalues
+                              (the workspace allocation excluded) */
   double t_matvec;         /* seconds in storage-precision matvecs */
   double t_refine;         /* seconds in the FP64 refinement */
+  double t_alloc;          /* seconds getting the Krylov workspace (0 when a
+                              previous call's workspace is reused) */
 } edhip_lanczos_stats;
 
 void edhip_lanczos_default_opts(edhip_lanczos_opts* opts);

@@ -83,6 +83,12 @@ def lib():
         L.orc_apply.argtypes = opr_args + [ctypes.c_int, dp, dp]
         L.orc_sparse.restype = i64
         L.orc_sparse.argtypes = opr_args + [ctypes.c_double, i64p, i64p, dp, dp, i64]
+        pp = ctypes.POINTER
+        L.orc_sparse_parallel.restype = i64
+        L.orc_sparse_parallel.argtypes = opr_args + [ctypes.c_double, ctypes.c_int, pp(i64p), pp(i64p), pp(dp),
+                                                     pp(dp)]
+        L.orc_free.restype = None
+        L.orc_free.argtypes = [ctypes.c_void_p]
         L.orc_get_element.restype = None
         L.orc_get_element.argtypes = opr_args + [i64, i64, dp, dp]
         L.orc_vector_reduce.restype = None
@@ -331,6 +337,26 @@ class Opr:
         nnz = lib().orc_sparse(*self._args(), float(tol), _p(colptr, ctypes.c_int64), _p(rowval, ctypes.c_int64), _p(nr, ctypes.c_double), _p(ni, ctypes.c_double), cap)
         assert nnz >= 0
         return colptr, rowval[:nnz].copy(), nr[:nnz] + 1j * ni[:nnz]
+
+    def sparse_parallel(self, tol=RTOL_DEFAULT, n_threads=None):
+        """(colptr, rowval, nzval) 0-based CSC, as sparse_parallel (column
+        blocks over n_threads OpenMP threads, default os.cpu_count())."""
+        nt = int(n_threads or os.cpu_count() or 1)
+        i64p = ctypes.POINTER(ctypes.c_int64)
+        dp = ctypes.POINTER(ctypes.c_double)
+        cp, rv, nr, ni = i64p(), i64p(), dp(), dp()
+        nnz = lib().orc_sparse_parallel(*self._args(), float(tol), nt, ctypes.byref(cp), ctypes.byref(rv),
+                                        ctypes.byref(nr), ctypes.byref(ni))
+        if nnz < 0:
+            raise MemoryError("orc_sparse_parallel: allocation failed")
+        try:
+            colptr = np.ctypeslib.as_array(cp, shape=(self.dim + 1,)).copy()
+            rowval = np.ctypeslib.as_array(rv, shape=(max(nnz, 1),))[:nnz].copy()
+            nz = np.ctypeslib.as_array(nr, shape=(max(nnz, 1),))[:nnz] + 1j * np.ctypeslib.as_array(ni, shape=(max(nnz, 1),))[:nnz]
+        finally:
+            for q in (cp, rv, nr, ni):
+                lib().orc_free(ctypes.cast(q, ctypes.c_void_p))
+        return colptr, rowval, nz
 
     def scipy_sparse(self, tol=RTOL_DEFAULT):
         import scipy.sparse as sp

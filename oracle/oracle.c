@@ -438,6 +438,132 @@ int64_t orc_sparse(ORC_OPR_ARGS, double tol, int64_t* colptr, int64_t* rowval, d
   return overflow ? -nnz : nnz;
 }
 
+/* sparse_parallel(opr; tol): abstract_operator_representation.jl:172-204.
+ * The columns are split into one contiguous block per thread (splitrange);
+ * each block builds its columns as sparse_serial does (merge in iteration
+ * order, choptol!, rows ascending) into block-local lists, and the lists are
+ * concatenated in block order after the cumulative sum of the column sizes.
+ * Same CSC as orc_sparse; OpenMP threads stand for Julia's threads.  The
+ * arrays are allocated here (colptr: dim + 1, the others: nnz entries) and
+ * released with orc_free.  Returns nnz, or -1 when an allocation failed. */
+int64_t orc_sparse_parallel(ORC_OPR_ARGS, double tol, int n_threads, int64_t** colptr_out, int64_t** rowval_out,
+                            double** nz_re_out, double** nz_im_out) {
+  orc_opr o;
+  fill_opr(&o, ORC_OPR_PASS);
+  const int64_t dim = opr_dim(&o);
+  int nblocks = n_threads > 0 ? n_threads : 1;
+  if (nblocks > dim && dim > 0) nblocks = (int)dim;
+  int64_t** lrow = (int64_t**)calloc((size_t)nblocks, sizeof(int64_t*));
+  cplx** lval = (cplx**)calloc((size_t)nblocks, sizeof(cplx*));
+  int64_t* lsize = (int64_t*)calloc((size_t)nblocks, sizeof(int64_t));
+  int64_t* colptr = (int64_t*)malloc(sizeof(int64_t) * (size_t)(dim + 1));
+  if (!lrow || !lval || !lsize || !colptr) {
+    free(lrow);
+    free(lval);
+    free(lsize);
+    free(colptr);
+    return -1;
+  }
+  int failed = 0;
+#pragma omp parallel for num_threads(nblocks) schedule(static, 1) reduction(| : failed)
+  for (int ib = 0; ib < nblocks; ++ib) {
+    const int64_t c0 = dim * ib / nblocks, c1 = dim * (ib + 1) / nblocks;
+    int64_t* idx = (int64_t*)malloc(sizeof(int64_t) * (size_t)(n_terms > 0 ? n_terms : 1));
+    cplx* val = (cplx*)malloc(sizeof(cplx) * (size_t)(n_terms > 0 ? n_terms : 1));
+    int64_t capb = 1024, nb = 0;
+    int64_t* rb = (int64_t*)malloc(sizeof(int64_t) * (size_t)capb);
+    cplx* vb = (cplx*)malloc(sizeof(cplx) * (size_t)capb);
+    int bad = !rb || !vb || !idx || !val;
+    for (int64_t c = c0; c < c1 && !bad; ++c) {
+      int64_t n = col_elements(&o, c, idx, val);
+      int64_t k2 = 0;
+      for (int64_t k = 0; k < n; ++k) {
+        if (idx[k] >= 0 && idx[k] < dim) { /* `if 1 <= irow <= m` */
+          idx[k2] = idx[k];
+          val[k2] = val[k];
+          ++k2;
+        }
+      }
+      sort_pairs(k2, idx, val);
+      const int64_t before = nb;
+      int64_t k = 0;
+      while (k < k2 && !bad) {
+        const int64_t r = idx[k];
+        cplx sum = 0.0;
+        while (k < k2 && idx[k] == r) {
+          sum = sum + val[k];
+          ++k;
+        }
+        if (cabs(sum) < tol) continue; /* choptol! */
+        if (nb == capb) {
+          capb *= 2;
+          int64_t* rb2 = (int64_t*)realloc(rb, sizeof(int64_t) * (size_t)capb);
+          if (rb2) rb = rb2;
+          cplx* vb2 = (cplx*)realloc(vb, sizeof(cplx) * (size_t)capb);
+          if (vb2) vb = vb2;
+          if (!rb2 || !vb2) {
+            bad = 1;
+            break;
+          }
+        }
+        rb[nb] = r;
+        vb[nb] = sum;
+        ++nb;
+      }
+      colptr[c + 1] = nb - before; /* column size, summed below */
+    }
+    failed |= bad;
+    lrow[ib] = rb;
+    lval[ib] = vb;
+    lsize[ib] = nb;
+    free(idx);
+    free(val);
+  }
+  colptr[0] = 0;
+  for (int64_t c = 0; c < dim; ++c) colptr[c + 1] += colptr[c];
+  const int64_t nnz = colptr[dim];
+  int64_t* rowval = NULL;
+  double *nr = NULL, *ni = NULL;
+  if (!failed) {
+    rowval = (int64_t*)malloc(sizeof(int64_t) * (size_t)(nnz > 0 ? nnz : 1));
+    nr = (double*)malloc(sizeof(double) * (size_t)(nnz > 0 ? nnz : 1));
+    ni = (double*)malloc(sizeof(double) * (size_t)(nnz > 0 ? nnz : 1));
+    failed = !rowval || !nr || !ni;
+  }
+  if (!failed) {
+    int64_t off = 0;
+    for (int ib = 0; ib < nblocks; ++ib) {
+      for (int64_t k = 0; k < lsize[ib]; ++k) {
+        rowval[off + k] = lrow[ib][k];
+        nr[off + k] = creal(lval[ib][k]);
+        ni[off + k] = cimag(lval[ib][k]);
+      }
+      off += lsize[ib];
+    }
+  }
+  for (int ib = 0; ib < nblocks; ++ib) {
+    free(lrow[ib]);
+    free(lval[ib]);
+  }
+  free(lrow);
+  free(lval);
+  free(lsize);
+  if (failed) {
+    free(colptr);
+    free(rowval);
+    free(nr);
+    free(ni);
+    return -1;
+  }
+  *colptr_out = colptr;
+  *rowval_out = rowval;
+  *nz_re_out = nr;
+  *nz_im_out = ni;
+  return nnz;
+}
+
+void orc_free(void* p) { free(p); }
+
 /* get_element(opr, irow, icol).  Plain: operator_representation.jl:109-119
  * (the SumOperator element, operator_iterator.jl:66-83).  Reduced:
  * reduced_operator_representation.jl:120-138 (sum over the column iterator). */

@@ -36,7 +36,9 @@ STRATS = [("terms", "search"), ("terms", "bitmap"), ("terms", "xspread"), ("pair
 
 
 @pytest.mark.parametrize("Lx,Ly,k", [(4, 4, (2, 0)), (4, 4, (1, 2)), (4, 6, (2, 0)), (4, 6, (1, 3)), (6, 4, (3, 0)),
-                                     (6, 4, (1, 2)), (3, 4, (0, 1)), (5, 4, (2, 1))])
+                                     (6, 4, (1, 2)), (3, 4, (0, 1)), (5, 4, (2, 1)),
+                                     # k = (0, 0): the unit-phase translation-only kernels
+                                     (4, 4, (0, 0)), (4, 6, (0, 0)), (6, 4, (0, 0))])
 @pytest.mark.parametrize("dtype", [np.complex128, np.float64, np.float32, np.complex64])
 @pytest.mark.parametrize("transpose", [False, True])
 def test_strategies_vs_oracle(ed, oracle, Lx, Ly, k, dtype, transpose):

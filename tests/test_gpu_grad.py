@@ -204,3 +204,28 @@ def test_term_expectations_reduced_vs_oracle(ed, oracle, k, dtype, lookup):
         o = oracle.Opr(red, (mask[t:t + 1], row[t:t + 1], col[t:t + 1], np.ones(1, complex)))
         ref = np.vdot(phi_c, o.matvec(psi_c))
         assert abs(got[t] - ref) <= tol * scale, (t, got[t], ref)
+
+
+def test_couplings_on_reduced_space(ed):
+    """On a translation sector, per-bond couplings are rejected (their reduced
+    expectations would not be the physical ones); translation-invariant sums
+    pass, and their Hellmann-Feynman gradient obeys E0 = sum_k c_k dE0/dc_k
+    and matches the unreduced Sz sector's at the same couplings (the ring's
+    ground state has k = 0)."""
+    from edhip.grad import LinearCouplings, energy_and_grad, xxz_model
+
+    L = 12
+    hs, s = ed.spin_half_system(L)
+    bonds = [(i, (i + 1) % L) for i in range(L)]
+    hsr = ed.represent(ed.HilbertSpaceSector(hs, 0))
+    rh = ed.symmetry_reduce(hsr, ed.TranslationIrrep(L, 0))
+    with pytest.raises(ed.ArgumentError):
+        xxz_model(hs, rh, bonds)
+    K = ed.simplify(sum((0.25 * (s(i, "x") * s(j, "x") + s(i, "y") * s(j, "y")) for i, j in bonds), ed.NullOperator()))
+    Z = ed.simplify(sum((0.25 * s(i, "z") * s(j, "z") for i, j in bonds), ed.NullOperator()))
+    c = np.array([1.0, 0.7])
+    E_r, g_r = energy_and_grad(LinearCouplings(rh, [K, Z]), c, np.eye(2), tol=1e-13)
+    E_u, g_u = energy_and_grad(LinearCouplings(hsr, [K, Z]), c, np.eye(2), tol=1e-13)
+    assert abs(np.dot(c, g_r) - E_r) < 1e-11 * abs(E_r)
+    assert abs(E_r - E_u) < 1e-11 * abs(E_u)
+    assert np.max(np.abs(g_r - g_u)) < 1e-8

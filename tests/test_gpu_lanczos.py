@@ -142,3 +142,52 @@ def test_restart_in_row_chunks_space_group_6x6(ed):
     res = ed.eigs(opr, 1, tol=1e-10, max_basis=12, max_restarts=400, criterion="eigenvalue")
     assert res.converged and res.restarts > 0
     assert abs(res.values[0] - (-18.13714754)) < 1e-8
+
+
+def _zz_chain(ed, L):
+    hs, s = ed.spin_half_system(L)
+    zz = ed.simplify(sum((s(i, "z") * s((i + 1) % L, "z") for i in range(L)), ed.NullOperator()))
+    return hs, zz
+
+
+@pytest.mark.parametrize("mode", ["fp64", "mixed"])
+@pytest.mark.parametrize("max_basis", [12, 120])
+def test_invariant_subspace_degenerate_levels(ed, mode, max_basis):
+    """A diagonal operator: the Krylov space of one start vector holds one copy
+    of each distinct eigenvalue (4 here), so the lowest three levels with
+    their multiplicities (-8, -8, -4 for sum_i sigma^z_i sigma^z_{i+1} on the
+    8-site ring, Sz = 0: the two Neel states first) need random directions
+    after each invariant subspace; the eigenvalues are the sorted diagonal.
+    max_basis 12 also restarts; 120 exceeds the dimension (70), where the
+    basis spans the whole space."""
+    L = 8
+    hs, zz = _zz_chain(ed, L)
+    hsr = ed.represent(ed.HilbertSpaceSector(hs, 0))
+    opr = ed.represent(hsr, zz)
+    words = hsr.basis_list
+    diag = np.zeros(len(words))
+    for i in range(L):
+        bi = ((words >> np.uint64(i)) & np.uint64(1)).astype(np.int64)
+        bj = ((words >> np.uint64((i + 1) % L)) & np.uint64(1)).astype(np.int64)
+        diag += 1 - 2 * (bi ^ bj)
+    ref = np.sort(diag)[:3]
+    assert ref.tolist() == [-8.0, -8.0, -4.0]
+    r = ed.eigs(opr, 3, tol=1e-12, mode=mode, max_basis=max_basis, max_restarts=400, return_vectors=True)
+    assert r.converged
+    assert np.all(np.abs(r.values - ref) < 1e-11), r.values
+    V = r.vectors.cpu().numpy()
+    assert np.abs(V @ V.T - np.eye(3)).max() < 1e-9
+    for i in range(3):
+        assert np.linalg.norm(diag * V[i] - r.values[i] * V[i]) < 1e-8
+
+
+@pytest.mark.parametrize("mode", ["fp64", "mixed"])
+def test_invariant_subspace_zero_operator(ed, mode):
+    """sum_i sigma^z_i vanishes on the Sz = 0 sector: every Krylov block has one
+    vector; four zero eigenvalues come back converged."""
+    L = 10
+    hs, s = ed.spin_half_system(L)
+    op = sum((s(i, "z") for i in range(L)), ed.NullOperator())
+    opr = ed.represent(ed.represent(ed.HilbertSpaceSector(hs, 0)), op)
+    r = ed.eigs(opr, 4, tol=1e-12, mode=mode, max_basis=20, max_restarts=50)
+    assert r.converged and np.all(np.abs(r.values) < 1e-12)

@@ -187,6 +187,29 @@ def test_device_tensors_stream_handoff(ed):
         assert abs(s - ref) < 1e-9 * max(1.0, abs(ref))
 
 
+def test_device_tensors_side_stream(ed, systems):
+    """The same hand-off on a caller's non-default torch stream (the library
+    adopts it as its own queue), then back on the default stream: products
+    chained through torch kernels without host synchronisation agree with
+    the host path."""
+    import torch
+
+    opr, _ = systems["chain16_k3"]
+    rng = np.random.default_rng(4)
+    x = rng.standard_normal(opr.dimension) + 1j * rng.standard_normal(opr.dimension)
+    ref = opr @ (opr @ (2.0 * x))
+    side = torch.cuda.Stream()
+    for stream in (side, torch.cuda.current_stream()):
+        with torch.cuda.stream(stream):
+            xt = torch.from_numpy(x).to("cuda", non_blocking=False) * 2.0
+            y1 = torch.zeros_like(xt)
+            ed.mul_(y1, opr, xt)
+            y2 = torch.zeros_like(xt)
+            ed.mul_(y2, opr, y1 * 1.0)
+            out = y2.cpu().numpy()
+        assert rel(out, ref) < 1e-13
+
+
 def test_hermiticity_and_linearity(ed, systems):
     for name in ("chain16_k3", "sq4x4_pi0", "dihedral14", "xxz14"):
         opr, _ = systems[name]

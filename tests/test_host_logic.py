@@ -200,3 +200,80 @@ def test_global_bit_flip_known(ed):
     irr = ed.make_product_irrep([(b0, 1.0), (b1, -1.0)], ed.TranslationIrrep(4, 1))
     assert len(irr) == 8 and irr[0][1] == 1.0
     assert len({ed.symmetry_apply(hs, op, 0b0011)[0] for op, _ in irr}) == 4
+
+
+# ------------------------------------------------ space-group characters
+@pytest.mark.parametrize("L,k", [(4, (0, 0)), (4, (2, 2)), (6, (0, 0)), (6, (3, 3))])
+def test_square_space_irrep_characters(ed, L, k):
+    """SquareSpaceIrrep against an independent construction (conftest: C4v
+    generated by the quarter turn and the axial mirror, characters from the
+    generators' images): the same elements with the same characters, every
+    irrep a homomorphism, and the four irreps orthonormal (class
+    orthogonality: sum_g chi_a(g) conj(chi_b(g)) = |G| delta_ab)."""
+    from conftest import space_group_independent
+
+    hs, _ = ed.spin_half_system(L * L)
+    names = ["A1", "A2", "B1", "B2"]
+    chis = {}
+    for name in names:
+        ops = ed.SquareSpaceIrrep(L, k, name).symops_and_amplitudes()
+        perms = np.stack([np.array(p.map) - 1 for p, _ in ops])
+        chi = np.array([complex(a) for _, a in ops])
+        ref_p, ref_chi = space_group_independent(L, k, name)
+        ref = {tuple(p): c for p, c in zip(ref_p.tolist(), ref_chi)}
+        assert len(ref) == len(ops) == 8 * L * L
+        assert np.array_equal(perms[0], np.arange(L * L))
+        for p, c in zip(perms.tolist(), chi):
+            assert abs(ref[tuple(p)] - c) < 1e-12, name
+        index = {tuple(p): i for i, p in enumerate(perms.tolist())}
+        rng = np.random.default_rng(5)
+        for _ in range(200):
+            a, b = rng.integers(0, len(ops), 2)
+            ab = tuple(perms[a][perms[b]])  # g_a after g_b
+            assert abs(chi[index[ab]] - chi[a] * chi[b]) < 1e-12
+        chis[name] = chi
+    G = 8 * L * L
+    for a in names:
+        for b in names:
+            s = np.vdot(chis[b], chis[a])
+            assert abs(s - (G if a == b else 0)) < 1e-9
+
+
+# ------------------------------------------------ coupling invariance
+def _fake_reduced(ed, hs, ops):
+    """A ReducedHilbertSpaceRepresentation shell carrying only its group (the
+    check below reads nothing else); no device call."""
+    r = object.__new__(ed.ReducedHilbertSpaceRepresentation)
+    r.hilbert_space = hs
+    r.symops_and_amplitudes = list(ops)
+    r._handle = None
+    return r
+
+
+def test_linear_couplings_invariance_check(ed):
+    """On a symmetry-reduced space a coupling operator must commute with every
+    group element: per-bond operators are rejected, their symmetric sums pass
+    (the reduced element formula applies an operator to the representative
+    of each row only)."""
+    from edhip.grad import _check_invariant
+
+    L = 8
+    hs, s = ed.spin_half_system(L)
+    bonds = [(i, (i + 1) % L) for i in range(L)]
+    per_bond = [ed.heisenberg_bond(hs, i, j) for i, j in bonds]
+    total = ed.simplify(sum(per_bond, ed.NullOperator()))
+    zz = ed.simplify(sum((s(i, "z") * s(j, "z") for i, j in bonds), ed.NullOperator()))
+    field = ed.simplify(sum((s(i, "z") for i in range(L)), ed.NullOperator()))
+    trans = ed.TranslationIrrep(L, 1).symops_and_amplitudes()
+    space = _fake_reduced(ed, hs, trans)
+    _check_invariant(space, [total, zz, field])
+    with pytest.raises(ed.ArgumentError):
+        _check_invariant(space, per_bond)
+    flip = ed.make_product_irrep([(ed.GlobalBitFlip(False), 1.0), (ed.GlobalBitFlip(True), 1.0)],
+                                 ed.TranslationIrrep(L, 0))
+    fspace = _fake_reduced(ed, hs, flip)
+    _check_invariant(fspace, [total, zz])
+    with pytest.raises(ed.ArgumentError):
+        _check_invariant(fspace, [field])  # sum_i S^z_i changes sign under the flip
+    # an unreduced space takes any operator (no group)
+    _check_invariant(object(), per_bond)

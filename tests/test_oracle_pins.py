@@ -296,6 +296,31 @@ def test_sparse_chop_after_merge(oracle):
     assert np.allclose(nz, [2e-7])
 
 
+@pytest.mark.parametrize("reduced", [False, True])
+@pytest.mark.parametrize("threads", [1, 3, 8])
+def test_sparse_parallel_equals_serial(oracle, reduced, threads):
+    """sparse_parallel (abstract_operator_representation.jl:172-204): the
+    column blocks built by each thread and concatenated in block order give
+    exactly sparse_serial's CSC (same merge order within a column), for any
+    thread count, including the chop-after-merge case."""
+    L = 12
+    parent = oracle.sector_basis(L, 0)
+    terms = heisenberg_terms_pauli(chain_bonds(L) + [(i, (i + 3) % L) for i in range(L)],
+                                   weights=np.linspace(0.5, 1.5, 2 * L))
+    space = parent
+    if reduced:
+        space = oracle.symmetry_reduce(parent, L, translation_group_perms(L), translation_chars(L, 1, 5, 0))
+    o = oracle.Opr(space, terms)
+    a = o.sparse()
+    b = o.sparse_parallel(n_threads=threads)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2])
+    basis = np.array([0, 1], np.uint64)
+    chop = (np.array([1, 1, 1, 1], np.uint64), np.array([0, 0, 1, 1], np.uint64),
+            np.array([1, 1, 0, 0], np.uint64), np.array([1.0, -1.0 + 1e-9, 1e-7, 1e-7 + 0j]))
+    c = oracle.Opr(basis, chop).sparse_parallel(n_threads=threads)
+    assert c[0].tolist() == [0, 1, 1] and c[1].tolist() == [1]
+
+
 # ---------------------------------------------------------------- eigenvalues (A12)
 def test_eigenvalues_brute_force(oracle, golden):
     ev = golden["eigenvalues_brute_force"]
