@@ -855,6 +855,10 @@ struct edhip_op {
   DevBuf off_row, off_col, diag, all_row, all_col;
   DevBuf mv_row, mv_col;  // paired copies of off_row / off_col for the matvec
   int n_mv = 0;
+  // incremental 6x6 canonical form: one descriptor per paired term (the same
+  // for both forms: the flip match ^ image is the same word)
+  DevBuf inc_desc;
+  bool inc_ok = false;
   int n_off = 0, n_diag = 0;
   std::vector<int64_t> diag_ids, off_ids;  // original term index of each device term
   OpDev dev_row{}, dev_col{};
@@ -872,7 +876,7 @@ struct edhip_op {
   // per-row diagonal elements for the term-list kernel (built on first use)
   bool diag_tried = false;
   DevBuf diag_re, diag_im;
-  int strat_kernel = 0;  // 0 auto, 1 term list, 2 pair groups
+  int strat_kernel = 0;  // 0 auto, 1 term list, 2 pair groups, 3 term list + incremental 6x6 canonical form
   int strat_lookup = 0;  // 0 auto, 1 sorted search (or rank), 2 hash table
   int used_kernel = 0, used_lookup = 0;
   double kt_ms = 0.0;  // summed durations of the timed matvec kernel launches
@@ -955,6 +959,14 @@ OpDev make_opdev(edhip_op* op, bool transpose) {
   if (d.lds_total > 64 * 1024)
     fail(EDHIP_ERR_ARGUMENT, "operator and symmetry tables exceed the 64 KiB LDS budget of the matvec kernels (" +
                                  std::to_string(d.lds_total) + " bytes)");
+  // the incremental 6x6 canonical form (translations alone, characters 1,
+  // flips within two adjacent rows), with its descriptors after the code table
+  d.inc_desc = nullptr;
+  const GroupDev& g = d.g;
+  if (op->inc_ok && d.n_mv && g.mode == kGroupTorus && g.pair_table && g.n_cosets == 1 && g.Lx == 6 && g.Ly == 6 &&
+      g.nb == 36 && g.unit_phases && d.codes_lds &&
+      align_up(d.codes_lds + (2u << kXTopBits), 16) + 4u * (uint32_t)d.n_mv <= 64 * 1024)
+    d.inc_desc = op->inc_desc.as<uint32_t>();
   return d;
 }
 
@@ -1138,7 +1150,10 @@ void op_matvec(edhip_op* op, int prec, int transpose, int accumulate, const void
       d.diag_re = op->diag_re.as<double>();
       d.diag_im = op->diag_im.as<double>();
     }
-    op->used_kernel = 1;
+    // the incremental canonical form where it applies, unless the plain
+    // term-list kernel is asked for
+    if (op->strat_kernel == 1) d.inc_desc = nullptr;
+    op->used_kernel = d.inc_desc ? 3 : 1;
     op->used_lookup = rank_mode ? 1 : (d.ix.bm_bits ? 3 : 1);
     if (attach_spread(op, prec, d, x, st)) op->used_lookup = 4;
     if (idx_lookup == 4 && !d.xm.values) fail(EDHIP_ERR_ARGUMENT, "x cannot be spread over the words of this space");
@@ -2138,6 +2153,17 @@ int edhip_op_create(edhip_ctx* ctx, const edhip_space* space, int64_t n_terms, c
     };
     const std::vector<OffTerm> mvr = pad_runs(offr), mvc = pad_runs(offc);
     op->n_mv = (int)mvr.size();
+    {
+      std::vector<uint32_t> dsc(mvr.size());
+      bool ok = !mvr.empty();
+      for (size_t t = 0; t < mvr.size(); ++t) {
+        const uint64_t flip = (mvr[t].match ^ mvr[t].image) & mvr[t].mask;
+        dsc[t] = inc6_descriptor(flip);
+        if (dsc[t] == 0xffffffffu) ok = false;
+      }
+      op->inc_ok = ok;
+      if (ok) upload(op->inc_desc, dsc, ctx->stream);
+    }
     upload(op->mv_row, mvr, ctx->stream);
     upload(op->mv_col, mvc, ctx->stream);
     upload(op->diag, dg, ctx->stream);
@@ -2202,10 +2228,12 @@ int edhip_apply_host(edhip_op* op, int prec, int transpose, int accumulate, int6
 }
 
 int edhip_op_set_strategy(edhip_op* op, int kernel, int lookup) {
-  if (!op || kernel < 0 || kernel > 2 || lookup < 0 || lookup > 4) return EDHIP_ERR_ARGUMENT;
+  if (!op || kernel < 0 || kernel > 3 || lookup < 0 || lookup > 4) return EDHIP_ERR_ARGUMENT;
   return guarded(op->ctx, [&] {
     if (kernel == 2 && !fast_path_usable(op))
       fail(EDHIP_ERR_ARGUMENT, "the pair-group kernel does not cover this operator or symmetry group");
+    if (kernel == 3 && !op->dev_row.inc_desc)
+      fail(EDHIP_ERR_ARGUMENT, "the incremental 6x6 canonical form does not cover this operator or symmetry group");
     if (lookup == 3) (void)op->space->index_dev(3);  // raises when no bitmap index exists
     op->strat_kernel = kernel;
     op->strat_lookup = lookup;
@@ -2426,11 +2454,14 @@ int edhip_lanczos(edhip_op* op, const edhip_lanczos_opts* opts, double* evals, v
     L.scale(L.vec(0), 1.0 / nrm);
 
     // The FP32 phase stops on the eigenvalue criterion (its residual cannot go
-    // below single-precision rounding) at tol_fp32 (default max(tol, 1e-8));
-    // the FP64 phase uses the caller's rule.
+    // below single-precision rounding) at max(tol, tol_fp32), tol_fp32 = 1e-10
+    // by default: converged that far, the FP32 Ritz vector needs one FP64
+    // product (its Rayleigh quotient and residual) to settle the eigenvalue
+    // (6x6 J1-J2: 65 FP32 + 1 FP64 products, against 57 + 10 when the FP32
+    // phase stops at 1e-8).  The FP64 phase uses the caller's rule.
     L.criterion = mixed ? 1 : opts->criterion;
     if (opts->tol_fp32 < 0.0) fail(EDHIP_ERR_ARGUMENT, "tol_fp32 must be >= 0");
-    const double tol_main = mixed ? std::max(opts->tol, opts->tol_fp32 > 0.0 ? opts->tol_fp32 : 1e-8) : opts->tol;
+    const double tol_main = mixed ? std::max(opts->tol, opts->tol_fp32 > 0.0 ? opts->tol_fp32 : 1e-10) : opts->tol;
     // Large bases check convergence at every step: the host eigensolve of the
     // small projected matrix costs far less than the product it may save.
     const bool large = n > (int64_t)1 << 22;

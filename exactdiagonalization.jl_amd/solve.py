@@ -44,7 +44,7 @@ def eigs(opr, nev: int = 1, which: str = "SR", *, mode: str = "fp64", tol: float
 
     mode="fp64": FP64 (ComplexF64) vectors.  mode="mixed": FP32 (ComplexF32)
     Krylov storage and products, FP64 reductions, run to the eigenvalue
-    criterion at max(tol, tol_fp32) (tol_fp32=0: max(tol, 1e-8)); then per
+    criterion at max(tol, tol_fp32) (tol_fp32=0: max(tol, 1e-10)); then per
     eigenvalue an FP64 residual correction started from the FP32 Ritz vector
     (FP64 Rayleigh quotient and residual; more FP64 Lanczos steps only when
     the residual bound does not hold yet).  Start vector: SplitMix64(seed) +
