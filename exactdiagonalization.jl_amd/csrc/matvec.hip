@@ -314,7 +314,7 @@ __device__ __forceinline__ bool inc6_canon(const GroupLds& L, uint64_t m3lo, uin
   const uint32_t mo = ((M >> 6) & 1u) ? (1u << (M & 7u)) : 3u;
   const uint32_t yw = k0 == best ? (dsc >> 22) & 7u : k1 == best ? (dsc >> 25) & 7u : k2 == best ? (dsc >> 28) & 7u : (M >> 3) & 7u;
   const uint32_t am = k0 == best ? e0 & 0xffu : k1 == best ? e1 & 0xffu : k2 == best ? e2 & 0xffu : mo;
-  const int a = (int)(__builtin_ctz(am | 0x40u) & 7u);
+  const int a = (int)(__builtin_ctz(am | 0x20u));
   const int sh = (5 - (int)yw) * 6;  // row yw moves to the top
   const uint64_t t = ((bp << sh) | (bp >> (36 - sh))) & 0xfffffffffull;
   s = ((t << a) & L.rot_hi[a]) | ((t >> (6 - a)) & L.rot_lo[a]);
@@ -367,7 +367,7 @@ __global__ __launch_bounds__(kMvBlock) void k_matvec_inc6(OpDev op, uint32_t inc
     const uint32_t ys = ka == mn ? (uint32_t)((Y + 3) % 6) : (kb == mn ? (uint32_t)((Y + 4) % 6) : (uint32_t)((Y + 5) % 6));
     const uint32_t am = e & 0xffu;
     const uint32_t uq = (nt == 1u && (am & (am - 1u)) == 0u) ? 1u : 0u;
-    const uint32_t as = __builtin_ctz(am | 0x40u) & 7u;
+    const uint32_t as = __builtin_ctz(am | 0x20u);
     const uint64_t packed = ((uint64_t)(mn >> 8) << 7) | (uq << 6) | (ys << 3) | as;
     m3[Y / 3] |= packed << (19 * (Y % 3));
   }
