@@ -160,7 +160,7 @@ def correlator_and_grad(model: LinearCouplings, c, dc_dtheta, observable_op, *, 
     lam, iters, res = solve.minres(H, rhs, shift=E0, deflate=psi, tol=minres_tol)
     ex = model.expectations(lam, psi)
     grad = -2.0 * (np.asarray(dc_dtheta).T @ ex.real)
-    return expO, grad, {"minres_iterations": iters, "minres_residual": res, "E0": E0}
+    return expO, grad, {"minres_iterations": iters, "minres_residual": res, "E0": E0, "psi": psi, "lambda": lam}
 
 
 def _torch_function():

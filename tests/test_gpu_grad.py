@@ -66,7 +66,7 @@ def test_energy_gradient_cfg5(ed, oracle):
     # CPU reference on a sample of bonds
     psi_h = psi.cpu().numpy().astype(np.complex128)
     parent = oracle.sector_basis(24, 0)
-    for b in (0, 7, 19, 40):
+    for b in (0, 5, 7, 19, 26, 33, 40, 47):  # 16 of the 96 components
         K = oracle.Opr(parent, ed.terms_arrays(model.operators[b]))
         Z = oracle.Opr(parent, ed.terms_arrays(model.operators[B + b]))
         kb = np.vdot(psi_h, K.matvec(psi_h)).real
@@ -106,6 +106,45 @@ def test_correlator_gradient_minres_vs_fd(ed):
     assert abs(corr(theta) - val) < 1e-10
     h = 1e-5
     for p in (0, 5, 24 + 2, 24 + 17):
+        tp, tm = theta.copy(), theta.copy()
+        tp[p] += h
+        tm[p] -= h
+        fd = (corr(tp) - corr(tm)) / (2 * h)
+        assert abs(fd - g[p]) < 1e-6, (p, fd, g[p])
+
+
+def test_correlator_gradient_cfg5(ed, oracle):
+    """4x6 random-bond XXZ (2,704,156 states): d<S^z_0 S^z_12>/dtheta through
+    the MINRES adjoint against an FP64 CPU reference -2 Re <lambda|O_k|psi>
+    (oracle apply of the bond operators on the device's lambda and psi) on
+    sampled bonds, and against central FD (h = 1e-5) on two parameters."""
+    from edhip.grad import correlator_and_grad, ground_state
+
+    hs, s, hsr, bonds, model, fn, theta = _xxz(ed, 4, 6, 2024)
+    B = len(bonds)
+    O = ed.simplify(0.25 * s(0, "z") * s(12, "z"))
+    c, J = fn(theta)
+    val, g, info = correlator_and_grad(model, c, J, O, tol=1e-13, minres_tol=1e-12)
+    assert info["minres_residual"] < 1e-10
+    psi = info["psi"].cpu().numpy().astype(np.complex128)
+    lam = info["lambda"].cpu().numpy().astype(np.complex128)
+    parent = oracle.sector_basis(24, 0)
+    for b in (2, 17, 30, 45):
+        kb = np.vdot(lam, oracle.Opr(parent, ed.terms_arrays(model.operators[b])).matvec(psi)).real
+        zb = np.vdot(lam, oracle.Opr(parent, ed.terms_arrays(model.operators[B + b])).matvec(psi)).real
+        assert abs(g[b] - (-2.0 * (kb + theta[B + b] * zb))) < 1e-10
+        assert abs(g[B + b] - (-2.0 * theta[b] * zb)) < 1e-10
+    Oop = ed.represent(hsr, O)
+
+    def corr(th):
+        cc, _ = fn(th)
+        _, rr = ground_state(model, cc, tol=1e-13)
+        v = rr.vectors[0].cpu().numpy()
+        return float(np.vdot(v, Oop @ v).real)
+
+    assert abs(corr(theta) - val) < 1e-10
+    h = 1e-5
+    for p in (4, B + 23):
         tp, tm = theta.copy(), theta.copy()
         tp[p] += h
         tm[p] -= h
