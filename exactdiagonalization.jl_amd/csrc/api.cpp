@@ -498,24 +498,15 @@ void build_group(GroupHost& gh, int nb, double tol, hipStream_t st) {
     if (gh.xr[g] != 0 && gh.xr[g] != full) fail(EDHIP_ERR_ARGUMENT, "a bit flip must flip every bit (GlobalBitFlip)");
     if (gh.xr[g]) gh.has_xor = true;
   }
-  // closure and inverses; an element is (permutation, mask):
-  // (p1, x1)(p2, x2) b = p1(p2(b) ^ x2) ^ x1 = p1(p2(b)) ^ (p1(x2) ^ x1), and p(full) = full
+  // closure and inverses (host_logic.hpp)
+  const std::string why = group_inverses(G, nb, gh.perm, gh.xr, gh.inv);
+  if (!why.empty()) fail(EDHIP_ERR_ARGUMENT, why);
+  // element index by (permutation, XOR word), for the coset decomposition
   std::map<std::pair<std::vector<int32_t>, uint64_t>, int> index;
-  for (int g = 0; g < G; ++g) {
-    std::vector<int32_t> p(gh.perm.begin() + (size_t)g * nb, gh.perm.begin() + (size_t)(g + 1) * nb);
-    auto key = std::make_pair(p, gh.xr[g]);
-    if (index.count(key)) fail(EDHIP_ERR_ARGUMENT, "repeated symmetry operation");
-    index[key] = g;
-  }
-  gh.inv.assign(G, -1);
+  for (int g = 0; g < G; ++g)
+    index[std::make_pair(std::vector<int32_t>(gh.perm.begin() + (size_t)g * nb, gh.perm.begin() + (size_t)(g + 1) * nb),
+                         gh.xr[g])] = g;
   std::vector<int32_t> comp(nb);
-  for (int g1 = 0; g1 < G; ++g1)
-    for (int g2 = 0; g2 < G; ++g2) {
-      for (int i = 0; i < nb; ++i) comp[i] = gh.perm[(size_t)g1 * nb + gh.perm[(size_t)g2 * nb + i]];
-      auto it = index.find(std::make_pair(comp, gh.xr[g1] ^ gh.xr[g2]));
-      if (it == index.end()) fail(EDHIP_ERR_ARGUMENT, "symmetry operations must form a group");
-      if (it->second == 0) gh.inv[g1] = g2;
-    }
   // canonicaliser choice: if every translation of an Lx x Ly torus (no flip)
   // is an element, the group is a union of cosets T p and canonicalises as
   // min over p of the translation-canonical form of p(b); else generic tables.
@@ -855,10 +846,6 @@ struct edhip_op {
   DevBuf off_row, off_col, diag, all_row, all_col;
   DevBuf mv_row, mv_col;  // paired copies of off_row / off_col for the matvec
   int n_mv = 0;
-  // incremental 6x6 canonical form: one descriptor per paired term (the same
-  // for both forms: the flip match ^ image is the same word)
-  DevBuf inc_desc;
-  bool inc_ok = false;
   int n_off = 0, n_diag = 0;
   std::vector<int64_t> diag_ids, off_ids;  // original term index of each device term
   OpDev dev_row{}, dev_col{};
@@ -876,7 +863,7 @@ struct edhip_op {
   // per-row diagonal elements for the term-list kernel (built on first use)
   bool diag_tried = false;
   DevBuf diag_re, diag_im;
-  int strat_kernel = 0;  // 0 auto, 1 term list, 2 pair groups, 3 term list + incremental 6x6 canonical form
+  int strat_kernel = 0;  // 0 auto, 1 term list, 2 pair groups
   int strat_lookup = 0;  // 0 auto, 1 sorted search (or rank), 2 hash table
   int used_kernel = 0, used_lookup = 0;
   double kt_ms = 0.0;  // summed durations of the timed matvec kernel launches
@@ -959,14 +946,6 @@ OpDev make_opdev(edhip_op* op, bool transpose) {
   if (d.lds_total > 64 * 1024)
     fail(EDHIP_ERR_ARGUMENT, "operator and symmetry tables exceed the 64 KiB LDS budget of the matvec kernels (" +
                                  std::to_string(d.lds_total) + " bytes)");
-  // the incremental 6x6 canonical form (translations alone, characters 1,
-  // flips within two adjacent rows), with its descriptors after the code table
-  d.inc_desc = nullptr;
-  const GroupDev& g = d.g;
-  if (op->inc_ok && d.n_mv && g.mode == kGroupTorus && g.pair_table && g.n_cosets == 1 && g.Lx == 6 && g.Ly == 6 &&
-      g.nb == 36 && g.unit_phases && d.codes_lds &&
-      align_up(d.codes_lds + (2u << kXTopBits), 16) + 4u * (uint32_t)d.n_mv <= 64 * 1024)
-    d.inc_desc = op->inc_desc.as<uint32_t>();
   return d;
 }
 
@@ -1150,10 +1129,7 @@ void op_matvec(edhip_op* op, int prec, int transpose, int accumulate, const void
       d.diag_re = op->diag_re.as<double>();
       d.diag_im = op->diag_im.as<double>();
     }
-    // the incremental canonical form where it applies, unless the plain
-    // term-list kernel is asked for
-    if (op->strat_kernel == 1) d.inc_desc = nullptr;
-    op->used_kernel = d.inc_desc ? 3 : 1;
+    op->used_kernel = 1;
     op->used_lookup = rank_mode ? 1 : (d.ix.bm_bits ? 3 : 1);
     if (attach_spread(op, prec, d, x, st)) op->used_lookup = 4;
     if (idx_lookup == 4 && !d.xm.values) fail(EDHIP_ERR_ARGUMENT, "x cannot be spread over the words of this space");
@@ -1194,120 +1170,6 @@ void op_matvec(edhip_op* op, int prec, int transpose, int accumulate, const void
 // Lanczos (thick restart, full reorthogonalisation) and its host linear algebra
 // ============================================================================
 namespace {
-
-// Householder reduction of a dense symmetric matrix (row-major, n x n) to
-// tridiagonal form A = Q T Q^T; returns d (diagonal), e (e[i] = T[i][i+1]).
-void householder_tridiag(int n, std::vector<double>& A, std::vector<double>& Q, std::vector<double>& d,
-                         std::vector<double>& e) {
-  Q.assign((size_t)n * n, 0.0);
-  for (int i = 0; i < n; ++i) Q[(size_t)i * n + i] = 1.0;
-  std::vector<double> v(n), p(n), w(n);
-  for (int k = 0; k + 2 < n; ++k) {
-    double nx = 0.0;
-    for (int i = k + 1; i < n; ++i) nx += A[(size_t)i * n + k] * A[(size_t)i * n + k];
-    nx = std::sqrt(nx);
-    if (nx == 0.0) continue;
-    double x0 = A[(size_t)(k + 1) * n + k];
-    double alpha = (x0 >= 0) ? -nx : nx;
-    for (int i = 0; i < n; ++i) v[i] = 0.0;
-    for (int i = k + 1; i < n; ++i) v[i] = A[(size_t)i * n + k];
-    v[k + 1] -= alpha;
-    double vn = 0.0;
-    for (int i = k + 1; i < n; ++i) vn += v[i] * v[i];
-    vn = std::sqrt(vn);
-    if (vn == 0.0) continue;
-    for (int i = k + 1; i < n; ++i) v[i] /= vn;
-    // p = A v ; K = v^T p ; w = p - K v ; A -= 2 (v w^T + w v^T)
-    for (int i = 0; i < n; ++i) {
-      double s = 0.0;
-      for (int j = k + 1; j < n; ++j) s += A[(size_t)i * n + j] * v[j];
-      p[i] = s;
-    }
-    double K = 0.0;
-    for (int i = k + 1; i < n; ++i) K += v[i] * p[i];
-    for (int i = 0; i < n; ++i) w[i] = p[i] - K * v[i];
-    for (int i = 0; i < n; ++i)
-      for (int j = 0; j < n; ++j) A[(size_t)i * n + j] -= 2.0 * (v[i] * w[j] + w[i] * v[j]);
-    // Q = Q H
-    for (int i = 0; i < n; ++i) {
-      double s = 0.0;
-      for (int j = k + 1; j < n; ++j) s += Q[(size_t)i * n + j] * v[j];
-      for (int j = k + 1; j < n; ++j) Q[(size_t)i * n + j] -= 2.0 * s * v[j];
-    }
-  }
-  d.assign(n, 0.0);
-  e.assign(n, 0.0);
-  for (int i = 0; i < n; ++i) d[i] = A[(size_t)i * n + i];
-  for (int i = 0; i + 1 < n; ++i) e[i] = A[(size_t)i * n + i + 1];
-}
-
-// Implicit QL with Wilkinson-type shifts on a symmetric tridiagonal matrix
-// (d diagonal, e[i] coupling i and i+1): the published tql2 procedure of
-// EISPACK (Bowdler, Martin, Reinsch and Wilkinson, "The QR and QL
-// algorithms for symmetric matrices", Numer. Math. 11 (1968) 293-306).  Z (row-major n x n) accumulates the
-// rotations: on entry the basis the tridiagonal matrix is expressed in, on
-// exit the eigenvectors as columns.  Eigenvalues are sorted ascending.
-bool tridiag_ql(int n, std::vector<double>& d, std::vector<double>& e, std::vector<double>& Z) {
-  const double eps = 2.220446049250313e-16;
-  for (int l = 0; l < n; ++l) {
-    int iter = 0;
-    while (true) {
-      int m = l;
-      for (; m < n - 1; ++m) {
-        double dd = std::fabs(d[m]) + std::fabs(d[m + 1]);
-        if (std::fabs(e[m]) <= eps * dd) break;
-      }
-      if (m == l) break;
-      if (++iter > 200) return false;
-      double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
-      double r = std::hypot(g, 1.0);
-      g = d[m] - d[l] + e[l] / (g + (g >= 0 ? r : -r));
-      double s = 1.0, c = 1.0, p = 0.0;
-      int i = m - 1;
-      bool early = false;
-      for (; i >= l; --i) {
-        double f = s * e[i];
-        double b = c * e[i];
-        r = std::hypot(f, g);
-        e[i + 1] = r;
-        if (r == 0.0) {
-          d[i + 1] -= p;
-          e[m] = 0.0;
-          early = true;
-          break;
-        }
-        s = f / r;
-        c = g / r;
-        g = d[i + 1] - p;
-        r = (d[i] - g) * s + 2.0 * c * b;
-        p = s * r;
-        d[i + 1] = g + p;
-        g = c * r - b;
-        for (int k = 0; k < n; ++k) {
-          double zf = Z[(size_t)k * n + i + 1];
-          Z[(size_t)k * n + i + 1] = s * Z[(size_t)k * n + i] + c * zf;
-          Z[(size_t)k * n + i] = c * Z[(size_t)k * n + i] - s * zf;
-        }
-      }
-      if (early) continue;
-      d[l] -= p;
-      e[l] = g;
-      e[m] = 0.0;
-    }
-  }
-  // sort ascending
-  std::vector<int> ord(n);
-  for (int i = 0; i < n; ++i) ord[i] = i;
-  std::sort(ord.begin(), ord.end(), [&](int a, int b) { return d[a] < d[b]; });
-  std::vector<double> d2(n), Z2((size_t)n * n);
-  for (int j = 0; j < n; ++j) {
-    d2[j] = d[ord[j]];
-    for (int k = 0; k < n; ++k) Z2[(size_t)k * n + j] = Z[(size_t)k * n + ord[j]];
-  }
-  d = d2;
-  Z = Z2;
-  return true;
-}
 
 // Eigen-decomposition of the symmetric m x m matrix T (row-major).
 void sym_eig(int m, const std::vector<double>& T, bool tridiagonal, std::vector<double>& w, std::vector<double>& Z) {
@@ -2153,17 +2015,6 @@ int edhip_op_create(edhip_ctx* ctx, const edhip_space* space, int64_t n_terms, c
     };
     const std::vector<OffTerm> mvr = pad_runs(offr), mvc = pad_runs(offc);
     op->n_mv = (int)mvr.size();
-    {
-      std::vector<uint32_t> dsc(mvr.size());
-      bool ok = !mvr.empty();
-      for (size_t t = 0; t < mvr.size(); ++t) {
-        const uint64_t flip = (mvr[t].match ^ mvr[t].image) & mvr[t].mask;
-        dsc[t] = inc6_descriptor(flip);
-        if (dsc[t] == 0xffffffffu) ok = false;
-      }
-      op->inc_ok = ok;
-      if (ok) upload(op->inc_desc, dsc, ctx->stream);
-    }
     upload(op->mv_row, mvr, ctx->stream);
     upload(op->mv_col, mvc, ctx->stream);
     upload(op->diag, dg, ctx->stream);
@@ -2228,12 +2079,10 @@ int edhip_apply_host(edhip_op* op, int prec, int transpose, int accumulate, int6
 }
 
 int edhip_op_set_strategy(edhip_op* op, int kernel, int lookup) {
-  if (!op || kernel < 0 || kernel > 3 || lookup < 0 || lookup > 4) return EDHIP_ERR_ARGUMENT;
+  if (!op || kernel < 0 || kernel > 2 || lookup < 0 || lookup > 4) return EDHIP_ERR_ARGUMENT;
   return guarded(op->ctx, [&] {
     if (kernel == 2 && !fast_path_usable(op))
       fail(EDHIP_ERR_ARGUMENT, "the pair-group kernel does not cover this operator or symmetry group");
-    if (kernel == 3 && !op->dev_row.inc_desc)
-      fail(EDHIP_ERR_ARGUMENT, "the incremental 6x6 canonical form does not cover this operator or symmetry group");
     if (lookup == 3) (void)op->space->index_dev(3);  // raises when no bitmap index exists
     op->strat_kernel = kernel;
     op->strat_lookup = lookup;

@@ -5,6 +5,7 @@
 #include <stdint.h>
 
 #include "device.hpp"
+#include "host_logic.hpp"
 
 namespace edhip {
 
@@ -114,45 +115,7 @@ struct OpDev {
   XMapDev xm;            // matvec: x read from its spread over word positions (xm.values != null)
   uint32_t codes_lds;    // LDS offset of xm's code table, after lds_total (0: no room)
   uint32_t lds_total;    // LDS of the kernels (the matvec adds xm's code table)
-  // 6x6 torus translations with every character 1 and every flip within two
-  // adjacent rows: one descriptor per paired term (inc6_descriptor), which
-  // selects the incremental canonical form of the matvec; null otherwise
-  const uint32_t* inc_desc;
 };
-
-// Descriptor of a flip word for the incremental 6x6 canonical form (0xffffffff
-// when the flip touches rows other than one row Y or the adjacent rows Y, Y+1
-// mod 6).  With Y the lower such row, the windows (row pairs) that can change
-// are Y, Y+1 and Y+2:
-//   bits 0-4, 5-9, 10-14: 6 * window index, the window's bit offset
-//   bit 15, bits 16-21:   word (Y / 3) and shift (19 * (Y % 3)) of the row's
-//                         packed minimum over the other windows Y+3..Y+5
-//   bits 22-30:           the three window indices, 3 bits each
-inline uint32_t inc6_descriptor(uint64_t flip) {
-  int rows = 0;
-  for (int r = 0; r < 6; ++r)
-    if ((flip >> (6 * r)) & 63ull) rows |= 1 << r;
-  if (flip >> 36) return 0xffffffffu;
-  int Y = -1;
-  for (int r = 0; r < 6; ++r) {
-    const int one = 1 << r, two = one | (1 << ((r + 1) % 6));
-    if (rows == one || rows == two) {
-      Y = r;
-      if (rows == two) break;  // both rows of a pair: its lower row (cyclic)
-    }
-  }
-  if (rows == 0) Y = 0;  // padding: no flip, never matched
-  if (Y < 0) return 0xffffffffu;
-  uint32_t d = 0;
-  for (int q = 0; q < 3; ++q) {
-    const uint32_t w = (uint32_t)((Y + q) % 6);
-    d |= (6u * w) << (5 * q);
-    d |= w << (22 + 3 * q);
-  }
-  d |= (uint32_t)(Y / 3) << 15;
-  d |= (uint32_t)(19 * (Y % 3)) << 16;
-  return d;
-}
 
 enum Prec : int { kF64 = 0, kC128 = 1, kF32 = 2, kC64 = 3 };
 
@@ -193,6 +156,7 @@ int64_t term_expect_blocks(const OpDev& op);
 // and the reduction's chunk sums); terms per pass for vectors of nc real
 // components (0: the tables leave no LDS room)
 int64_t term_expect_scratch_rows(const OpDev& op);
+
 int term_expect_window(const OpDev& op, int nc);
 
 // ---- fast path: bond-pair term groups + register canonicaliser + hash lookup

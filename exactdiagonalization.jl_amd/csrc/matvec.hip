@@ -272,196 +272,12 @@ __global__ __launch_bounds__(kMvBlock) void k_matvec(OpDev op, const V* __restri
   y[row] = accumulate ? vadd(y[row], acc) : acc;
 }
 
-// ---------------------------------------------------------------------------
-// 6x6 torus, the 36 translations alone, every character 1, every off-diagonal
-// term flipping bits of one row or of two adjacent rows (nearest- and
-// next-nearest-neighbour bonds): the canonical form of an image b' = b ^ f
-// computed incrementally from its row b.
-//
-// The smallest translation image of b' starts with the smallest (top row,
-// second row) pair over the six windows (row y on top, row y-1 below it) and
-// the six rotations; the row-pair table gives each window's smallest pair and
-// the rotations reaching it.  A flip within rows Y, Y+1 changes windows Y,
-// Y+1 and Y+2 only.  The row's own minimum over the other three windows
-// (Y+3..Y+5), with the window and rotation reaching it and whether that is
-// the only one, is computed once per row for each Y (19 bits each, packed in
-// two words); an image then needs three table reads instead of six and no
-// candidate walk.  When exactly one (window, rotation) reaches the smallest
-// pair, it gives the canonical form s = T(y, a)(b') and the stabiliser of s
-// is trivial (cnt = 1).  Otherwise (about 1.5% of the 6x6 J1-J2 images: ties
-// of the row pair) the image goes to a deferred pass with the full
-// canonicaliser at the end of its term segment.  Same s and cnt as the full
-// canonicaliser for every image, hence the same matrix, summed in a
-// different order only for the deferred images.
-// ---------------------------------------------------------------------------
-
-// Canonical form of bp = b ^ f through the descriptor dsc of f (see
-// inc6_descriptor) and the row's packed minima m3lo / m3hi; false when the
-// smallest pair is not reached by exactly one (window, rotation).
-__device__ __forceinline__ bool inc6_canon(const GroupLds& L, uint64_t m3lo, uint64_t m3hi, uint32_t dsc, uint64_t bp,
-                                           uint64_t& s) {
-  const uint64_t w = (bp << 6) | (bp >> 30);  // row 5 copied below row 0: window y at bits 6y..6y+11
-  const uint32_t e0 = L.pair[(uint32_t)(w >> (dsc & 31u)) & 0xfffu];
-  const uint32_t e1 = L.pair[(uint32_t)(w >> ((dsc >> 5) & 31u)) & 0xfffu];
-  const uint32_t e2 = L.pair[(uint32_t)(w >> ((dsc >> 10) & 31u)) & 0xfffu];
-  const uint32_t M = (uint32_t)((((dsc >> 15) & 1u) ? m3hi : m3lo) >> ((dsc >> 16) & 63u)) & 0x7ffffu;
-  const uint32_t km = ((M >> 7) << 8) | 0xffu;
-  const uint32_t k0 = e0 | 0xffu, k1 = e1 | 0xffu, k2 = e2 | 0xffu;
-  const uint32_t best = min(min(k0, k1), min(k2, km));
-  const uint32_t n = (uint32_t)(k0 == best) + (uint32_t)(k1 == best) + (uint32_t)(k2 == best) + (uint32_t)(km == best);
-  // the (window, rotation mask) reaching best; the row's own minimum carries
-  // its single rotation, or the two-bit mask 3 when it is not unique
-  const uint32_t mo = ((M >> 6) & 1u) ? (1u << (M & 7u)) : 3u;
-  const uint32_t yw = k0 == best ? (dsc >> 22) & 7u : k1 == best ? (dsc >> 25) & 7u : k2 == best ? (dsc >> 28) & 7u : (M >> 3) & 7u;
-  const uint32_t am = k0 == best ? e0 & 0xffu : k1 == best ? e1 & 0xffu : k2 == best ? e2 & 0xffu : mo;
-  const int a = (int)(__builtin_ctz(am | 0x20u));
-  const int sh = (5 - (int)yw) * 6;  // row yw moves to the top
-  const uint64_t t = ((bp << sh) | (bp >> (36 - sh))) & 0xfffffffffull;
-  s = ((t << a) & L.rot_hi[a]) | ((t >> (6 - a)) & L.rot_lo[a]);
-  return n == 1u && (am & (am - 1u)) == 0u;
-}
-
-template <typename V, bool TRANS>
-__global__ __launch_bounds__(kMvBlock) void k_matvec_inc6(OpDev op, uint32_t inc_lds, const V* __restrict__ x,
-                                                          V* __restrict__ y, int accumulate) {
-  extern __shared__ unsigned char lds[];
-  const GroupLds L = stage_group(op.g, op.lay, lds);
-  const MvTerms T = stage_mv_terms(op, lds, true);
-  lds_copy(lds + op.w_lds, op.ix.W, op.w_bytes);
-  const uint64_t* W = (const uint64_t*)(lds + op.w_lds);
-  uint16_t* codes = (uint16_t*)(lds + op.codes_lds);
-  if (op.xm.values) lds_copy((unsigned char*)codes, op.xm.topcode, 2u << op.xm.top_bits);
-  uint32_t* D = (uint32_t*)(lds + inc_lds);
-  for (int t = threadIdx.x; t < op.n_mv; t += blockDim.x) D[t] = op.inc_desc[t];
-  __syncthreads();
-  const int64_t row = (int64_t)blockIdx.x * kMvBlock + threadIdx.x;
-  if (row >= op.dim) return;
-  using R = typename VT<V>::R;
-  const uint64_t b = op.rows[row];
-  double dre = 0.0, dim_ = 0.0;
-  if (op.diag_re) {
-    dre = op.diag_re[row];
-    if (op.diag_im) dim_ = op.diag_im[row];
-  } else {
-    diag_sum(op, b, dre, dim_);
-  }
-  V acc = vzero<V>();
-  vfma(acc, (R)dre, (R)dim_, x[row]);
-  const double sqrt_own = sqrt((double)op.orbit[row]);
-  const double w1 = sqrt_own * L.sqcnt[1];  // weight factor of an image with a trivial stabiliser
-
-  // the row's windows; for each Y the minimum over windows Y+3..Y+5, packed
-  // as (pair << 7) | (unique << 6) | (window << 3) | rotation
-  const uint64_t wr = (b << 6) | (b >> 30);
-  uint32_t er[6];
-#pragma unroll
-  for (int q = 0; q < 6; ++q) er[q] = L.pair[(uint32_t)(wr >> (6 * q)) & 0xfffu];
-  uint64_t m3[2] = {0ull, 0ull};
-#pragma unroll
-  for (int Y = 0; Y < 6; ++Y) {
-    const uint32_t ea = er[(Y + 3) % 6], eb = er[(Y + 4) % 6], ec = er[(Y + 5) % 6];
-    const uint32_t ka = ea | 0xffu, kb = eb | 0xffu, kc = ec | 0xffu;
-    const uint32_t mn = min(min(ka, kb), kc);
-    const uint32_t nt = (uint32_t)(ka == mn) + (uint32_t)(kb == mn) + (uint32_t)(kc == mn);
-    const uint32_t e = ka == mn ? ea : (kb == mn ? eb : ec);
-    const uint32_t ys = ka == mn ? (uint32_t)((Y + 3) % 6) : (kb == mn ? (uint32_t)((Y + 4) % 6) : (uint32_t)((Y + 5) % 6));
-    const uint32_t am = e & 0xffu;
-    const uint32_t uq = (nt == 1u && (am & (am - 1u)) == 0u) ? 1u : 0u;
-    const uint32_t as = __builtin_ctz(am | 0x20u);
-    const uint64_t packed = ((uint64_t)(mn >> 8) << 7) | (uq << 6) | (ys << 3) | as;
-    m3[Y / 3] |= packed << (19 * (Y % 3));
-  }
-
-  const int n_terms = op.n_mv;
-  for (int seg = 0; seg < n_terms; seg += 64) {
-    const int nseg = (n_terms - seg) < 64 ? (n_terms - seg) : 64;
-    uint64_t bits = 0;
-    for (int j = 0; j < nseg; j += 2) {  // two terms with one mask per test (nseg is even)
-      const uint64_t* m = T.mm + 3 * ((seg + j) >> 1);
-      const uint64_t xm = b & m[0];
-      bits |= (uint64_t)((uint32_t)(xm == m[1]) | ((uint32_t)(xm == m[2]) << 1)) << j;
-    }
-    uint64_t defer = 0;  // images of this segment left to the full canonicaliser
-    while (bits) {
-      int t[2];
-      bool ok1;
-      t[0] = seg + __builtin_ctzll(bits);
-      bits &= bits - 1;
-      ok1 = bits != 0;
-      t[1] = ok1 ? seg + __builtin_ctzll(bits) : t[0];
-      bits &= ok1 ? bits - 1 : bits;
-      ulonglong2 fa[2];
-      uint64_t sk[2];
-      bool fast[2];
-#pragma unroll
-      for (int k = 0; k < 2; ++k) {
-        fa[k] = *(const ulonglong2*)(T.fa + 2 * t[k]);
-        fast[k] = inc6_canon(L, m3[0], m3[1], D[t[k]], b ^ fa[k].x, sk[k]);
-      }
-      defer |= fast[0] ? 0ull : (1ull << (t[0] - seg));
-      defer |= (!ok1 || fast[1]) ? 0ull : (1ull << (t[1] - seg));
-      const bool use0 = fast[0], use1 = ok1 && fast[1];
-      V xv[2];
-      bool f[2];
-      if (op.xm.values) {
-        xmap_fetch2(op.xm, codes, sk[0], sk[1], xv[0], xv[1], f[0], f[1]);
-      } else {
-        int64_t i0, i1;
-        index_of2(op.ix, W, sk[0], sk[1], i0, i1);
-        f[0] = i0 >= 0;
-        f[1] = i1 >= 0;
-        xv[0] = x[f[0] ? i0 : 0];
-        xv[1] = x[f[1] ? i1 : 0];
-      }
-      if (use0 && f[0]) vfma(acc, (R)(__longlong_as_double((long long)fa[0].y) * w1),
-                             (R)(VT<V>::cx ? T.ai[t[0]] * w1 : 0.0), xv[0]);
-      if (use1 && f[1]) vfma(acc, (R)(__longlong_as_double((long long)fa[1].y) * w1),
-                             (R)(VT<V>::cx ? T.ai[t[1]] * w1 : 0.0), xv[1]);
-    }
-    while (defer) {
-      const int tt = seg + __builtin_ctzll(defer);
-      defer &= defer - 1;
-      const ulonglong2 fd = *(const ulonglong2*)(T.fa + 2 * tt);
-      uint64_t s;
-      int h, cnt;
-      Canon<kTransPairs6x6>::template run<false>(op.g, L, b ^ fd.x, s, h, cnt);
-      V xv;
-      bool found;
-      if (op.xm.values) {
-        found = xmap_fetch1(op.xm, codes, s, xv);
-      } else {
-        const int64_t i = index_of(op.ix, W, s);
-        found = i >= 0;
-        xv = x[found ? i : 0];
-      }
-      if (found) {
-        const double sc = sqrt_own * L.sqcnt[cnt];
-        vfma(acc, (R)(__longlong_as_double((long long)fd.y) * sc), (R)(VT<V>::cx ? T.ai[tt] * sc : 0.0), xv);
-      }
-    }
-  }
-  y[row] = accumulate ? vadd(y[row], acc) : acc;
-}
-
 template <typename V, int MODE>
 hipError_t launch_mv_t(const OpDev& op, int transpose, int accumulate, const void* x, void* y, hipStream_t st) {
   dim3 grid((unsigned)((op.dim + kMvBlock - 1) / kMvBlock));
   // unit-phase kernels only where the canonicaliser can drop the element (row-pair tori)
   const bool unit = MODE >= kGroupTorusPairs && op.g.unit_phases;
   const uint32_t lds = op.xm.values ? op.codes_lds + (2u << op.xm.top_bits) : op.lds_total;
-  if constexpr (MODE == kTransPairs6x6) {
-    if (unit && op.inc_desc && op.n_mv) {  // the incremental canonical form
-      const uint32_t inc_lds = (lds + 15u) & ~15u;
-      const uint32_t total = inc_lds + 4u * (uint32_t)op.n_mv;
-      if (transpose)
-        hipLaunchKernelGGL((k_matvec_inc6<V, true>), grid, dim3(kMvBlock), total, st, op, inc_lds, (const V*)x, (V*)y,
-                           accumulate);
-      else
-        hipLaunchKernelGGL((k_matvec_inc6<V, false>), grid, dim3(kMvBlock), total, st, op, inc_lds, (const V*)x,
-                           (V*)y, accumulate);
-      return hipGetLastError();
-    }
-  }
 #define EDHIP_MV(T, U)                                                                                                 \
   hipLaunchKernelGGL((k_matvec<V, MODE, T, U>), grid, dim3(kMvBlock), lds, st, op, (const V*)x, (V*)y, accumulate)
   if (transpose) {

@@ -582,7 +582,7 @@ class AbstractOperatorRepresentation:
         return np.dtype(np.complex128) if self.is_complex else np.dtype(np.float64)
 
     # ---- strategy (performance knob; every choice gives the same matrix) -------
-    _KERNELS = {"auto": 0, "terms": 1, "pairs": 2, "incremental": 3}
+    _KERNELS = {"auto": 0, "terms": 1, "pairs": 2}
     _LOOKUPS = {"auto": 0, "search": 1, "hash": 2, "bitmap": 3, "xspread": 4}
 
     def set_strategy(self, kernel: str = "auto", lookup: str = "auto"):
@@ -591,7 +591,7 @@ class AbstractOperatorRepresentation:
     def last_strategy(self):
         k, l = ctypes.c_int(), ctypes.c_int()
         self.ctx.check(_lib.lib().edhip_op_last_strategy(self._handle, ctypes.byref(k), ctypes.byref(l)))
-        return ({0: None, 1: "terms", 2: "pairs", 3: "incremental"}[k.value],
+        return ({0: None, 1: "terms", 2: "pairs"}[k.value],
                 {0: None, 1: "search", 2: "hash", 3: "bitmap", 4: "xspread"}[l.value])
 
     # ---- apply ---------------------------------------------------------------

@@ -198,13 +198,9 @@ int edhip_apply_host(edhip_op* op, int prec, int transpose, int accumulate, int6
  * workload tried (e.g. 514 vs 97 ms per 6x6 product for the table), and are
  * kept only as alternative implementations the parity tests cross-check
  * against (test-only).
- *   kernel: 0 auto (3 where it applies, else 1), 1 term-list kernel,
- *           2 bond-pair kernel (operators whose terms act on one or two
- *           bits, plain or torus-translation spaces; test-only),
- *           3 term-list kernel with the incremental canonical form of the
- *           6x6 torus translations (every character 1, every flip within
- *           one row or two adjacent rows: nearest- and next-nearest-
- *           neighbour bonds)
+ *   kernel: 0 auto (term list), 1 term-list kernel, 2 bond-pair kernel
+ *           (operators whose terms act on one or two bits, plain or
+ *           torus-translation spaces; test-only)
  *   lookup: 0 auto (term-list kernel: x spread over word positions when
  *             it fits, for 4-byte elements, for full popcount sectors and
  *             where the spread is at most 256 MB; else bitmap, else sorted

@@ -89,8 +89,6 @@ def test_pairs_rejected_where_not_covered(ed):
     opr = ed.represent(rh, H)  # ring of 14: rows wider than the 12-bit rotation table
     with pytest.raises(ed.ArgumentError):
         opr.set_strategy("pairs")
-    with pytest.raises(ed.ArgumentError):  # the incremental canonical form is for the 6x6 torus alone
-        opr.set_strategy("incremental")
     three = ed.represent(rh, ed.simplify(s(0, "z") * s(1, "z") * s(2, "z")))
     with pytest.raises(ed.ArgumentError):
         three.set_strategy("pairs")
@@ -139,8 +137,8 @@ def test_torus6x6_properties(ed):
     vH = torch.empty_like(v)
     ed.mul_(vH, v, opr)
     assert (torch.linalg.norm(vH - Hv) / torch.linalg.norm(Hv)).item() < 1e-13
-    assert opr.last_strategy() == ("incremental", "bitmap")  # FP64: auto keeps the rank index
-    for strat in (("pairs", "bitmap"), ("terms", "xspread"), ("terms", "bitmap"), ("incremental", "xspread")):
+    assert opr.last_strategy() == ("terms", "bitmap")  # FP64: auto keeps the rank index
+    for strat in (("pairs", "bitmap"), ("terms", "xspread")):
         opr.set_strategy(*strat)
         Hv2 = torch.empty_like(v)
         ed.mul_(Hv2, opr, v)
@@ -151,9 +149,9 @@ def test_torus6x6_properties(ed):
     v32 = v.float()
     Hv32 = torch.empty_like(v32)
     ed.mul_(Hv32, opr, v32)
-    assert opr.last_strategy() == ("incremental", "xspread")  # FP32: x spread over the word positions
+    assert opr.last_strategy() == ("terms", "xspread")  # FP32: x spread over the word positions
     assert (torch.linalg.norm(Hv32.double() - Hv) / torch.linalg.norm(Hv)).item() < 1e-5
-    opr.set_strategy("incremental", "bitmap")
+    opr.set_strategy("terms", "bitmap")
     Hv32b = torch.empty_like(v32)
     ed.mul_(Hv32b, opr, v32)
     opr.set_strategy()

@@ -1,9 +1,8 @@
 """The 6x6 J1-J2 k = (0,0) translation sector (252,091,362 representatives),
 the benchmarked workload, through the products the benchmark times: the
-translation-only 6x6 torus kernels (the incremental canonical form, and the
-full row-pair canonicaliser with the shape fixed at compile time), FP32
-products reading x spread over word positions and FP64 products with the
-blocked rank index, row form and column form (GPU).
+translation-only torus kernels with the 6x6 shape fixed at compile time,
+FP32 products reading x spread over word positions and FP64 products with
+the blocked rank index, row form and column form (GPU).
 
 No oracle fits this sector (its parent arrays hold 9.08e9 words), so the
 checks are independent of the product's own inputs and of its kernels:
@@ -113,39 +112,28 @@ def test_sampled_rows_vs_host_sums(ed, sector):
     ref32, mag32 = _host_rows(rows, basis, orbit, terms, T, at(x32))
     rows_t = torch.as_tensor(rows, device="cuda")
     ys = {}
-    # auto: the incremental canonical form (the benchmarked products); the
-    # plain term-list kernel with the full canonicaliser beside it
-    for kernel in ("auto", "terms"):
-        opr.set_strategy(kernel)
-        want = "incremental" if kernel == "auto" else "terms"
-        for dt, lookup in ((torch.float64, "bitmap"), (torch.float32, "xspread")):
-            xv = x if dt == torch.float64 else x32
-            for transpose in (False, True):
-                y = torch.empty_like(xv)
-                if transpose:
-                    ed.mul_(y, xv, opr)
-                else:
-                    ed.mul_(y, opr, xv)
-                assert opr.last_strategy() == (want, lookup)  # auto lookups: the benchmarked ones
-                got = y[rows_t].double().cpu().numpy()
-                ref, mag = (ref64, mag64) if dt == torch.float64 else (ref32, mag32)
-                tol = 1e-13 if dt == torch.float64 else 1e-5
-                err = np.abs(got - ref) / mag
-                assert err.max() < tol, (kernel, dt, transpose, err.max())
-                ys[(kernel, dt, transpose)] = y
-    opr.set_strategy()
+    for dt, lookup in ((torch.float64, "bitmap"), (torch.float32, "xspread")):
+        xv = x if dt == torch.float64 else x32
+        for transpose in (False, True):
+            y = torch.empty_like(xv)
+            if transpose:
+                ed.mul_(y, xv, opr)
+            else:
+                ed.mul_(y, opr, xv)
+            assert opr.last_strategy() == ("terms", lookup)  # auto: the benchmarked lookups
+            got = y[rows_t].double().cpu().numpy()
+            ref, mag = (ref64, mag64) if dt == torch.float64 else (ref32, mag32)
+            tol = 1e-13 if dt == torch.float64 else 1e-5
+            err = np.abs(got - ref) / mag
+            assert err.max() < tol, (dt, transpose, err.max())
+            ys[(dt, transpose)] = y
     # the real symmetric matrix: row form = column form; FP32 products agree
-    # with FP64 ones to single precision over the whole vector; both kernels
-    # give the same products (the incremental one sums its deferred images
-    # last, so equal to rounding)
-    y64, y64c = ys[("auto", torch.float64, False)], ys[("auto", torch.float64, True)]
+    # with FP64 ones to single precision over the whole vector
+    y64, y64c = ys[(torch.float64, False)], ys[(torch.float64, True)]
     assert (torch.linalg.norm(y64c - y64) / torch.linalg.norm(y64)).item() < 1e-14
-    for kernel in ("auto", "terms"):
-        for tr in (False, True):
-            d = torch.linalg.norm(ys[(kernel, torch.float32, tr)].double() - y64) / torch.linalg.norm(y64)
-            assert d.item() < 1e-6
-            d = torch.linalg.norm(ys[(kernel, torch.float64, tr)] - y64) / torch.linalg.norm(y64)
-            assert d.item() < 1e-14
+    for tr in (False, True):
+        d = torch.linalg.norm(ys[(torch.float32, tr)].double() - y64) / torch.linalg.norm(y64)
+        assert d.item() < 1e-6
     del ys, y64, y64c, x, x32
 
 
