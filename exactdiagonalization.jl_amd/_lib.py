@@ -151,6 +151,7 @@ SIGNATURES = {
     "edhip_lanczos_default_opts": (None, [_P(LanczosOpts)]),
     "edhip_lanczos": (_i32, [_vp, _P(LanczosOpts), _P(_dbl), _vp, _P(LanczosStats)]),
     "edhip_term_expect": (_i32, [_vp, _i32, _vp, _vp, _P(_dbl)]),
+    "edhip_term_expect_noalloc": (_i32, [_vp, _i32, _vp, _vp, _P(_dbl)]),
     "edhip_minres": (_i32, [_vp, _i32, _dbl, _i32, _vp, _vp, _vp, _dbl, _i32, _P(MinresStats)]),
     "edhip_vec_dot": (_i32, [_vp, _i32, _i64, _vp, _vp, _P(_dbl)]),
     "edhip_vec_gaussian": (_i32, [_vp, _i32, _i64, _u64, _vp]),

@@ -1521,6 +1521,8 @@ int edhip_create(int device, edhip_ctx** out) {
     HIP_OR_THROW(hipStreamCreateWithFlags(&ctx->own, hipStreamNonBlocking));
     ctx->stream = ctx->own;
     for (auto& ev : ctx->ev) HIP_OR_THROW(hipEventCreate(&ev));
+    // this device's address of the allocation-free term pass's scratch
+    HIP_OR_THROW(term_expect_noalloc_setup(device));
   });
   if (rc != EDHIP_OK) {
     delete ctx;
@@ -2406,6 +2408,44 @@ int edhip_term_expect(edhip_op* op, int prec, const void* phi, const void* psi, 
     HIP_OR_THROW(hipMemcpyAsync(h.data(), op->ex_out.p, h.size() * 8, hipMemcpyDeviceToHost, op->ctx->stream));
     op->ctx->sync();
     for (int64_t t = 0; t < op->n_terms; ++t) out[2 * t] = out[2 * t + 1] = 0.0;
+    for (int i = 0; i < op->n_diag; ++i) {
+      out[2 * op->diag_ids[i]] = h[2 * (size_t)i];
+      out[2 * op->diag_ids[i] + 1] = h[2 * (size_t)i + 1];
+    }
+    for (int i = 0; i < op->n_off; ++i) {
+      out[2 * op->off_ids[i]] = h[2 * (size_t)(op->n_diag + i)];
+      out[2 * op->off_ids[i] + 1] = h[2 * (size_t)(op->n_diag + i) + 1];
+    }
+  });
+}
+
+int edhip_term_expect_noalloc(edhip_op* op, int prec, const void* phi, const void* psi, double* out) {
+  if (!op || !out) return EDHIP_ERR_ARGUMENT;
+  return guarded(op->ctx, [&] {
+    check_prec_for_op(op, prec);
+    const int nt = op->n_diag + op->n_off;
+    if (nt > kNoAllocMaxTerms)
+      fail(EDHIP_ERR_ARGUMENT, "edhip_term_expect_noalloc: " + std::to_string(nt) + " terms exceed its maximum of " +
+                                   std::to_string(kNoAllocMaxTerms));
+    for (int64_t t = 0; t < op->n_terms; ++t) out[2 * t] = out[2 * t + 1] = 0.0;
+    if (nt == 0) return;
+    double* scr = term_expect_noalloc_scratch(op->ctx->device);
+    if (!scr) fail(EDHIP_ERR_DEVICE, "the scratch of this device was not set up");
+    // the operator's lookup as built at its creation (no spread of psi: that
+    // would allocate), partial rows of this operator's width in the scratch
+    const OpDev& d = op->dev_row;
+    const int nc = prec_is_complex(prec) ? 2 : 1;
+    if (term_expect_window(d, nc) < 1)
+      fail(EDHIP_ERR_ARGUMENT, "operator and symmetry tables leave no LDS room for the fused expectation kernel");
+    const int64_t nb = term_expect_blocks(d);
+    const size_t width = (size_t)nt * 2;
+    double* res = scr + (size_t)term_expect_scratch_rows(d) * width;
+    if ((size_t)(res - scr) + width > term_expect_noalloc_doubles())
+      fail(EDHIP_ERR_ARGUMENT, "edhip_term_expect_noalloc: scratch too small for this operator");
+    HIP_OR_THROW(launch_term_expect(d, prec, phi, psi, scr, nb, res, op->ctx->stream));
+    double h[2 * kNoAllocMaxTerms];
+    HIP_OR_THROW(hipMemcpyAsync(h, res, width * 8, hipMemcpyDeviceToHost, op->ctx->stream));
+    op->ctx->sync();
     for (int i = 0; i < op->n_diag; ++i) {
       out[2 * op->diag_ids[i]] = h[2 * (size_t)i];
       out[2 * op->diag_ids[i] + 1] = h[2 * (size_t)i + 1];

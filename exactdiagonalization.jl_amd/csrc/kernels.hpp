@@ -157,6 +157,16 @@ int64_t term_expect_blocks(const OpDev& op);
 // components (0: the tables leave no LDS room)
 int64_t term_expect_scratch_rows(const OpDev& op);
 
+// Fixed device scratch of the allocation-free term pass (one array per GPU,
+// held until the process ends): room for the per-block partial sums and the
+// result of at most kNoAllocMaxTerms terms.  term_expect_noalloc_setup runs
+// once per device with that device current (the context's creation) and
+// keeps the device address of the array; term_expect_noalloc_scratch returns
+// it (nullptr before the setup).
+constexpr int kNoAllocMaxTerms = 512;
+hipError_t term_expect_noalloc_setup(int device);
+double* term_expect_noalloc_scratch(int device);
+size_t term_expect_noalloc_doubles();
 int term_expect_window(const OpDev& op, int nc);
 
 // ---- fast path: bond-pair term groups + register canonicaliser + hash lookup

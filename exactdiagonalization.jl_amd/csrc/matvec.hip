@@ -873,6 +873,36 @@ int64_t term_expect_scratch_rows(const OpDev& op) {
   return nb + (nb + 63) / 64;
 }
 
+// The allocation-free term pass's scratch: (blocks + chunks) rows of partial
+// sums of 2 kNoAllocMaxTerms doubles, then the 2 kNoAllocMaxTerms results.
+// Every entry a call reads was stored by the same call (the per-block passes
+// store each (block, term) entry once; the reductions store theirs), so no
+// clearing is needed between calls.
+constexpr size_t kNoAllocRows = (size_t)kExpectBlocksMax + ((size_t)kExpectBlocksMax + 63) / 64;
+constexpr size_t kNoAllocDoubles = kNoAllocRows * 2 * kNoAllocMaxTerms + 2 * kNoAllocMaxTerms;
+__device__ __align__(256) double scratch[kNoAllocDoubles];
+
+constexpr int kNoAllocMaxDevices = 64;
+double* g_scratch_ptr[kNoAllocMaxDevices] = {};
+
+hipError_t term_expect_noalloc_setup(int device) {
+  if (device < 0 || device >= kNoAllocMaxDevices) return hipErrorInvalidDevice;
+  if (g_scratch_ptr[device]) return hipSuccess;
+  void* p = nullptr;
+  // the device address of this device's copy of the array (the name itself is
+  // a host-side stand-in and never reaches a kernel)
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return e;
+  g_scratch_ptr[device] = (double*)p;
+  return hipSuccess;
+}
+
+double* term_expect_noalloc_scratch(int device) {
+  return (device >= 0 && device < kNoAllocMaxDevices) ? g_scratch_ptr[device] : nullptr;
+}
+
+size_t term_expect_noalloc_doubles() { return kNoAllocDoubles; }
+
 hipError_t launch_term_expect(const OpDev& op, int prec, const void* phi, const void* psi, double* d_partial,
                               int64_t n_blocks, double* d_out, hipStream_t st) {
   switch (prec) {

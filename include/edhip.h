@@ -317,6 +317,17 @@ int edhip_lanczos(edhip_op* op, const edhip_lanczos_opts* opts, double* evals_ho
  * 2*n_terms doubles (re, im), in the order of the terms given to
  * edhip_op_create.  phi, psi: device vectors of type prec. */
 int edhip_term_expect(edhip_op* op, int prec, const void* phi, const void* psi, double* out_host);
+/* The same per-term forms with no device allocation: no counterpart in the
+ * reference; for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * partial sums live in a fixed per-GPU scratch array (about 17 MB, held
+ * until the process ends), which limits the operator to 512 terms
+ * (n_diag + n_off; more: EDHIP_ERR_ARGUMENT); psi is read through the
+ * operator's index as built by edhip_op_create (never through a spread of
+ * psi, which would be allocated).  Every call on one GPU shares that GPU's
+ * scratch: calls that overlap on one GPU (other streams, other threads,
+ * other contexts) race. */
+int edhip_term_expect_noalloc(edhip_op* op, int prec, const void* phi, const void* psi, double* out_host);
 
 typedef struct {
   int iterations;
