@@ -193,9 +193,12 @@ struct GroupHost {
 
 // Top bits of the words coded in the spread of x (a table of 2^12 u16 codes,
 // 8 KiB, staged in LDS by the matvec), and the largest spread, in elements
-// per basis word, the matvec builds (6x6 k=(0,0): 18.2).
+// per basis word, the matvec builds (6x6 k=(0,0): 4.07 in the colex form).
 constexpr int kXTopBits = 12;
 constexpr uint64_t kXMapRatio = 32;
+// Spreads of 8- and 16-byte elements are used by default when they hold at
+// most this many positions per basis word
+constexpr uint64_t kXMapDenseRatio = 8;
 // Spreads of 8- and 16-byte elements are used by default only up to this
 // size (about what the last-level cache holds)
 constexpr uint64_t kXMapCachedBytes = 256ull << 20;
@@ -304,6 +307,15 @@ struct edhip_space {
   mutable DevBuf xtop;  // code of each value of the top bits of the words
   mutable int xtop_bits = -1, xtop_shift = 0, xtop_K = 0;
   mutable uint64_t xtop_codes = 0;
+  mutable DevBuf xcolex;  // colex tables of the low 12 bits (single-popcount spaces)
+  mutable bool xcolex_on = false;
+
+  // The popcount every word of the space has, or -1: a popcount sector with
+  // one allowed popcount, or a reduction of one.
+  int single_popcount() const {
+    if (parent) return parent->single_popcount();
+    return (is_sector && popcounts.size() == 1) ? popcounts[0] : -1;
+  }
 
   void release_xmaps() const {
     for (XMapHost& X : xmap) {
@@ -322,7 +334,11 @@ struct edhip_space {
       HIP_OR_THROW(hipMemcpyAsync(&max_key, keys.as<uint64_t>() + (dim - 1), 8, hipMemcpyDeviceToHost, ctx->stream));
       ctx->sync();
       const int K = std::max(4, bit_length(max_key));
-      const int t = std::min(kXTopBits, K);
+      // colex positions for the low 12 bits when every word has one popcount
+      // (the popcount of those bits is then fixed by the rest of the word);
+      // the top code covers what lies above bit 11 (at most kXTopBits bits)
+      xcolex_on = single_popcount() >= 0 && K >= 16;
+      const int t = xcolex_on ? std::min(kXTopBits, K - 12) : std::min(kXTopBits, K);
       const int sh = K - t;
       DevBuf flags;
       flags.alloc(((size_t)1 << t) * 4);
@@ -336,12 +352,13 @@ struct edhip_space {
       for (size_t v = 0; v < fl.size(); ++v)
         if (fl[v]) codes[v] = (uint16_t)nc++;
       upload(xtop, codes, ctx->stream);
+      if (xcolex_on) upload(xcolex, colex12_tables(), ctx->stream);
       xtop_bits = t;
       xtop_shift = sh;
       xtop_K = K;
       xtop_codes = nc;
     }
-    return xtop_codes << xtop_shift;
+    return xcolex_on ? (xtop_codes << (xtop_shift - 12)) * (uint64_t)kColexStride : xtop_codes << xtop_shift;
   }
 
   const XMapHost* ensure_xmap(size_t es) const {
@@ -358,7 +375,8 @@ struct edhip_space {
       X.values.alloc(n_pos * es);
       HIP_OR_THROW(hipMemsetAsync(X.values.p, 0, n_pos * es, ctx->stream));
       ctx->sync();
-      X.dev = XMapDev{X.values.p, xtop.as<uint16_t>(), xtop_K, xtop_bits, xtop_shift, 0};
+      X.dev = XMapDev{X.values.p, xtop.as<uint16_t>(), xcolex_on ? xcolex.as<uint16_t>() : nullptr, xtop_K,
+                      xtop_bits, xtop_shift, xcolex_on ? xtop_shift - 12 : 0, single_popcount(), 0};
       X.ok = true;
       return &X;
     } catch (const Error& e) {
@@ -942,7 +960,8 @@ OpDev make_opdev(edhip_op* op, bool transpose) {
   d.lds_total = align_up(off, 16);
   // the matvec alone stages the code table of the spread x after the rest
   d.xm = XMapDev{};
-  d.codes_lds = d.lds_total + (2u << kXTopBits) <= 64 * 1024 ? d.lds_total : 0;
+  // (room for the largest code table and the colex tables)
+  d.codes_lds = d.lds_total + (2u << kXTopBits) + 2u * (64u + 64u * 7u) <= 64 * 1024 ? d.lds_total : 0;
   if (d.lds_total > 64 * 1024)
     fail(EDHIP_ERR_ARGUMENT, "operator and symmetry tables exceed the 64 KiB LDS budget of the matvec kernels (" +
                                  std::to_string(d.lds_total) + " bytes)");
@@ -1091,18 +1110,21 @@ bool ensure_diag(edhip_op* op) {
 
 // Points d at the spread of x over the word positions, filled with x, when
 // the operator's lookup strategy calls for it: lookup 4, and auto for 4-byte
-// elements (6x6 J1-J2 FP32: 118 -> 97 ms); for wider ones on full sectors,
+// elements (6x6 J1-J2 FP32: 118 -> 87 ms); for wider ones on full sectors,
 // whose other lookup is rank arithmetic, a chain of table reads per image
-// (4x7 J1-J2: FP32 14.9 -> 4.6 ms, FP64 14.9 -> 8.1 ms), and where the spread
-// stays cache-sized; a large FP64 spread is no faster than the blocked rank
-// index (6x6: 127 vs 128 ms).  Returns whether it did.
+// (4x7 J1-J2: FP32 14.9 -> 4.6 ms, FP64 14.9 -> 8.1 ms), where the spread
+// stays cache-sized, and where it is dense (at most kXMapDenseRatio positions
+// per basis word, as the colex form of single-popcount spaces gives: 6x6
+// FP64 127 -> 91 ms; the sparser plain form, 18 positions per word, is no
+// faster than the blocked rank index there).  Returns whether it did.
 bool attach_spread(edhip_op* op, int prec, OpDev& d, const void* x, hipStream_t st) {
   const edhip_space* sp = op->space;
   const bool rank_mode = sp->is_sector && !sp->group;
   const int idx_lookup = (op->strat_lookup == 2) ? 0 : op->strat_lookup;
   const size_t es = elem_size(prec);
   const bool want = idx_lookup == 4 ||
-                    (idx_lookup == 0 && (es == 4 || rank_mode || sp->ensure_xtop() * es <= kXMapCachedBytes));
+                    (idx_lookup == 0 && (es == 4 || rank_mode || sp->ensure_xtop() * es <= kXMapCachedBytes ||
+                                           sp->ensure_xtop() <= kXMapDenseRatio * (uint64_t)sp->dim));
   if (!want || !d.codes_lds) return false;
   const edhip_space::XMapHost* X = sp->ensure_xmap(es);
   if (!X) return false;

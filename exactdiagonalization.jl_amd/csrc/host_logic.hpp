@@ -157,4 +157,36 @@ inline std::string group_inverses(int G, int nb, const std::vector<int32_t>& per
   return std::string();
 }
 
+// Colex (combinatorial number system) rank of a 12-bit word among the 12-bit
+// words of its popcount j, in [0, C(12, j)) with C(12, j) <= 924, from two
+// 6-bit halves: rank(l) = lo[l & 63] + hi[(l >> 6) * 7 + popcount(l & 63)],
+// lo[v] = sum_i C(p_i, i) over the set bits p_1 < p_2 < ... of v, and
+// hi[v * 7 + m] = sum_i C(6 + q_i, m + i) over the set bits q_1 < ... of v.
+// Returns the 64 + 64 * 7 entries (lo, then hi).
+constexpr int kColexStride = 924;  // C(12, 6): room for every popcount class
+inline std::vector<uint16_t> colex12_tables() {
+  auto binom = [](int n, int k) -> uint32_t {
+    if (k < 0 || k > n) return 0;
+    uint64_t r = 1;
+    for (int i = 1; i <= k; ++i) r = r * (uint64_t)(n - k + i) / (uint64_t)i;
+    return (uint32_t)r;
+  };
+  std::vector<uint16_t> t(64 + 64 * 7, 0);
+  for (int v = 0; v < 64; ++v) {
+    uint32_t r = 0;
+    int i = 0;
+    for (int p = 0; p < 6; ++p)
+      if ((v >> p) & 1) r += binom(p, ++i);
+    t[v] = (uint16_t)r;
+    for (int m = 0; m <= 6; ++m) {
+      uint32_t h = 0;
+      int q = 0;
+      for (int p = 0; p < 6; ++p)
+        if ((v >> p) & 1) h += binom(6 + p, m + (++q));
+      t[64 + v * 7 + m] = (uint16_t)h;
+    }
+  }
+  return t;
+}
+
 }  // namespace edhip

@@ -181,7 +181,7 @@ __global__ __launch_bounds__(kMvBlock) void k_matvec(OpDev op, const V* __restri
   const uint64_t* W = (const uint64_t*)(lds + op.w_lds);
   // code table of the spread x (the host sets xm only with room for it)
   uint16_t* codes = (uint16_t*)(lds + op.codes_lds);
-  if (op.xm.values) lds_copy((unsigned char*)codes, op.xm.topcode, 2u << op.xm.top_bits);
+  if (op.xm.values) xmap_stage(op.xm, codes);
   __syncthreads();
   const int64_t row = (int64_t)blockIdx.x * kMvBlock + threadIdx.x;
   if (row >= op.dim) return;
@@ -277,7 +277,7 @@ hipError_t launch_mv_t(const OpDev& op, int transpose, int accumulate, const voi
   dim3 grid((unsigned)((op.dim + kMvBlock - 1) / kMvBlock));
   // unit-phase kernels only where the canonicaliser can drop the element (row-pair tori)
   const bool unit = MODE >= kGroupTorusPairs && op.g.unit_phases;
-  const uint32_t lds = op.xm.values ? op.codes_lds + (2u << op.xm.top_bits) : op.lds_total;
+  const uint32_t lds = op.xm.values ? op.codes_lds + xmap_lds_bytes(op.xm) : op.lds_total;
 #define EDHIP_MV(T, U)                                                                                                 \
   hipLaunchKernelGGL((k_matvec<V, MODE, T, U>), grid, dim3(kMvBlock), lds, st, op, (const V*)x, (V*)y, accumulate)
   if (transpose) {
@@ -526,7 +526,7 @@ __device__ __forceinline__ void load_c(const V* p, int64_t i, double& re, double
 // LDS offset of the term pass's per-wave bins: after the operator's tables,
 // and after the spread's code table when psi is read from the spread.
 __host__ __device__ __forceinline__ uint32_t expect_bins_offset(const OpDev& op) {
-  return op.xm.values ? op.codes_lds + (2u << op.xm.top_bits) : op.lds_total;
+  return op.xm.values ? op.codes_lds + xmap_lds_bytes(op.xm) : op.lds_total;
 }
 
 constexpr int kExpectBlocksMax = 2048;
@@ -554,7 +554,7 @@ __global__ __launch_bounds__(kBlock) void k_term_expect(OpDev op, const V* __res
   // psi of the images from its spread over the word positions (op.xm set):
   // the code table after the operator's tables, then the bins
   uint16_t* codes = (uint16_t*)(lds + op.codes_lds);
-  if (op.xm.values) lds_copy((unsigned char*)codes, op.xm.topcode, 2u << op.xm.top_bits);
+  if (op.xm.values) xmap_stage(op.xm, codes);
   double* bins = (double*)(lds + expect_bins_offset(op));  // [wave][nw][NC]
   for (int i = threadIdx.x; i < kExpectWaves * nw * NC; i += kBlock) bins[i] = 0.0;
   __syncthreads();

@@ -318,7 +318,7 @@ __global__ __launch_bounds__(kBlock) void k_xmap_fill(XMapDev X, const uint64_t*
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= dim) return;
   const uint64_t k = keys[i];
-  const uint64_t pos = ((uint64_t)X.topcode[k >> X.top_shift] << X.top_shift) | (k & ((1ull << X.top_shift) - 1ull));
+  const uint64_t pos = xmap_pos(X, X.colex, X.topcode[k >> X.top_shift], k);
   ((V*)X.values)[pos] = x[i];
 }
 

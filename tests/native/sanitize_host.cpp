@@ -1,7 +1,7 @@
 // Host-only checks built with AddressSanitizer and UndefinedBehaviorSanitizer
 // (tests/test_sanitizers.py): the product's host logic (the eigensolver's
-// tridiagonal / arrowhead eigensolve, the group closure and inverses) and
-// the oracle's C restatement on
+// tridiagonal / arrowhead eigensolve, the group closure and inverses, the
+// colex ranks of the spread of x) and the oracle's C restatement on
 // small inputs, so that an out-of-bounds access or undefined behaviour on
 // these paths aborts the run.  Returns 0 when every check holds.
 #include <stdint.h>
@@ -137,6 +137,25 @@ static void test_group_closure() {
   CHECK(!edhip::group_inverses(2, 2, rep, std::vector<uint64_t>{0, 0}, inv).empty());
 }
 
+static void test_colex() {
+  // the colex rank of a 12-bit word is a bijection of the words of each
+  // popcount j onto [0, C(12, j)), below the stride 924
+  const std::vector<uint16_t> t = edhip::colex12_tables();
+  CHECK(t.size() == 64 + 64 * 7);
+  for (int j = 0; j <= 12; ++j) {
+    std::vector<int> seen(edhip::kColexStride, 0);
+    int n = 0;
+    for (uint32_t l = 0; l < 4096; ++l) {
+      if (__builtin_popcount(l) != j) continue;
+      ++n;
+      const uint32_t r = t[l & 63u] + t[64u + (l >> 6) * 7u + (uint32_t)__builtin_popcount(l & 63u)];
+      CHECK(r < (uint32_t)edhip::kColexStride);
+      if (r < (uint32_t)edhip::kColexStride) ++seen[r];
+    }
+    for (int r = 0; r < edhip::kColexStride; ++r) CHECK(seen[r] == (r < n ? 1 : 0));
+  }
+}
+
 static void test_oracle() {
   // 8-site Heisenberg ring, Sz = 0, reduced by translations at k = 1 and by
   // translations x flip; apply (both forms) and sparse (serial and parallel)
@@ -226,6 +245,7 @@ static void test_oracle() {
 int main() {
   test_linear_algebra();
   test_group_closure();
+  test_colex();
   test_oracle();
   if (failures) {
     fprintf(stderr, "%d check(s) failed\n", failures);

@@ -1,5 +1,5 @@
 """The product's host logic (tridiagonal / arrowhead eigensolve, group
-closure and inverses) and the oracle's C
+closure and inverses, colex ranks of the spread of x) and the oracle's C
 restatement, built with AddressSanitizer and UndefinedBehaviorSanitizer
 (g++ / gcc, host code only) and run on small inputs (no GPU)."""
 import os
