@@ -244,6 +244,11 @@ def lanczos_times(ed, opr, tol_fp32=0.0):
             "refine_seconds": r.t_refine,
             "other_seconds": r.t_total - r.t_matvec - r.t_refine,
             "reorth_second_passes": r.second_passes,
+            # orthogonalisation kernels (dot products and updates against the
+            # basis): device time from HIP events, algorithmic bytes over it
+            "ortho_kernel_seconds": r.t_ortho,
+            "ortho_GBps": r.ortho_bytes / r.t_ortho / 1e9 if r.t_ortho > 0 else None,
+            "ortho_frac_of_8TBps": r.ortho_bytes / r.t_ortho / 8e12 if r.t_ortho > 0 else None,
             "residual": float(r.residuals[0]),
             "converged": r.converged,
         }
@@ -287,6 +292,28 @@ def grad_times(ed):
         ex = model.expectations(psi, psi)
     t_bw = (time.perf_counter() - t0) / reps
     g = J.T @ ex.real
+    # the fused term pass alone: its kernel and partial-sum reduction, HIP
+    # events around each launch; algorithmic bytes = D * (word + phi + psi)
+    # + images * (psi gather), the sector's index being rank arithmetic
+    from edhip import _lib
+
+    L = _lib.lib()
+    st = model.structure
+    ctx = st.ctx
+    n_img = ctypes.c_int64()
+    ctx.check(L.edhip_op_count_images(st._handle, ctypes.byref(n_img)))
+    tot, nl = ctypes.c_double(), ctypes.c_int64()
+    ctx.check(L.edhip_op_kernel_time(st._handle, 1, ctypes.byref(tot), ctypes.byref(nl)))
+    ctx.check(L.edhip_set_kernel_timing(ctx.handle, 1))
+    try:
+        for _ in range(reps):
+            model.expectations(psi, psi)
+    finally:
+        ctx.check(L.edhip_set_kernel_timing(ctx.handle, 0))
+    ctx.check(L.edhip_op_kernel_time(st._handle, 1, ctypes.byref(tot), ctypes.byref(nl)))
+    t_tk = tot.value / max(nl.value, 1) * 1e-3
+    D = psi.numel()
+    tk_bytes = D * (8 + 8 + 8) + n_img.value * 8
     y = torch.empty_like(psi)
     ed.mul_(y, H, psi)
     torch.cuda.synchronize()
@@ -307,6 +334,11 @@ def grad_times(ed):
         "energy_backward_ms": t_bw * 1e3,
         "fp64_matvec_ms": t_mv * 1e3,
         "energy_backward_matvec_equivalents": t_bw / t_mv,
+        "term_pass_kernel_ms": t_tk * 1e3,
+        "term_pass_images": int(n_img.value),
+        "term_pass_alg_bytes": tk_bytes,
+        "term_pass_GBps": tk_bytes / t_tk / 1e9,
+        "term_pass_frac_of_8TBps": tk_bytes / t_tk / 8e12,
         "homogeneity_residual": float(abs(np.dot(theta[:B], g[:B]) - r.values[0])),
         "correlator": "Sz_0 Sz_12",
         "correlator_value": float(val),

@@ -97,6 +97,8 @@ class LanczosStats(ctypes.Structure):
         ("t_matvec", ctypes.c_double),
         ("t_refine", ctypes.c_double),
         ("t_alloc", ctypes.c_double),
+        ("t_ortho", ctypes.c_double),
+        ("ortho_bytes", ctypes.c_double),
     ]
 
 

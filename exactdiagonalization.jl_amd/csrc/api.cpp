@@ -107,6 +107,7 @@ struct edhip_ctx {
   // allocations of hundreds of GB after frees cost seconds per call.
   DevBuf krylov;
   hipEvent_t ev[16] = {};
+  hipEvent_t ev_ortho[2] = {};  // the eigensolver's timing of its orthogonalisation kernels
   bool kernel_timing = false;  // time every matvec kernel launch (edhip_set_kernel_timing)
 
   char* krylov_arena(size_t bytes) {
@@ -884,7 +885,7 @@ struct edhip_op {
   int strat_kernel = 0;  // 0 auto, 1 term list, 2 pair groups
   int strat_lookup = 0;  // 0 auto, 1 sorted search (or rank), 2 hash table
   int used_kernel = 0, used_lookup = 0;
-  double kt_ms = 0.0;  // summed durations of the timed matvec kernel launches
+  double kt_ms = 0.0;  // summed durations of the timed matvec and term-pass kernel launches
   int64_t kt_n = 0;
 
   bool is_complex() const { return complex_amp || (space->group && space->group->complex_phase); }
@@ -1251,6 +1252,9 @@ struct TRLanczos {
   int n_locked = 0;
   int matvecs = 0;
   double t_matvec = 0.0;
+  // device time (HIP events) and algorithmic bytes of the orthogonalisation
+  // kernels (dot products and updates against the basis), large bases only
+  double t_ortho = 0.0, ortho_bytes = 0.0;
   int iterations = 0, restarts = 0;
 
   size_t es() const { return elem_size(prec); }
@@ -1305,10 +1309,29 @@ struct TRLanczos {
     coef_d.ensure((size_t)(2 * k + 2) * sizeof(double));
     double* nn_d = coef_d.as<double>() + 2 * k;
     std::vector<double> hc(2 * (size_t)k + 2, 0.0);
+    // device time of the kernels between two events, and their algorithmic
+    // bytes: per pass the basis and w read by the dot products, then again by
+    // the update, which writes w (plus one read of w by its norm)
+    const bool timed = n > (1 << 22);
+    auto t_begin = [&] {
+      if (timed) HIP_OR_THROW(hipEventRecord(ctx->ev_ortho[0], ctx->stream));
+    };
+    auto t_end = [&](double vectors) {
+      if (!timed) return;
+      HIP_OR_THROW(hipEventRecord(ctx->ev_ortho[1], ctx->stream));
+      HIP_OR_THROW(hipEventSynchronize(ctx->ev_ortho[1]));
+      float ms = 0.f;
+      HIP_OR_THROW(hipEventElapsedTime(&ms, ctx->ev_ortho[0], ctx->ev_ortho[1]));
+      t_ortho += 1e-3 * ms;
+      ortho_bytes += vectors * (double)n * (double)es();
+    };
+    const double pass_vectors = 2.0 * (k + 1) + 1.0;
+    t_begin();
     if (adaptive) HIP_OR_THROW(vec_dot(prec, n, w, w, part, nn_d, ctx->stream));
     HIP_OR_THROW(vec_multi_dot(prec, n, k, B, ldb, w, part, coef_d.as<double>(), ctx->stream));
     HIP_OR_THROW(hipMemcpyAsync(hc.data(), coef_d.p, hc.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     HIP_OR_THROW(vec_multi_axpy(prec, n, k, B, ldb, coef_d.as<double>(), w, ctx->stream));
+    t_end(pass_vectors + (adaptive ? 1.0 : 0.0));
     ctx->sync();
     std::copy(hc.begin(), hc.begin() + 2 * (size_t)k, c1.begin());
     bool again = !adaptive;
@@ -1320,8 +1343,10 @@ struct TRLanczos {
     }
     if (again) {
       ++second_passes;
+      t_begin();
       HIP_OR_THROW(vec_multi_dot(prec, n, k, B, ldb, w, part, coef_d.as<double>(), ctx->stream));
       HIP_OR_THROW(vec_multi_axpy(prec, n, k, B, ldb, coef_d.as<double>(), w, ctx->stream));
+      t_end(pass_vectors);
       ctx->sync();
     }
     return c1;
@@ -1543,6 +1568,7 @@ int edhip_create(int device, edhip_ctx** out) {
     HIP_OR_THROW(hipStreamCreateWithFlags(&ctx->own, hipStreamNonBlocking));
     ctx->stream = ctx->own;
     for (auto& ev : ctx->ev) HIP_OR_THROW(hipEventCreate(&ev));
+    for (auto& ev : ctx->ev_ortho) HIP_OR_THROW(hipEventCreate(&ev));
     // this device's address of the allocation-free term pass's scratch
     HIP_OR_THROW(term_expect_noalloc_setup(device));
   });
@@ -1559,6 +1585,8 @@ void edhip_destroy(edhip_ctx* ctx) {
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
   for (auto& ev : ctx->ev)
+    if (ev) (void)hipEventDestroy(ev);
+  for (auto& ev : ctx->ev_ortho)
     if (ev) (void)hipEventDestroy(ev);
   ctx->partial.release();
   ctx->small.release();
@@ -2345,6 +2373,8 @@ int edhip_lanczos(edhip_op* op, const edhip_lanczos_opts* opts, double* evals, v
     st.matvecs = L.matvecs;
     st.second_passes = L.second_passes;
     st.t_matvec = L.t_matvec;
+    st.t_ortho = L.t_ortho;
+    st.ortho_bytes = L.ortho_bytes;
     for (int i = 0; i < nev && i < 16; ++i) st.unrefined[i] = theta[i];
 
     if (mixed) {
@@ -2388,6 +2418,8 @@ int edhip_lanczos(edhip_op* op, const edhip_lanczos_opts* opts, double* evals, v
         res[i] = rs1.empty() ? 0.0 : rs1[0];
         st.matvecs_fp64 += R.matvecs;
         st.second_passes += R.second_passes;
+        st.t_ortho += R.t_ortho;
+        st.ortho_bytes += R.ortho_bytes;
         conv = conv && ok;
       }
       st.t_refine = tr.sec();
@@ -2424,8 +2456,18 @@ int edhip_term_expect(edhip_op* op, int prec, const void* phi, const void* psi, 
     // code table leaves LDS room for the bins
     OpDev d = op->dev_row;
     if (attach_spread(op, prec, d, psi, op->ctx->stream) && term_expect_window(d, nc) < 1) d.xm = XMapDev{};
+    edhip_ctx* ctx = op->ctx;
+    if (ctx->kernel_timing) HIP_OR_THROW(hipEventRecord(ctx->ev[14], ctx->stream));
     HIP_OR_THROW(launch_term_expect(d, prec, phi, psi, op->ex_partial.as<double>(), nb, op->ex_out.as<double>(),
-                                    op->ctx->stream));
+                                    ctx->stream));
+    if (ctx->kernel_timing) {
+      HIP_OR_THROW(hipEventRecord(ctx->ev[15], ctx->stream));
+      HIP_OR_THROW(hipEventSynchronize(ctx->ev[15]));
+      float f = 0.f;
+      HIP_OR_THROW(hipEventElapsedTime(&f, ctx->ev[14], ctx->ev[15]));
+      op->kt_ms += f;
+      ++op->kt_n;
+    }
     std::vector<double> h((size_t)nt * 2);
     HIP_OR_THROW(hipMemcpyAsync(h.data(), op->ex_out.p, h.size() * 8, hipMemcpyDeviceToHost, op->ctx->stream));
     op->ctx->sync();

@@ -622,6 +622,86 @@ __global__ __launch_bounds__(kBlock) void k_term_expect(OpDev op, const V* __res
   }
 }
 
+// The off-diagonal terms of the term pass without a symmetry group (unit
+// weights), with the lanes in lock-step over a window of KT terms
+// (blockIdx.y): one lane per row (rows grid-stride over blockIdx.x), each
+// term's sum held in the lane's registers; at the end the window's sums are
+// reduced over the wave (shuffles) and over the block's waves in a fixed
+// order into partial[block][t][2].  A lane whose row does not match a term
+// idles through that term's steps, which costs less than the LDS atomics of
+// k_term_expect, whose lanes add to the bins of different terms at once.
+#ifndef EDHIP_TE_KT
+#define EDHIP_TE_KT 16
+#endif
+template <typename V, int KT>
+__global__ __launch_bounds__(kBlock) void k_term_expect_lock(OpDev op, const V* __restrict__ phi,
+                                                             const V* __restrict__ psi, double* __restrict__ partial) {
+  constexpr bool CX = VT<V>::cx;
+  extern __shared__ unsigned char lds[];
+  __shared__ double sums[kExpectWaves][KT][2];
+  lds_copy(lds + op.w_lds, op.ix.W, op.w_bytes);
+  uint16_t* codes = (uint16_t*)(lds + op.codes_lds);
+  if (op.xm.values) xmap_stage(op.xm, codes);
+  __syncthreads();
+  const uint64_t* W = (const uint64_t*)(lds + op.w_lds);
+  const int j0 = (int)blockIdx.y * KT;  // first off-diagonal term of the window
+  double ar[KT], ai[KT];
+#pragma unroll
+  for (int q = 0; q < KT; ++q) ar[q] = ai[q] = 0.0;
+  for (int64_t row = (int64_t)blockIdx.x * kBlock + threadIdx.x; row < op.dim; row += (int64_t)gridDim.x * kBlock) {
+    const uint64_t b = op.rows[row];
+    double fr, fi;
+    load_c(phi, row, fr, fi);
+    // every image's lookup and load first, then the sums
+    double sr[KT], si[KT];
+#pragma unroll
+    for (int q = 0; q < KT; ++q) {
+      sr[q] = si[q] = 0.0;
+      const int j = j0 + q;
+      if (j < op.n_off) {  // the same for every lane
+        const uint64_t mask = op.off[j].mask, match = op.off[j].match;
+        if ((b & mask) == match) {
+          const uint64_t s = (b & ~mask) | op.off[j].image;
+          if (op.xm.values) {
+            V xv;
+            if (xmap_fetch1(op.xm, codes, s, xv)) load_c(&xv, 0, sr[q], si[q]);
+          } else {
+            const int64_t idx = index_of(op.ix, W, s);
+            if (idx >= 0) load_c(psi, idx, sr[q], si[q]);
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < KT; ++q) {
+      ar[q] += fr * sr[q] + fi * si[q];  // conj(phi_r) psi_s
+      if constexpr (CX) ai[q] += fr * si[q] - fi * sr[q];
+    }
+  }
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+  for (int q = 0; q < KT; ++q) {
+    double a = ar[q], c = ai[q];
+    for (int o = 32; o > 0; o >>= 1) {
+      a += __shfl_xor(a, o, 64);
+      if constexpr (CX) c += __shfl_xor(c, o, 64);
+    }
+    if (lane == 0) {
+      sums[wave][q][0] = a;
+      sums[wave][q][1] = CX ? c : 0.0;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < 2 * KT) {
+    const int q = threadIdx.x >> 1, c = threadIdx.x & 1;
+    if (j0 + q < op.n_off) {
+      double v = 0.0;
+      for (int w = 0; w < kExpectWaves; ++w) v += sums[w][q][c];
+      partial[(int64_t)blockIdx.x * 2 * (op.n_diag + op.n_off) + 2 * (op.n_diag + j0 + q) + c] = v;
+    }
+  }
+}
+
 // Sum of the rows [64 c, 64 c + 64) of partial (n_parts x width) for 64
 // columns per block: inter[c * width + col], rows added in order.
 __global__ __launch_bounds__(64) void k_sum_parts_chunks(const double* __restrict__ partial, int64_t n_parts, int width,
@@ -656,8 +736,9 @@ __global__ __launch_bounds__(kBlock) void k_diag_expect(OpDev op, const V* __res
                                                         const V* __restrict__ psi, double* __restrict__ partial,
                                                         int width) {
   constexpr bool CX = VT<V>::cx;
-  __shared__ uint64_t rb[kBlock];
-  __shared__ double rv[kBlock], ri[kBlock];
+  __shared__ __align__(16) uint64_t rb[kBlock];
+  __shared__ __align__(16) double rv[kBlock];
+  __shared__ __align__(16) double ri[kBlock];
   for (int tt = 0; tt < op.n_diag; tt += kBlock) {
     const int t = tt + (int)threadIdx.x;
     uint64_t m = 0, c = 1;  // no term: (b & 0) != 1, never matches
@@ -679,11 +760,21 @@ __global__ __launch_bounds__(kBlock) void k_diag_expect(OpDev op, const V* __res
       }
       __syncthreads();
       const int nr = op.dim - r0 < kBlock ? (int)(op.dim - r0) : kBlock;
-      for (int j = 0; j < nr; ++j) {
-        if ((rb[j] & m) == c) {
-          ar += rv[j];
-          if constexpr (CX) ai += ri[j];
+      int j = 0;
+      for (; j + 1 < nr; j += 2) {  // two rows per 16-byte LDS read
+        const ulonglong2 bb = *(const ulonglong2*)(rb + j);
+        const double2 vv = *(const double2*)(rv + j);
+        ar += ((bb.x & m) == c) ? vv.x : 0.0;
+        ar += ((bb.y & m) == c) ? vv.y : 0.0;
+        if constexpr (CX) {
+          const double2 wi = *(const double2*)(ri + j);
+          ai += ((bb.x & m) == c) ? wi.x : 0.0;
+          ai += ((bb.y & m) == c) ? wi.y : 0.0;
         }
+      }
+      if (j < nr && (rb[j] & m) == c) {
+        ar += rv[j];
+        if constexpr (CX) ai += ri[j];
       }
     }
     if (t < op.n_diag) {
@@ -706,7 +797,17 @@ hipError_t launch_expect_t(const OpDev& op, const void* phi, const void* psi, do
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
-  for (int t0 = op.n_diag; t0 < n_terms; t0 += nw_max) {
+  // without a symmetry group (unit weights, no canonical form): the lanes take the terms in lock-step
+  const bool own = MODE == kGroupNone;
+  if (own && op.n_off > 0) {
+    constexpr int KT = VT<V>::cx ? EDHIP_TE_KT / 2 : EDHIP_TE_KT;
+    const dim3 grid((unsigned)n_blocks, (unsigned)((op.n_off + KT - 1) / KT));
+    hipLaunchKernelGGL((k_term_expect_lock<V, KT>), grid, dim3(kBlock), expect_bins_offset(op), st, op,
+                       (const V*)phi, (const V*)psi, d_partial);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  for (int t0 = op.n_diag; !own && t0 < n_terms; t0 += nw_max) {
     const int nw = n_terms - t0 < nw_max ? n_terms - t0 : nw_max;
     const uint32_t lds = expect_bins_offset(op) + (uint32_t)(kExpectWaves * nw * (VT<V>::cx ? 2 : 1) * 8);
     hipLaunchKernelGGL((k_term_expect<V, MODE>), dim3((unsigned)n_blocks), dim3(kBlock), lds, st, op, (const V*)phi,

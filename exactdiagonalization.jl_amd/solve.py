@@ -34,6 +34,8 @@ class EigsResult:
     t_refine: float
     t_alloc: float = 0.0
     second_passes: int = 0
+    t_ortho: float = 0.0
+    ortho_bytes: float = 0.0
 
 
 def eigs(opr, nev: int = 1, which: str = "SR", *, mode: str = "fp64", tol: float = 1e-10, max_basis: int = 120,
@@ -49,7 +51,9 @@ def eigs(opr, nev: int = 1, which: str = "SR", *, mode: str = "fp64", tol: float
     (FP64 Rayleigh quotient and residual; more FP64 Lanczos steps only when
     the residual bound does not hold yet).  Start vector: SplitMix64(seed) +
     Box-Muller Gaussian.  ``t_total`` excludes the one-time allocation of the
-    Krylov workspace, reported as ``t_alloc``.
+    Krylov workspace, reported as ``t_alloc``.  For vectors over 2^22
+    elements ``t_ortho`` is the device time of the orthogonalisation kernels
+    against the basis and ``ortho_bytes`` their algorithmic bytes.
 
     criterion="residual": converged when ||H y - theta y|| <= tol*max(1,|theta|).
     criterion="eigenvalue": also when r^2/gap <= tol*max(1,|theta|) with theta
@@ -109,6 +113,8 @@ def eigs(opr, nev: int = 1, which: str = "SR", *, mode: str = "fp64", tol: float
         t_matvec=st.t_matvec,
         t_refine=st.t_refine,
         t_alloc=st.t_alloc,
+        t_ortho=st.t_ortho,
+        ortho_bytes=st.ortho_bytes,
     )
 
 

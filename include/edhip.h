@@ -301,6 +301,11 @@ typedef struct {
   double t_refine;         /* seconds in the FP64 refinement */
   double t_alloc;          /* seconds getting the Krylov workspace (0 when a
                               previous call's workspace is reused) */
+  double t_ortho;          /* device seconds (HIP events) of the
+                              orthogonalisation kernels against the basis,
+                              counted for vectors over 2^22 elements */
+  double ortho_bytes;      /* their algorithmic bytes: per pass (2k + 3)
+                              vectors of n elements for k basis vectors */
 } edhip_lanczos_stats;
 
 void edhip_lanczos_default_opts(edhip_lanczos_opts* opts);
@@ -350,11 +355,12 @@ int edhip_vec_convert(edhip_ctx* ctx, int prec_in, int prec_out, int64_t n, cons
 int edhip_event_record(edhip_ctx* ctx, int slot); /* slot 0..15 (14, 15 taken while kernel timing is on) */
 int edhip_event_elapsed(edhip_ctx* ctx, int slot0, int slot1, double* ms);
 /* Measurement (no counterpart in the reference): with timing on, every
- * term-list matvec kernel launch of the context is bracketed by events
- * (slots 14 and 15) and waited for, and its duration is added to its
- * operator's total, which edhip_op_kernel_time reports (and clears when
- * reset != 0).  The steps before the kernel (e.g. spreading x) are not
- * included. */
+ * term-list matvec kernel launch and every fused term pass (edhip_term_expect:
+ * its kernel and the reduction of its partial sums) of the context is
+ * bracketed by events (slots 14 and 15) and waited for, and its duration is
+ * added to its operator's total, which edhip_op_kernel_time reports (and
+ * clears when reset != 0).  The steps before the kernel (e.g. spreading x)
+ * are not included. */
 int edhip_set_kernel_timing(edhip_ctx* ctx, int on);
 int edhip_op_kernel_time(edhip_op* op, int reset, double* ms_total, int64_t* launches);
 

@@ -245,6 +245,60 @@ def test_term_expectations_reduced_vs_oracle(ed, oracle, k, dtype, lookup):
         assert abs(got[t] - ref) <= tol * scale, (t, got[t], ref)
 
 
+@pytest.mark.parametrize("dtype", ["complex128", "complex64", "float64", "float32"])
+@pytest.mark.parametrize("space,lookup", [("sector", "auto"), ("sector", "search"), ("two_sectors", "auto"),
+                                          ("list", "auto"), ("list", "search")])
+def test_term_expectations_unreduced_vs_oracle(ed, oracle, dtype, space, lookup):
+    """The per-term pass without a symmetry group (the lanes take the
+    off-diagonal terms in lock-step): a 14-site chain with random XXZ bonds,
+    next-nearest bonds and a longitudinal and transverse field (images that
+    leave the space), on the Sz = 0 sector (rank arithmetic, with and without
+    the spread of psi), on the quantum numbers {0, 2} (two popcounts) and on a random
+    list of words (sorted search, bitmap rank index); every term's
+    <phi|T_t|psi> (unit amplitude) against the oracle's representation of
+    that term alone."""
+    import torch
+
+    n = 14
+    hs, s = ed.spin_half_system(n)
+    rng = np.random.default_rng(23)
+    bonds = [(i, (i + 1) % n) for i in range(n)] + [(i, (i + 2) % n) for i in range(n)]
+    H = ed.simplify(sum([ed.heisenberg_bond(hs, i, j, rng.uniform(0.5, 1.5), rng.uniform(0.8, 1.2)) for i, j in bonds]
+                        + [0.3 * s(i, "z") + 0.15 * s(i, "x") for i in range(n)], ed.NullOperator()))
+    if space == "sector":
+        rep = ed.represent(ed.HilbertSpaceSector(hs, 0))
+        basis = oracle.sector_basis(n, 0)
+    elif space == "two_sectors":
+        rep = ed.represent(ed.HilbertSpaceSector(hs, [0, 2]))
+        basis = np.sort(np.concatenate([oracle.sector_basis(n, 0), oracle.sector_basis(n, 2)]))
+    else:
+        basis = np.sort(rng.choice(1 << n, 5000, replace=False).astype(np.uint64))
+        rep = ed.represent(hs, basis)
+    assert np.array_equal(rep.basis_list, basis)
+    opr = ed.represent(rep, H)
+    opr.set_strategy("terms", lookup)
+    D = opr.dimension
+    phi = rng.standard_normal(D) + 1j * rng.standard_normal(D)
+    psi = rng.standard_normal(D) + 1j * rng.standard_normal(D)
+    if not dtype.startswith("complex"):
+        phi, psi = phi.real.copy(), psi.real.copy()
+    tdt = getattr(torch, dtype)
+    phi_d = torch.tensor(phi, dtype=tdt, device="cuda")
+    psi_d = torch.tensor(psi, dtype=tdt, device="cuda")
+    got = ed.term_expectations(opr, phi_d, psi_d)
+    phi_c = phi_d.cpu().numpy().astype(np.complex128)
+    psi_c = psi_d.cpu().numpy().astype(np.complex128)
+    mask, row, col, _ = ed.terms_arrays(H)
+    assert (row != col).sum() > 16  # more than one window of off-diagonal terms
+    tol = 1e-12 if dtype in ("complex128", "float64") else 1e-5
+    scale = np.linalg.norm(phi_c) * np.linalg.norm(psi_c)
+    assert got.size == mask.size
+    for t in range(mask.size):
+        o = oracle.Opr(basis, (mask[t:t + 1], row[t:t + 1], col[t:t + 1], np.ones(1, complex)))
+        ref = np.vdot(phi_c, o.matvec(psi_c))
+        assert abs(got[t] - ref) <= tol * scale, (t, got[t], ref)
+
+
 def test_couplings_on_reduced_space(ed):
     """On a translation sector, per-bond couplings are rejected (their reduced
     expectations would not be the physical ones); translation-invariant sums
