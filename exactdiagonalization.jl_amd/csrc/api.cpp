@@ -308,7 +308,7 @@ struct edhip_space {
   mutable DevBuf xtop;  // code of each value of the top bits of the words
   mutable int xtop_bits = -1, xtop_shift = 0, xtop_K = 0;
   mutable uint64_t xtop_codes = 0;
-  mutable DevBuf xcolex;  // colex tables of the low 12 bits (single-popcount spaces)
+  mutable DevBuf xcolex;  // colex rank of every 12-bit word (single-popcount spaces)
   mutable bool xcolex_on = false;
 
   // The popcount every word of the space has, or -1: a popcount sector with
@@ -353,7 +353,7 @@ struct edhip_space {
       for (size_t v = 0; v < fl.size(); ++v)
         if (fl[v]) codes[v] = (uint16_t)nc++;
       upload(xtop, codes, ctx->stream);
-      if (xcolex_on) upload(xcolex, colex12_tables(), ctx->stream);
+      if (xcolex_on) upload(xcolex, colex12_full(), ctx->stream);
       xtop_bits = t;
       xtop_shift = sh;
       xtop_K = K;
@@ -861,6 +861,7 @@ struct edhip_op {
   std::vector<uint64_t> mask, row, col;
   std::vector<double> are, aim;
   bool complex_amp = false;
+  bool keeps_popcount = true;  // every off-diagonal term's image has its row's popcount
   // device terms
   DevBuf off_row, off_col, diag, all_row, all_col;
   DevBuf mv_row, mv_col;  // paired copies of off_row / off_col for the matvec
@@ -961,8 +962,8 @@ OpDev make_opdev(edhip_op* op, bool transpose) {
   d.lds_total = align_up(off, 16);
   // the matvec alone stages the code table of the spread x after the rest
   d.xm = XMapDev{};
-  // (room for the largest code table and the colex tables)
-  d.codes_lds = d.lds_total + (2u << kXTopBits) + 2u * (64u + 64u * 7u) <= 64 * 1024 ? d.lds_total : 0;
+  // (room for the largest code table and the colex rank table)
+  d.codes_lds = d.lds_total + (2u << kXTopBits) + 2u * 4096u <= 64 * 1024 ? d.lds_total : 0;
   if (d.lds_total > 64 * 1024)
     fail(EDHIP_ERR_ARGUMENT, "operator and symmetry tables exceed the 64 KiB LDS budget of the matvec kernels (" +
                                  std::to_string(d.lds_total) + " bytes)");
@@ -1131,6 +1132,9 @@ bool attach_spread(edhip_op* op, int prec, OpDev& d, const void* x, hipStream_t 
   if (!X) return false;
   HIP_OR_THROW(launch_xmap_fill(prec, X->dev, sp->keys.as<uint64_t>(), sp->dim, x, st));
   d.xm = X->dev;
+  // every image of a term that keeps the popcount has the space's popcount:
+  // the products skip the test
+  if (op->keeps_popcount) d.xm.pop = -1;
   return true;
 }
 
@@ -2042,6 +2046,7 @@ int edhip_op_create(edhip_ctx* ctx, const edhip_space* space, int64_t n_terms, c
         dg.push_back(DiagTerm{mask[t], row[t], are[t], aim[t]});
         op->diag_ids.push_back(t);
       } else {
+        if (__builtin_popcountll(row[t]) != __builtin_popcountll(col[t])) op->keeps_popcount = false;
         offr.push_back(OffTerm{mask[t], row[t], col[t], are[t], aim[t]});
         offc.push_back(OffTerm{mask[t], col[t], row[t], are[t], aim[t]});
         op->off_ids.push_back(t);

@@ -384,22 +384,27 @@ __device__ __forceinline__ void canonicalize_torus_pairs_t(const GroupDev& g, co
   // row Ly-1 copied below row 0: the window at y*Lx holds (row_y << Lx) | row_{y-1}
   const uint64_t w = (b << Lx) | (b >> (NB - Lx));
   uint32_t e[8];
-  uint32_t best = 0xffffffffu;
+  uint32_t lo = 0xffffffffu;  // the smallest entry: its value part is the smallest pair
 #pragma unroll
   for (int y = 0; y < 8; ++y) {
     e[y] = 0xffffffffu;
     if (y < Ly) {
       e[y] = L.pair[(uint32_t)(w >> (y * Lx)) & m2];
-      best = min(best, e[y] | 0xffu);
+      lo = min(lo, e[y]);
     }
   }
-  uint32_t cand_lo = 0, cand_hi = 0;
+  // an entry of the smallest value, less that value, is its rotation mask
+  // (below 256); every other entry is at least 256 above: min(e - base, 256)
+  // leaves the mask or 0 in the low byte, and the low bytes of the eight rows
+  // are packed into the candidate mask (v_perm_b32, selectors 0x0c: zero)
+  const uint32_t base = lo & ~0xffu;
+  uint32_t m[8];
 #pragma unroll
-  for (int y = 0; y < 4; ++y)
-    cand_lo |= ((e[y] | 0xffu) == best ? (e[y] & 0xffu) : 0u) << (8 * y);
-#pragma unroll
-  for (int y = 4; y < 8; ++y)
-    cand_hi |= ((e[y] | 0xffu) == best ? (e[y] & 0xffu) : 0u) << (8 * (y - 4));
+  for (int y = 0; y < 8; ++y) m[y] = min(e[y] - base, 256u);
+  const uint32_t cand_lo = __builtin_amdgcn_perm(m[1], m[0], 0x0c0c0400u) |
+                           (__builtin_amdgcn_perm(m[3], m[2], 0x0c0c0400u) << 16);
+  const uint32_t cand_hi = __builtin_amdgcn_perm(m[5], m[4], 0x0c0c0400u) |
+                           ((LYC > 0 && LYC <= 6) ? 0u : (__builtin_amdgcn_perm(m[7], m[6], 0x0c0c0400u) << 16));
   uint64_t cand = ((uint64_t)cand_hi << 32) | cand_lo;
   s = ~0ull;
   cnt = 0;
@@ -899,36 +904,35 @@ __device__ __forceinline__ uint32_t hash_word(uint64_t s) {
 struct XMapDev {
   void* values;             // null: not used
   const uint16_t* topcode;  // 2^top_bits codes (global memory; the kernels copy it to LDS)
-  const uint16_t* colex;    // colex12 tables (global; copied to LDS after the codes), or null
+  const uint16_t* colex;    // colex rank of every 12-bit word, u16[4096] (global; copied to LDS after the codes), or null
   int bm_bits;              // words must be below 2^bm_bits
   int top_bits, top_shift;
   int hbits;                // colex: top_shift - 12
-  int pop;                  // colex: the popcount of every word (a word of another popcount has no position)
+  int pop;                  // colex: the popcount of every word (a word of another popcount has no position);
+                            // -1 where every image keeps the popcount (no test needed)
   int pad_;
 };
 
 // LDS bytes of the code table and, when used, the colex tables after it.
 __host__ __device__ __forceinline__ uint32_t xmap_lds_bytes(const XMapDev& X) {
-  return (2u << X.top_bits) + (X.colex ? 2u * (64u + 64u * 7u) : 0u);
+  return (2u << X.top_bits) + (X.colex ? 2u * 4096u : 0u);
 }
 
-// Position of word k whose top bits have code c; cx: the colex tables (LDS
-// or global), used when X.colex is set.
+// Position of word k whose top bits have code c; cx: the colex rank table
+// (LDS or global), used when X.colex is set.
 __device__ __forceinline__ uint64_t xmap_pos(const XMapDev& X, const uint16_t* cx, uint32_t c, uint64_t k) {
   if (X.colex) {
-    const uint32_t l = (uint32_t)k & 0xfffu;
     const uint64_t h = (k >> 12) & ((1ull << X.hbits) - 1ull);
-    const uint32_t r = (uint32_t)cx[l & 63u] + (uint32_t)cx[64u + (l >> 6) * 7u + (uint32_t)__builtin_popcount(l & 63u)];
-    return ((((uint64_t)c << X.hbits) | h) * 924ull) + r;
+    return ((((uint64_t)c << X.hbits) | h) * 924ull) + (uint32_t)cx[(uint32_t)k & 0xfffu];
   }
   return ((uint64_t)c << X.top_shift) | (k & ((1ull << X.top_shift) - 1ull));
 }
 
 // x element of one canonical image k (false: the word has no position).
-// codes: the code table in LDS, followed by the colex tables when used.
+// codes: the code table in LDS, followed by the colex rank table when used.
 template <typename V>
 __device__ __forceinline__ bool xmap_fetch1(const XMapDev& X, const uint16_t* codes, uint64_t k, V& x) {
-  const bool in = (k >> X.bm_bits) == 0 && (!X.colex || __builtin_popcountll(k) == X.pop);
+  const bool in = (k >> X.bm_bits) == 0 && (!X.colex || X.pop < 0 || __builtin_popcountll(k) == X.pop);
   const uint32_t c = codes[in ? (uint32_t)(k >> X.top_shift) : 0u];
   const bool f = in && c != 0xffffu;
   x = ((const V*)X.values)[f ? xmap_pos(X, codes + (1u << X.top_bits), c, k) : 0ull];
@@ -942,8 +946,11 @@ template <typename V>
 __device__ __forceinline__ void xmap_fetch2(const XMapDev& X, const uint16_t* codes, uint64_t k0, uint64_t k1, V& x0,
                                             V& x1, bool& f0, bool& f1) {
   // (colex positions are unique within one popcount only)
-  const bool in0 = (k0 >> X.bm_bits) == 0 && (!X.colex || __builtin_popcountll(k0) == X.pop);
-  const bool in1 = (k1 >> X.bm_bits) == 0 && (!X.colex || __builtin_popcountll(k1) == X.pop);
+  bool in0 = (k0 >> X.bm_bits) == 0, in1 = (k1 >> X.bm_bits) == 0;
+  if (X.colex && X.pop >= 0) {  // the same branch in every lane
+    in0 = in0 && __builtin_popcountll(k0) == X.pop;
+    in1 = in1 && __builtin_popcountll(k1) == X.pop;
+  }
   const uint32_t c0 = codes[in0 ? (uint32_t)(k0 >> X.top_shift) : 0u];
   const uint32_t c1 = codes[in1 ? (uint32_t)(k1 >> X.top_shift) : 0u];
   f0 = in0 && c0 != 0xffffu;
@@ -958,7 +965,7 @@ __device__ __forceinline__ void xmap_fetch2(const XMapDev& X, const uint16_t* co
 // Stages the code table and the colex tables of X into LDS at codes.
 __device__ __forceinline__ void xmap_stage(const XMapDev& X, uint16_t* codes) {
   lds_copy((unsigned char*)codes, X.topcode, 2u << X.top_bits);
-  if (X.colex) lds_copy((unsigned char*)(codes + (1u << X.top_bits)), X.colex, 2u * (64u + 64u * 7u));
+  if (X.colex) lds_copy((unsigned char*)(codes + (1u << X.top_bits)), X.colex, 2u * 4096u);
 }
 
 struct HashDev {

@@ -189,4 +189,14 @@ inline std::vector<uint16_t> colex12_tables() {
   return t;
 }
 
+// The rank of every 12-bit word (4096 entries) from the two halves above:
+// one table read per word on the device.
+inline std::vector<uint16_t> colex12_full() {
+  const std::vector<uint16_t> t = colex12_tables();
+  std::vector<uint16_t> f(4096);
+  for (uint32_t l = 0; l < 4096; ++l)
+    f[l] = (uint16_t)(t[l & 63u] + t[64u + (l >> 6) * 7u + (uint32_t)__builtin_popcount(l & 63u)]);
+  return f;
+}
+
 }  // namespace edhip

@@ -154,6 +154,11 @@ static void test_colex() {
     }
     for (int r = 0; r < edhip::kColexStride; ++r) CHECK(seen[r] == (r < n ? 1 : 0));
   }
+  // the device's one-read table holds the same ranks
+  const std::vector<uint16_t> f = edhip::colex12_full();
+  CHECK(f.size() == 4096);
+  for (uint32_t l = 0; l < 4096; ++l)
+    CHECK(f[l] == t[l & 63u] + t[64u + (l >> 6) * 7u + (uint32_t)__builtin_popcount(l & 63u)]);
 }
 
 static void test_oracle() {
