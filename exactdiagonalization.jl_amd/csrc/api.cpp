@@ -337,21 +337,30 @@ struct edhip_space {
       const int K = std::max(4, bit_length(max_key));
       // colex positions for the low 12 bits when every word has one popcount
       // (the popcount of those bits is then fixed by the rest of the word);
-      // the top code covers what lies above bit 11 (at most kXTopBits bits)
-      xcolex_on = single_popcount() >= 0 && K >= 16;
-      const int t = xcolex_on ? std::min(kXTopBits, K - 12) : std::min(kXTopBits, K);
-      const int sh = K - t;
-      DevBuf flags;
-      flags.alloc(((size_t)1 << t) * 4);
-      HIP_OR_THROW(hipMemsetAsync(flags.p, 0, ((size_t)1 << t) * 4, ctx->stream));
-      HIP_OR_THROW(launch_mark_tops(keys.as<uint64_t>(), dim, sh, flags.as<uint32_t>(), ctx->stream));
-      std::vector<uint32_t> fl((size_t)1 << t);
-      HIP_OR_THROW(hipMemcpyAsync(fl.data(), flags.p, fl.size() * 4, hipMemcpyDeviceToHost, ctx->stream));
-      ctx->sync();
-      std::vector<uint16_t> codes(fl.size(), 0xffff);
+      // the top code covers what lies above bit 11 (at most kXTopBits bits).
+      // The colex form keeps its positions in 32 bits (the device computes
+      // them so): words of at most 44 bits and fewer than 2^32 positions,
+      // else the plain form.
+      std::vector<uint16_t> codes;
       uint64_t nc = 0;
-      for (size_t v = 0; v < fl.size(); ++v)
-        if (fl[v]) codes[v] = (uint16_t)nc++;
+      int t = 0, sh = 0;
+      for (int pass = 0; pass < 2; ++pass) {
+        xcolex_on = pass == 0 && single_popcount() >= 0 && K >= 16 && K <= 44;
+        t = xcolex_on ? std::min(kXTopBits, K - 12) : std::min(kXTopBits, K);
+        sh = K - t;
+        DevBuf flags;
+        flags.alloc(((size_t)1 << t) * 4);
+        HIP_OR_THROW(hipMemsetAsync(flags.p, 0, ((size_t)1 << t) * 4, ctx->stream));
+        HIP_OR_THROW(launch_mark_tops(keys.as<uint64_t>(), dim, sh, flags.as<uint32_t>(), ctx->stream));
+        std::vector<uint32_t> fl((size_t)1 << t);
+        HIP_OR_THROW(hipMemcpyAsync(fl.data(), flags.p, fl.size() * 4, hipMemcpyDeviceToHost, ctx->stream));
+        ctx->sync();
+        codes.assign(fl.size(), 0xffff);
+        nc = 0;
+        for (size_t v = 0; v < fl.size(); ++v)
+          if (fl[v]) codes[v] = (uint16_t)nc++;
+        if (!xcolex_on || ((nc << (sh - 12)) * (uint64_t)kColexStride >> 32) == 0) break;
+      }
       upload(xtop, codes, ctx->stream);
       if (xcolex_on) upload(xcolex, colex12_full(), ctx->stream);
       xtop_bits = t;

@@ -922,8 +922,12 @@ __host__ __device__ __forceinline__ uint32_t xmap_lds_bytes(const XMapDev& X) {
 // (LDS or global), used when X.colex is set.
 __device__ __forceinline__ uint64_t xmap_pos(const XMapDev& X, const uint16_t* cx, uint32_t c, uint64_t k) {
   if (X.colex) {
-    const uint64_t h = (k >> 12) & ((1ull << X.hbits) - 1ull);
-    return ((((uint64_t)c << X.hbits) | h) * 924ull) + (uint32_t)cx[(uint32_t)k & 0xfffu];
+    // (c 2^hbits + bits 12 .. 11 + hbits) 924 + the rank of the low 12 bits,
+    // in 32 bits: the host keeps the colex form to words of at most 44 bits
+    // (hbits <= 20) and spreads of fewer than 2^32 positions
+    const uint32_t lo = (uint32_t)k;
+    const uint32_t hc = (c << X.hbits) | __builtin_amdgcn_ubfe(lo, 12u, (uint32_t)X.hbits);
+    return (uint64_t)(__umul24(hc, 924u) + (uint32_t)cx[lo & 0xfffu]);
   }
   return ((uint64_t)c << X.top_shift) | (k & ((1ull << X.top_shift) - 1ull));
 }
