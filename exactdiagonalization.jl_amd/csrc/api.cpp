@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <complex>
@@ -56,6 +57,19 @@ inline bool prec_is_complex(int prec) { return prec == EDHIP_C128 || prec == EDH
   } while (0)
 
 [[noreturn]] void fail(int code, const std::string& msg) { throw Error{code, msg}; }
+
+// Injected allocation failures (edhip_debug_fail_allocs): the next
+// g_fail_count allocations of the kinds in g_fail_kinds fail as if the
+// device were out of memory, before any device call is made.
+constexpr int kAllocSpread = 1, kAllocWorkspace = 2;
+std::atomic<int> g_fail_kinds{0}, g_fail_count{0};
+void injected_alloc_failure(int kind) {
+  if (!(g_fail_kinds.load() & kind)) return;
+  int c = g_fail_count.load();
+  while (c > 0 && !g_fail_count.compare_exchange_weak(c, c - 1)) {
+  }
+  if (c > 0) fail(EDHIP_ERR_MEMORY, "injected allocation failure");
+}
 
 // Device buffer owned by a handle.
 struct DevBuf {
@@ -111,6 +125,7 @@ struct edhip_ctx {
   bool kernel_timing = false;  // time every matvec kernel launch (edhip_set_kernel_timing)
 
   char* krylov_arena(size_t bytes) {
+    if (krylov.bytes < bytes) injected_alloc_failure(kAllocWorkspace);
     krylov.ensure(bytes);
     return krylov.as<char>();
   }
@@ -382,6 +397,7 @@ struct edhip_space {
       const uint64_t n_pos = ensure_xtop();
       // the spread may take up to kXMapRatio times the memory of the vector
       if (n_pos > (uint64_t)kXMapRatio * (uint64_t)dim && n_pos > (1ull << 20)) return nullptr;
+      injected_alloc_failure(kAllocSpread);
       X.values.alloc(n_pos * es);
       HIP_OR_THROW(hipMemsetAsync(X.values.p, 0, n_pos * es, ctx->stream));
       ctx->sync();
@@ -2661,6 +2677,14 @@ int edhip_vec_convert(edhip_ctx* ctx, int pi, int po, int64_t n, const void* x, 
 int edhip_event_record(edhip_ctx* ctx, int slot) {
   if (!ctx || slot < 0 || slot >= 16) return EDHIP_ERR_ARGUMENT;
   return guarded(ctx, [&] { HIP_OR_THROW(hipEventRecord(ctx->ev[slot], ctx->stream)); });
+}
+
+int edhip_debug_fail_allocs(int kinds, int count) {
+  if (kinds < 0 || kinds > (kAllocSpread | kAllocWorkspace) || count < 0) return EDHIP_ERR_ARGUMENT;
+  g_fail_count.store(0);
+  g_fail_kinds.store(kinds);
+  g_fail_count.store(count);
+  return EDHIP_OK;
 }
 
 int edhip_set_kernel_timing(edhip_ctx* ctx, int on) {

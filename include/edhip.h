@@ -354,6 +354,17 @@ int edhip_vec_convert(edhip_ctx* ctx, int prec_in, int prec_out, int64_t n, cons
 /* Timing on the context's stream (hipEvent pairs). */
 int edhip_event_record(edhip_ctx* ctx, int slot); /* slot 0..15 (14, 15 taken while kernel timing is on) */
 int edhip_event_elapsed(edhip_ctx* ctx, int slot0, int slot1, double* ms);
+/* Testing (no counterpart in the reference): the next `count` device
+ * allocations of the given kinds fail as if the device were out of memory
+ * (process-wide; count 0 turns it off).  kinds: bit 0 the spread of an
+ * input vector over the word positions (the products then read x through
+ * the rank index or the sorted search), bit 1 the eigensolver's Krylov
+ * workspace (on a failure the spreads are freed and the allocation tried
+ * once more; a second failure returns EDHIP_ERR_MEMORY). */
+#define EDHIP_ALLOC_SPREAD 1
+#define EDHIP_ALLOC_WORKSPACE 2
+int edhip_debug_fail_allocs(int kinds, int count);
+
 /* Measurement (no counterpart in the reference): with timing on, every
  * term-list matvec kernel launch and every fused term pass (edhip_term_expect:
  * its kernel and the reduction of its partial sums) of the context is

@@ -2,6 +2,8 @@
 
 Tolerances: FP64 / ComplexF64 vectors 1e-12 relative (2-norm); FP32 /
 ComplexF32 vectors 1e-5 relative (single-precision products and sums)."""
+import math
+
 import numpy as np
 import pytest
 
@@ -220,3 +222,36 @@ def test_hermiticity_and_linearity(ed, systems):
         Hx, Hy = opr @ x, opr @ y
         assert abs(np.vdot(y, Hx) - np.vdot(Hy, x)) < 1e-11 * np.linalg.norm(x) * np.linalg.norm(y) * 10
         assert rel(opr @ (2.0 * x - 3.0j * y), 2.0 * Hx - 3.0j * Hy) < 1e-13
+
+
+@pytest.mark.parametrize("n,p", [(20, 3), (30, 2), (40, 2), (48, 2)])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_wide_words_few_particles(ed, oracle, n, p, dtype):
+    """Sectors of few up spins on wide words (K = 20 ... 48 bits): the spread
+    of x takes its colex form where the positions fit in 32 bits, the plain
+    form or none otherwise (then the rank index or the sorted search); an
+    XY ring with next-nearest hopping and random fields (every image keeps
+    the popcount: the products skip the popcount test) and the same plus a
+    transverse field on two sites (images of another popcount, dropped), in
+    the row and column forms, against the oracle."""
+    hs, s = ed.spin_half_system(n)
+    rng = np.random.default_rng(n + p)
+    hop = [(i, (i + 1) % n) for i in range(n)] + [(i, (i + 3) % n) for i in range(n)]
+    H = ed.simplify(sum([ed.heisenberg_bond(hs, i, j, rng.uniform(0.5, 1.5), 0.0) for i, j in hop]
+                        + [rng.uniform(-1, 1) * s(i, "z") for i in range(n)], ed.NullOperator()))
+    Hx = ed.simplify(H + 0.7 * s(1, "x") + 0.4 * s(n - 2, "x"))
+    hsr = ed.represent(ed.HilbertSpaceSector(hs, 2 * p - n))
+    basis = np.array([int(b) for b in hsr.basis_list], dtype=np.uint64)
+    assert basis.size == math.comb(n, p)
+    x = rng.standard_normal(basis.size).astype(dtype)
+    for op in (H, Hx):
+        opr = ed.represent(hsr, op)
+        o = oracle.Opr(basis, ed.terms_arrays(op))
+        for transpose in (False, True):
+            ref = o.apply(np.zeros(basis.size, complex), x.astype(complex), transpose=transpose)
+            out = np.zeros(basis.size, dtype)
+            if transpose:
+                ed.mul_(out, x, opr)
+            else:
+                ed.mul_(out, opr, x)
+            assert rel(out, ref) < TOL[dtype], (n, p, transpose)
