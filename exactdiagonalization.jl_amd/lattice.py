@@ -93,29 +93,73 @@ def point_perm(L: int, m) -> SitePermutation:
     return SitePermutation(out)
 
 
+# One-dimensional irreps of the little groups of C4v on a torus, by name.
+# C4v: characters on its classes.  C2v (at (L/2, 0) and (0, L/2): C2 and the
+# two axial mirrors; a diagonal C2v would need 2k = 0 on a diagonal, where
+# the little group is all of C4v): characters on (E, C2, x -> -x, y -> -y).
+# Cs (one mirror): on (E, the mirror).  C1: the identity alone.
+_MX, _MY = ((-1, 0), (0, 1)), ((1, 0), (0, -1))
+
+
+def _little_irreps(kind):
+    if kind == "C4v":
+        return dict(_D4_IRREPS)
+    if kind == "C2v":
+        # characters on (E, C2, first mirror, second mirror)
+        return {"A1": (1, 1, 1, 1), "A2": (1, 1, -1, -1), "B1": (1, -1, 1, -1), "B2": (1, -1, -1, 1)}
+    if kind == "Cs":
+        return {"A'": (1, 1), "A''": (1, -1)}
+    return {"A": (1,)}
+
+
 class SquareSpaceIrrep:
-    """One-dimensional irreps of the space group T x| C4v of an L x L torus at
-    a momentum fixed by every point operation, k = (0, 0) or (L/2, L/2):
-    chi(T(a, b) p) = exp(-2 pi i (kx a + ky b) / L) * chi_point(p) for the
-    point irreps A1, A2, B1, B2 (the role of LatticeTools' space-symmetry
-    irrep components in examples/spinhalf_triangular.jl).  8 L^2 elements,
+    """One-dimensional irreps of the symmorphic space group T x| C4v of an
+    L x L torus at momentum k (integers, k_i in [0, L)): the elements T(a, b) p
+    with p in the little group of k (the point operations with p k = k mod L)
+    and chi(T(a, b) p) = exp(-2 pi i (kx a + ky b) / L) chi_point(p), chi_point
+    a one-dimensional irrep of the little group (the elements the reference's
+    symmorphic reduction keeps: those with a nonzero character,
+    symmetry_reduce_symmorphic.jl:27-46; LatticeTools' space-symmetry irrep
+    components in examples/spinhalf_triangular.jl).  Little groups and their
+    irreps: C4v at k = (0, 0) and (L/2, L/2) (A1, A2, B1, B2); C2v at (L/2, 0)
+    and (0, L/2) (A1, A2, B1, B2 with the characters of _little_irreps); Cs on
+    a mirror line (A', A''); C1 elsewhere (A).  |little group| L^2 elements,
     identity first."""
 
     def __init__(self, L: int, k=(0, 0), point: str = "A1"):
-        if point not in _D4_IRREPS:
-            raise ValueError("point irrep must be one of A1, A2, B1, B2")
-        k = tuple(k)
-        if any((2 * kk) % L for kk in k) or k[0] != k[1]:
-            raise ValueError("the little group of k is the whole point group only for k = (0,0) or (L/2, L/2)")
+        k = (int(k[0]) % L, int(k[1]) % L)
         self.L, self.k, self.point = L, k, point
+        self.little = [m for m in _D4 if self._fixes(m)]
+        self.kind = {8: "C4v", 4: "C2v", 2: "Cs", 1: "C1"}[len(self.little)]
+        assert self.kind != "C2v" or (_MX in self.little and _MY in self.little)
+        irreps = _little_irreps(self.kind)
+        if point not in irreps:
+            raise ValueError(f"the little group of k = {k} is {self.kind}; its irreps are {sorted(irreps)}")
+
+    def _fixes(self, m):
+        (a, b), (c, d) = m
+        kx, ky = self.k
+        L = self.L
+        return ((a * kx + b * ky - kx) % L, (c * kx + d * ky - ky) % L) == (0, 0)
+
+    def point_character(self, m) -> int:
+        """chi_point of a little-group element (a 2x2 integer matrix)."""
+        chars = _little_irreps(self.kind)[self.point]
+        if self.kind == "C4v":
+            return chars[_d4_class(m)]
+        if m == _D4[0]:
+            return chars[0]
+        if self.kind == "C2v":
+            return chars[{((-1, 0), (0, -1)): 1, _MX: 2, _MY: 3}[m]]
+        return chars[1]  # Cs: the mirror
 
     def symops_and_amplitudes(self):
         L = self.L
         tr = TranslationIrrep((L, L), self.k)
         out = []
-        for m in _D4:
+        for m in self.little:
             p = point_perm(L, m)
-            cp = _D4_IRREPS[self.point][_d4_class(m)]
+            cp = self.point_character(m)
             for b in range(L):
                 for a in range(L):
                     out.append((translation_perm((L, L), a, b) * p, tr.character(a, b) * cp))
@@ -123,7 +167,7 @@ class SquareSpaceIrrep:
 
     @property
     def group_order(self):
-        return 8 * self.L * self.L
+        return len(self.little) * self.L * self.L
 
 
 def chain_bonds(L: int):

@@ -167,6 +167,46 @@ def test_space_group_4x4_vs_oracle(ed, oracle, k, point, parity):
     assert abs(E - oracle.lowest_eigenvalues(o, 1)[0]) < 1e-11 * abs(E)
 
 
+@pytest.mark.parametrize("k,names", [((2, 0), ["A1", "A2", "B1", "B2"]), ((1, 0), ["A'", "A''"]),
+                                     ((1, 3), ["A'", "A''"])])
+def test_little_groups_4x4_vs_oracle(ed, oracle, k, names):
+    """Little groups away from (0,0) and (pi,pi) (symmetry_reduce_symmorphic.jl
+    keeps the point elements of nonzero character): C2v at (pi, 0) and Cs on
+    the axial and anti-diagonal mirror lines, translations x little group
+    (64 or 32 elements, complex characters at k = (1, 0) and (1, 3)); basis,
+    mapping and products vs the oracle's generic reduction, and the lowest
+    eigenvalue over the irreps equal to the translation sector's."""
+    n = 16
+    hs, _ = ed.spin_half_system(n)
+    j1, j2 = ed.square_bonds(4, 4)
+    H = _heis(ed, hs, j1 + j2, [1.0] * len(j1) + [0.5] * len(j2))
+    hsr = ed.represent(ed.HilbertSpaceSector(hs, 0))
+    e_irreps = []
+    for name in names:
+        ops = ed.SquareSpaceIrrep(4, k, name).symops_and_amplitudes()
+        r = ed.symmetry_reduce(hsr, ops)
+        red = _oracle_reduce(ed, oracle, hs, ops, n)
+        assert np.array_equal(r.basis_list, red.basis_list)
+        assert np.array_equal(r.basis_mapping_index, red.basis_mapping_index)
+        assert np.abs(r.basis_mapping_amplitude - red.basis_mapping_amplitude).max(initial=0) < 1e-14
+        opr = ed.represent(r, H)
+        o = oracle.Opr(red, ed.terms_arrays(H))
+        rng = np.random.default_rng(2)
+        x = rng.standard_normal(r.dimension) + 1j * rng.standard_normal(r.dimension)
+        for transpose in (False, True):
+            ref = o.apply(np.zeros(r.dimension, complex), x, transpose=transpose)
+            out = np.zeros(r.dimension, complex)
+            if transpose:
+                ed.mul_(out, x, opr)
+            else:
+                ed.mul_(out, opr, x)
+            assert np.linalg.norm(out - ref) <= 1e-13 * np.linalg.norm(ref), (name, transpose)
+        e_irreps.append(ed.eigs(opr, 1, tol=1e-12, max_basis=min(100, r.dimension)).values[0])
+    rk = ed.symmetry_reduce(hsr, ed.TranslationIrrep((4, 4), k))
+    e_k = ed.eigs(ed.represent(rk, H), 1, tol=1e-12, max_basis=100).values[0]
+    assert abs(min(e_irreps) - e_k) < 1e-10 * abs(e_k)
+
+
 @pytest.mark.parametrize("flip", [False, True])
 def test_space_group_6x6_a1(ed, flip):
     """6x6 J1-J2, k = (0,0), point irrep A1 of C4v (288 elements; 576 with the

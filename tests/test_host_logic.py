@@ -239,6 +239,73 @@ def test_square_space_irrep_characters(ed, L, k):
             assert abs(s - (G if a == b else 0)) < 1e-9
 
 
+@pytest.mark.parametrize("L,k,kind", [(4, (2, 0), "C2v"), (4, (0, 2), "C2v"), (6, (3, 0), "C2v"), (4, (1, 0), "Cs"),
+                                      (4, (1, 1), "Cs"), (4, (1, 2), "Cs"), (5, (2, 0), "Cs"), (5, (1, 2), "C1"),
+                                      (6, (1, 2), "C1")])
+def test_little_group_irreps(ed, L, k, kind):
+    """Little groups of C4v at k = (L/2, 0), on mirror lines and elsewhere:
+    every element fixes k (p k = k mod L) and the rest of C4v does not, the
+    elements T(a, b) p form a group (closed under composition, identity
+    first, each once), every irrep is a homomorphism, and the irreps are
+    orthonormal: sum_g chi_a(g) conj(chi_b(g)) = |G| delta_ab."""
+    hs, _ = ed.spin_half_system(L * L)
+    names = {"C2v": ["A1", "A2", "B1", "B2"], "Cs": ["A'", "A''"], "C1": ["A"]}[kind]
+    chis = {}
+    for name in names:
+        irr = ed.SquareSpaceIrrep(L, k, name)
+        assert irr.kind == kind
+        for m in irr.little:
+            (a, b), (c, d) = m
+            assert ((a * k[0] + b * k[1] - k[0]) % L, (c * k[0] + d * k[1] - k[1]) % L) == (0, 0)
+        ops = irr.symops_and_amplitudes()
+        assert len(ops) == irr.group_order == len(irr.little) * L * L
+        perms = np.stack([np.array(p.map) - 1 for p, _ in ops])
+        chi = np.array([complex(a) for _, a in ops])
+        assert np.array_equal(perms[0], np.arange(L * L)) and chi[0] == 1
+        index = {tuple(p): i for i, p in enumerate(perms.tolist())}
+        assert len(index) == len(ops)
+        for a in range(len(ops)):
+            for b in np.random.default_rng(a).integers(0, len(ops), 8):
+                ab = tuple(perms[a][perms[b]])  # g_a after g_b
+                assert ab in index
+                assert abs(chi[index[ab]] - chi[a] * chi[b]) < 1e-12
+        chis[name] = chi
+    G = len(chis[names[0]])
+    for a in names:
+        for b in names:
+            assert abs(np.vdot(chis[b], chis[a]) - (G if a == b else 0)) < 1e-9
+    with pytest.raises(ValueError):
+        ed.SquareSpaceIrrep(L, k, "E")
+
+
+@pytest.mark.parametrize("L,k", [(4, (2, 0)), (4, (1, 0)), (4, (1, 2)), (4, (1, 3))])
+def test_little_group_sectors_split_the_momentum_sector(ed, oracle, L, k):
+    """The symmorphic sectors of the irreps of k's little group split the
+    translation sector at k (oracle reductions of the Sz = 0 sector of the
+    L x L torus): their dimensions add up to its dimension."""
+    n = L * L
+    hs, _ = ed.spin_half_system(n)
+    parent = oracle.sector_basis(n, 0)
+
+    def dim(ops):
+        perms = np.stack([np.array(p.map) - 1 for p, _ in ops])
+        return oracle.symmetry_reduce(parent, n, perms, [complex(a) for _, a in ops]).basis_list.size
+
+    total = dim(ed.TranslationIrrep((L, L), k).symops_and_amplitudes())
+    kind = None
+    for probe in ("A1", "A'", "A"):
+        try:
+            kind = ed.SquareSpaceIrrep(L, k, probe).kind
+            break
+        except ValueError:
+            pass
+    assert kind in ("C2v", "Cs", "C1")  # abelian: every irrep one-dimensional
+    names = {"C2v": ["A1", "A2", "B1", "B2"], "Cs": ["A'", "A''"], "C1": ["A"]}[kind]
+    parts = [dim(ed.SquareSpaceIrrep(L, k, name).symops_and_amplitudes()) for name in names]
+    assert sum(parts) == total, (parts, total)
+    assert all(p > 0 for p in parts)
+
+
 # ------------------------------------------------ coupling invariance
 def _fake_reduced(ed, hs, ops):
     """A ReducedHilbertSpaceRepresentation shell carrying only its group (the
