@@ -959,6 +959,11 @@ OpDev make_opdev(edhip_op* op, bool transpose) {
   }
   d.n_off = op->n_off;
   d.n_diag = op->n_diag;
+  d.word_bits = sp->n_bits;
+  for (int i = 0; i < op->n_diag; ++i) {
+    const int64_t t = op->diag_ids[i];
+    d.word_bits = std::max({d.word_bits, bit_length(op->mask[t]), bit_length(op->row[t])});
+  }
   d.off = transpose ? op->off_col.as<OffTerm>() : op->off_row.as<OffTerm>();
   d.diag = op->diag.as<DiagTerm>();
   // the matvec stages 36 bytes per padded term, the other kernels 40 per term

@@ -99,6 +99,8 @@ struct OpDev {
   int64_t dim;
   int n_off;             // number of off-diagonal terms (in the row or column sense)
   int n_diag;
+  int word_bits;         // bits of the words and of every diagonal term's mask and match
+  int pad_word_bits_;
   const OffTerm* off;    // row form: match = R, image = C ; column form: match = C, image = R
   // The matvec's copy of off: every run of equal masks padded to an even
   // length (padding terms carry the run's mask, a match outside it and no

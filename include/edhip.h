@@ -325,7 +325,7 @@ int edhip_term_expect(edhip_op* op, int prec, const void* phi, const void* psi, 
 /* The same per-term forms with no device allocation: no counterpart in the
  * reference; for callers that cannot allocate device memory (code under HIP
  * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
- * partial sums live in a fixed per-GPU scratch array (about 17 MB, held
+ * partial sums live in a fixed per-GPU scratch array (about 8.5 MB, held
  * until the process ends), which limits the operator to 512 terms
  * (n_diag + n_off; more: EDHIP_ERR_ARGUMENT); psi is read through the
  * operator's index as built by edhip_op_create (never through a spread of
