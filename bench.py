@@ -408,8 +408,8 @@ def main():
         return
     if args.gpus != 1:
         print(
-            f"bench.py runs on one GPU: this build has no multi-GPU path (asked for {args.gpus}); "
-            "the sharding design is described in DESIGN.md",
+            f"bench.py runs on one GPU: there is no multi-GPU path (asked for {args.gpus}); "
+            "the representatives would shard by contiguous ranges, with an all-gather of x before each product",
             file=sys.stderr,
         )
         sys.exit(2)
