@@ -224,10 +224,11 @@ def test_hermiticity_and_linearity(ed, systems):
         assert rel(opr @ (2.0 * x - 3.0j * y), 2.0 * Hx - 3.0j * Hy) < 1e-13
 
 
-@pytest.mark.parametrize("n,p", [(20, 3), (30, 2), (40, 2), (48, 2)])
+@pytest.mark.parametrize("n,p", [(20, 3), (30, 2), (40, 2), (48, 2), (63, 2)])
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 def test_wide_words_few_particles(ed, oracle, n, p, dtype):
-    """Sectors of few up spins on wide words (K = 20 ... 48 bits): the spread
+    """Sectors of few up spins on wide words (K = 20 ... 63 bits, the widest
+    a UInt64 space allows): the spread
     of x takes its colex form where the positions fit in 32 bits, the plain
     form or none otherwise (then the rank index or the sorted search); an
     XY ring with next-nearest hopping and random fields (every image keeps

@@ -238,3 +238,36 @@ def test_reduce_6x6_properties(ed):
                 imgs.add(out & full)
         assert min(imgs) == w
         assert len(imgs) == orb[i]
+
+
+@pytest.mark.parametrize("k", [0, 5])
+def test_ring63_translations_vs_oracle(ed, oracle, k):
+    """The widest words a UInt64 space allows (63 sites, hilbert_space_representation.jl:63-69):
+    a 63-site ring with two up spins reduced by its 63 translations at
+    momentum k (complex characters at k = 5): basis, mapping bit-exact, and
+    the XY products (row and column forms) against the oracle."""
+    n = 63
+    hs, s = ed.spin_half_system(n)
+    hsr = ed.represent(ed.HilbertSpaceSector(hs, 4 - n))
+    parent = oracle.sector_basis(n, 4 - n)
+    assert np.array_equal(hsr.basis_list, parent)
+    irrep = ed.TranslationIrrep(n, k)
+    rh = ed.symmetry_reduce(hsr, irrep)
+    perms = np.stack([ed.bit_permutation(hs, p) for p, _ in irrep.symops_and_amplitudes()])
+    red = oracle.symmetry_reduce(parent, n, perms, [a for _, a in irrep.symops_and_amplitudes()])
+    assert np.array_equal(rh.basis_list, red.basis_list)
+    assert np.array_equal(rh.basis_mapping_index, red.basis_mapping_index)
+    assert np.abs(rh.basis_mapping_amplitude - red.basis_mapping_amplitude).max(initial=0) < 1e-14
+    H = ed.simplify(sum((ed.heisenberg_bond(hs, i, (i + 1) % n, 1.0, 0.0) for i in range(n)), ed.NullOperator()))
+    opr = ed.represent(rh, H)
+    o = oracle.Opr(red, ed.terms_arrays(H))
+    rng = np.random.default_rng(k)
+    x = rng.standard_normal(rh.dimension) + 1j * rng.standard_normal(rh.dimension)
+    for transpose in (False, True):
+        ref = o.apply(np.zeros(rh.dimension, complex), x, transpose=transpose)
+        out = np.zeros(rh.dimension, complex)
+        if transpose:
+            ed.mul_(out, x, opr)
+        else:
+            ed.mul_(out, opr, x)
+        assert np.linalg.norm(out - ref) <= 1e-13 * np.linalg.norm(ref)
