@@ -260,6 +260,66 @@ def lanczos_times(ed, opr, tol_fp32=0.0):
     return out
 
 
+def small_config_times(ed):
+    """The CPU baseline's configurations on the GPU, for a like-for-like view
+    beside it: the 16-site chain, the 24-site chain at k = 3 (4 levels), the
+    4x6 J1-J2 (pi,0) sector and the 4x6 random-bond XXZ sector (seed 2024).
+    Per config: the device basis and reduction (wall seconds), one FP64
+    product (HIP events, kernel launches only, averaged over 20), and the
+    FP64 Lanczos to the lowest levels at tol 1e-10 (seconds from the start
+    vector, the workspace excluded), with the levels."""
+    import torch
+    from edhip import _lib
+
+    L = _lib.lib()
+
+    def heis(hs, bonds, w, d=None):
+        return ed.simplify(sum((ed.heisenberg_bond(hs, i, j, w[k], 1.0 if d is None else d[k])
+                                for k, (i, j) in enumerate(bonds)), ed.NullOperator()))
+
+    def run(name, n, H, irrep, nev):
+        hs, _ = ed.spin_half_system(n)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        space = ed.represent(ed.HilbertSpaceSector(hs, 0))
+        if irrep is not None:
+            space = ed.symmetry_reduce(space, irrep)
+        dim = space.dimension
+        t_space = time.perf_counter() - t0
+        opr = ed.represent(space, H)
+        ctx = opr.ctx
+        dt = torch.complex128 if opr.is_complex else torch.float64
+        x = torch.randn(dim, dtype=torch.float64, device="cuda").to(dt)
+        y = torch.empty_like(x)
+        ed.mul_(y, opr, x)
+        tot, nl = ctypes.c_double(), ctypes.c_int64()
+        ctx.check(L.edhip_op_kernel_time(opr._handle, 1, ctypes.byref(tot), ctypes.byref(nl)))
+        ctx.check(L.edhip_set_kernel_timing(ctx.handle, 1))
+        try:
+            for _ in range(20):
+                ed.mul_(y, opr, x)
+        finally:
+            ctx.check(L.edhip_set_kernel_timing(ctx.handle, 0))
+        ctx.check(L.edhip_op_kernel_time(opr._handle, 1, ctypes.byref(tot), ctypes.byref(nl)))
+        r = ed.eigs(opr, nev, tol=1e-10, max_basis=min(100, dim), criterion="eigenvalue")
+        return {"config": name, "dim": dim, "space_s": t_space, "matvec_kernel_ms": tot.value / max(nl.value, 1),
+                "eigs_s": r.t_total, "matvecs": r.matvecs, "E": [float(v) for v in r.values]}
+
+    rows = []
+    hs, _ = ed.spin_half_system(16)
+    rows.append(run("chain16", 16, heis(hs, ed.chain_bonds(16), [1.0] * 16), None, 1))
+    hs, _ = ed.spin_half_system(24)
+    rows.append(run("chain24_k3", 24, heis(hs, ed.chain_bonds(24), [1.0] * 24), ed.TranslationIrrep(24, 3), 4))
+    j1, j2 = ed.square_bonds(4, 6)
+    rows.append(run("j1j2_4x6_pi0", 24, heis(hs, j1 + j2, [1.0] * len(j1) + [0.5] * len(j2)),
+                    ed.TranslationIrrep((4, 6), (2, 0)), 1))
+    rng = np.random.default_rng(2024)
+    B = len(j1)
+    J, Dl = rng.uniform(0.5, 1.5, B), rng.uniform(0.8, 1.2, B)
+    rows.append(run("xxz_4x6_random", 24, heis(hs, j1, J, Dl), None, 1))
+    return rows
+
+
 def grad_times(ed):
     """Backward pass on the 4x6 random-bond XXZ model (Sz = 0, 2,704,156
     states, 48 bonds, theta = (J_b, Delta_b), J_b ~ U(0.5,1.5), Delta_b ~
@@ -486,6 +546,7 @@ def main():
         line["space_group"] = space_group_times(ed, cfg, args.steps, args.warmup)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(ed)
+        line["small_configs_gpu"] = small_config_times(ed)
     print(json.dumps(line), flush=True)
 
 
