@@ -117,6 +117,7 @@ SIGNATURES = {
     "edhip_set_stream": (_i32, [_vp, _vp]),
     "edhip_get_stream": (_i32, [_vp, _P(_vp)]),
     "edhip_debug_fail_allocs": (_i32, [_i32, _i32]),
+    "edhip_debug_asm_shape": (_i32, [_i32, _i32]),
     "edhip_synchronize": (_i32, [_vp]),
     "edhip_malloc": (_i32, [_vp, _u64, _P(_vp)]),
     "edhip_free": (_i32, [_vp, _vp]),
@@ -146,6 +147,7 @@ SIGNATURES = {
     "edhip_op_count_images": (_i32, [_vp, _P(_i64)]),
     "edhip_op_set_strategy": (_i32, [_vp, _i32, _i32]),
     "edhip_op_last_strategy": (_i32, [_vp, _P(_i32), _P(_i32)]),
+    "edhip_op_assembled_info": (_i32, [_vp, _P(_u64), _P(_u64), _P(_dbl)]),
     "edhip_iterate": (_i32, [_vp, _i32, _i64, _P(_i64), _P(_i64), _P(_dbl), _P(_dbl)]),
     "edhip_sparse_build": (_i32, [_vp, _dbl, _P(_i64)]),
     "edhip_sparse_export": (_i32, [_vp, _P(_i64), _P(_i64), _P(_dbl), _P(_dbl)]),

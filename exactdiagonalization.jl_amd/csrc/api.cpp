@@ -61,7 +61,7 @@ inline bool prec_is_complex(int prec) { return prec == EDHIP_C128 || prec == EDH
 // Injected allocation failures (edhip_debug_fail_allocs): the next
 // g_fail_count allocations of the kinds in g_fail_kinds fail as if the
 // device were out of memory, before any device call is made.
-constexpr int kAllocSpread = 1, kAllocWorkspace = 2;
+constexpr int kAllocSpread = 1, kAllocWorkspace = 2, kAllocAssembled = 4;
 std::atomic<int> g_fail_kinds{0}, g_fail_count{0};
 void injected_alloc_failure(int kind) {
   if (!(g_fail_kinds.load() & kind)) return;
@@ -908,11 +908,28 @@ struct edhip_op {
   // per-row diagonal elements for the term-list kernel (built on first use)
   bool diag_tried = false;
   DevBuf diag_re, diag_im;
-  int strat_kernel = 0;  // 0 auto, 1 term list, 2 pair groups
+  int strat_kernel = 0;  // 0 auto, 1 term list, 2 pair groups, 3 assembled rows
   int strat_lookup = 0;  // 0 auto, 1 sorted search (or rank), 2 hash table
   int used_kernel = 0, used_lookup = 0;
   double kt_ms = 0.0;  // summed durations of the timed matvec and term-pass kernel launches
   int64_t kt_n = 0;
+  // assembled rows of the row form (strategy 3), built on first use
+  bool asm_built = false;
+  DevBuf asm_rowptr, asm_ent, asm_w, asm_acls, asm_pcls, asm_xt;
+  AsmDev asm_dev{};
+  uint64_t asm_nnz = 0;
+  double asm_build_s = 0.0;
+  void release_assembled() {
+    asm_built = false;
+    asm_rowptr.release();
+    asm_ent.release();
+    asm_w.release();
+    asm_acls.release();
+    asm_pcls.release();
+    asm_xt.release();
+    asm_dev = AsmDev{};
+    asm_nnz = 0;
+  }
 
   bool is_complex() const { return complex_amp || (space->group && space->group->complex_phase); }
 };
@@ -1168,9 +1185,150 @@ bool attach_spread(edhip_op* op, int prec, OpDev& d, const void* x, hipStream_t 
   return true;
 }
 
+// Lanes per row and entries per lane per batch of the assembled product
+// (0: chosen from the mean entries per row)
+std::atomic<int> g_asm_lanes{0}, g_asm_batch{0};
+
+void asm_shape(const edhip_op* op, int& lanes, int& batch) {
+  const double mean = op->space->dim ? (double)op->asm_nnz / (double)op->space->dim : 0.0;
+  lanes = g_asm_lanes.load();
+  if (!lanes) lanes = mean < 12.0 ? 4 : mean < 40.0 ? 8 : mean < 160.0 ? 16 : 32;
+  batch = g_asm_batch.load();
+  if (!batch) batch = 8;
+}
+
+// Assembled rows of the row form (strategy 3): the entry count of every row
+// (matched terms), their offsets, then the entries written by the matvec's own
+// matching and canonicalisation.  Weight classes: the distinct amplitudes of
+// the terms times the distinct character phases of the group.  Leaves nothing
+// allocated when it fails (EDHIP_ERR_MEMORY when the arrays do not fit).
+void ensure_assembled(edhip_op* op) {
+  if (op->asm_built) return;
+  const auto t0 = std::chrono::steady_clock::now();
+  const edhip_space* sp = op->space;
+  hipStream_t st = op->ctx->stream;
+  const int64_t D = sp->dim;
+  int col_bits = 1;
+  while (col_bits < 32 && (1ull << col_bits) < (uint64_t)D) ++col_bits;
+  if (col_bits > 31) fail(EDHIP_ERR_ARGUMENT, "assembled rows hold at most 2^31 basis states");
+  OpDev d = op->dev_row;
+  d.ix = sp->index_dev(0);
+  const int nt = d.n_mv ? d.n_mv : d.n_off;
+  std::vector<OffTerm> terms((size_t)nt);
+  if (nt)
+    HIP_OR_THROW(hipMemcpy(terms.data(), d.n_mv ? d.mv_off : d.off, (size_t)nt * sizeof(OffTerm), hipMemcpyDeviceToHost));
+  std::vector<std::complex<double>> amps, phs;
+  std::vector<uint32_t> acls((size_t)std::max(nt, 1), 1u);
+  for (int t = 0; t < nt; ++t) {
+    if (terms[(size_t)t].match == terms[(size_t)t].image) continue;  // padding: never matches
+    const std::complex<double> a(terms[(size_t)t].are, terms[(size_t)t].aim);
+    const size_t k = (size_t)(std::find(amps.begin(), amps.end(), a) - amps.begin());
+    if (k == amps.size()) amps.push_back(a);
+    acls[(size_t)t] = 1u + (uint32_t)k;
+  }
+  const GroupHost* g = sp->group.get();
+  const bool use_ph = g && !g->dev.unit_phases;
+  std::vector<uint32_t> pcls;
+  if (use_ph) {
+    pcls.resize((size_t)g->G);
+    for (int h = 0; h < g->G; ++h) {
+      const std::complex<double> p = std::conj(g->chi[(size_t)g->inv[(size_t)h]]);
+      const size_t k = (size_t)(std::find(phs.begin(), phs.end(), p) - phs.begin());
+      if (k == phs.size()) phs.push_back(p);
+      pcls[(size_t)h] = (uint32_t)k;
+    }
+  }
+  const int n_ph = use_ph ? (int)phs.size() : 1;
+  const int n_cls = 1 + (int)std::max<size_t>(amps.size(), 1) * n_ph;
+  const int max_cls = (32 - col_bits) >= 8 ? kAsmMaxClasses : (1 << (32 - col_bits));
+  if (n_cls > max_cls)
+    fail(EDHIP_ERR_ARGUMENT, "assembled rows: " + std::to_string(n_cls) + " element weights exceed the " +
+                                 std::to_string(max_cls) + " classes an entry can name at this dimension");
+  std::vector<double> w(2 * (size_t)n_cls, 0.0);
+  for (size_t a = 0; a < amps.size(); ++a)
+    for (int p = 0; p < n_ph; ++p) {
+      const std::complex<double> v = use_ph ? amps[a] * phs[(size_t)p] : amps[a];
+      w[1 + a * n_ph + p] = v.real();
+      w[(size_t)n_cls + 1 + a * n_ph + p] = v.imag();
+    }
+  try {
+    {
+      DevBuf cnt, tmp;
+      cnt.alloc((size_t)(D + 1) * 4);
+      HIP_OR_THROW(hipMemsetAsync(cnt.p, 0, (size_t)(D + 1) * 4, st));
+      HIP_OR_THROW(launch_asm_count(d, cnt.as<uint32_t>(), st));
+      op->asm_rowptr.alloc((size_t)(D + 1) * 8);
+      size_t tb = 0;
+      HIP_OR_THROW(exclusive_scan_u32_to_u64(cnt.as<uint32_t>(), op->asm_rowptr.as<uint64_t>(), (uint64_t)D + 1,
+                                             nullptr, tb, st));
+      tmp.alloc(tb);
+      HIP_OR_THROW(exclusive_scan_u32_to_u64(cnt.as<uint32_t>(), op->asm_rowptr.as<uint64_t>(), (uint64_t)D + 1, tmp.p,
+                                             tb, st));
+      HIP_OR_THROW(hipMemcpyAsync(&op->asm_nnz, op->asm_rowptr.as<uint64_t>() + D, 8, hipMemcpyDeviceToHost, st));
+      HIP_OR_THROW(hipStreamSynchronize(st));
+    }
+    injected_alloc_failure(kAllocAssembled);
+    op->asm_ent.alloc((size_t)op->asm_nnz * 4);
+    upload(op->asm_acls, acls, st);
+    if (use_ph) upload(op->asm_pcls, pcls, st);
+    upload(op->asm_w, w, st);
+    AsmFillDev f{op->asm_rowptr.as<uint64_t>(), op->asm_ent.as<uint32_t>(), op->asm_acls.as<uint32_t>(),
+                 use_ph ? op->asm_pcls.as<uint32_t>() : nullptr, n_ph, col_bits};
+    HIP_OR_THROW(launch_asm_fill(d, f, st));
+    HIP_OR_THROW(hipStreamSynchronize(st));
+  } catch (...) {
+    (void)hipGetLastError();
+    op->release_assembled();
+    throw;
+  }
+  op->asm_dev = AsmDev{op->asm_rowptr.as<uint64_t>(), op->asm_ent.as<uint32_t>(), op->asm_w.as<double>(),
+                       op->asm_w.as<double>() + n_cls, n_cls, col_bits};
+  op->asm_built = true;
+  op->asm_build_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// Row-form product from the assembled rows: x / sqrt(N) first (reduced spaces),
+// then the streaming product.
+void asm_matvec(edhip_op* op, int prec, int accumulate, const void* x, void* y) {
+  hipStream_t st = op->ctx->stream;
+  const edhip_space* sp = op->space;
+  OpDev d = op->dev_row;
+  if (ensure_diag(op)) {
+    d.diag_re = op->diag_re.as<double>();
+    d.diag_im = op->diag_im.as<double>();
+  }
+  const size_t es = elem_size(prec);
+  if (op->ctx->kernel_timing) HIP_OR_THROW(hipEventRecord(op->ctx->ev[14], st));
+  const void* xt = x;
+  if (d.orbit) {
+    op->asm_xt.ensure((size_t)sp->dim * es);
+    HIP_OR_THROW(launch_asm_scale(prec, sp->dim, d.orbit, x, op->asm_xt.p, st));
+    xt = op->asm_xt.p;
+  }
+  int lanes, batch;
+  asm_shape(op, lanes, batch);
+  HIP_OR_THROW(launch_asm_spmv(d, op->asm_dev, prec, lanes, batch, accumulate, x, xt, y, st));
+  if (op->ctx->kernel_timing) {
+    HIP_OR_THROW(hipEventRecord(op->ctx->ev[15], st));
+    HIP_OR_THROW(hipEventSynchronize(op->ctx->ev[15]));
+    float f = 0.f;
+    HIP_OR_THROW(hipEventElapsedTime(&f, op->ctx->ev[14], op->ctx->ev[15]));
+    op->kt_ms += f;
+    ++op->kt_n;
+  }
+}
+
 void op_matvec(edhip_op* op, int prec, int transpose, int accumulate, const void* x, void* y) {
   hipStream_t st = op->ctx->stream;
   const edhip_space* sp = op->space;
+  // assembled rows on request (row form; the column form runs the term list)
+  if (op->strat_kernel == 3 && !transpose) {
+    ensure_assembled(op);
+    op->used_kernel = 3;
+    op->used_lookup = 0;
+    asm_matvec(op, prec, accumulate, x, y);
+    return;
+  }
   // auto: the term-list kernel (measured faster on the 6x6 J1-J2 sector: its
   // lanes walk the terms in lock-step, which keeps their lookups in the same
   // cache lines); the pair-group kernel on request
@@ -1268,6 +1426,16 @@ char* krylov_arena_for(edhip_op* op, size_t bytes) {
     if (e.code != EDHIP_ERR_MEMORY) throw;
     (void)hipGetLastError();
     op->space->release_xmaps();
+  }
+  try {
+    return op->ctx->krylov_arena(bytes);
+  } catch (const Error& e) {
+    // the workspace takes precedence over the operator's assembled rows: they
+    // are dropped and the products run matrix-free (strategy back to auto)
+    if (e.code != EDHIP_ERR_MEMORY || !op->asm_built) throw;
+    (void)hipGetLastError();
+    op->release_assembled();
+    op->strat_kernel = 0;
     return op->ctx->krylov_arena(bytes);
   }
 }
@@ -2166,14 +2334,29 @@ int edhip_apply_host(edhip_op* op, int prec, int transpose, int accumulate, int6
 }
 
 int edhip_op_set_strategy(edhip_op* op, int kernel, int lookup) {
-  if (!op || kernel < 0 || kernel > 2 || lookup < 0 || lookup > 4) return EDHIP_ERR_ARGUMENT;
+  if (!op || kernel < 0 || kernel > 3 || lookup < 0 || lookup > 4) return EDHIP_ERR_ARGUMENT;
   return guarded(op->ctx, [&] {
     if (kernel == 2 && !fast_path_usable(op))
       fail(EDHIP_ERR_ARGUMENT, "the pair-group kernel does not cover this operator or symmetry group");
     if (lookup == 3) (void)op->space->index_dev(3);  // raises when no bitmap index exists
+    if (kernel == 3) {
+      ensure_assembled(op);
+    } else {
+      op->release_assembled();
+    }
     op->strat_kernel = kernel;
     op->strat_lookup = lookup;
   });
+}
+
+int edhip_op_assembled_info(const edhip_op* op, uint64_t* entries, uint64_t* bytes, double* build_seconds) {
+  if (!op || !entries || !bytes || !build_seconds) return EDHIP_ERR_ARGUMENT;
+  *entries = op->asm_built ? op->asm_nnz : 0;
+  *bytes = op->asm_built ? (uint64_t)(op->asm_ent.bytes + op->asm_rowptr.bytes + op->asm_w.bytes +
+                                      op->asm_acls.bytes + op->asm_pcls.bytes)
+                         : 0;
+  *build_seconds = op->asm_built ? op->asm_build_s : 0.0;
+  return EDHIP_OK;
 }
 
 int edhip_op_last_strategy(const edhip_op* op, int* kernel, int* lookup) {
@@ -2685,10 +2868,18 @@ int edhip_event_record(edhip_ctx* ctx, int slot) {
 }
 
 int edhip_debug_fail_allocs(int kinds, int count) {
-  if (kinds < 0 || kinds > (kAllocSpread | kAllocWorkspace) || count < 0) return EDHIP_ERR_ARGUMENT;
+  if (kinds < 0 || kinds > (kAllocSpread | kAllocWorkspace | kAllocAssembled) || count < 0) return EDHIP_ERR_ARGUMENT;
   g_fail_count.store(0);
   g_fail_kinds.store(kinds);
   g_fail_count.store(count);
+  return EDHIP_OK;
+}
+
+int edhip_debug_asm_shape(int lanes, int batch) {
+  if (lanes != 0 && lanes != 4 && lanes != 8 && lanes != 16 && lanes != 32) return EDHIP_ERR_ARGUMENT;
+  if (batch != 0 && batch != 4 && batch != 8 && batch != 16) return EDHIP_ERR_ARGUMENT;
+  g_asm_lanes.store(lanes);
+  g_asm_batch.store(batch);
   return EDHIP_OK;
 }
 

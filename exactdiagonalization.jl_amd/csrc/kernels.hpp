@@ -125,6 +125,34 @@ enum Prec : int { kF64 = 0, kC128 = 1, kF32 = 2, kC64 = 3 };
 hipError_t launch_matvec(const OpDev& op, int prec, int transpose, int accumulate, const void* x, void* y,
                          hipStream_t st);
 
+// ---- assembled rows (row form): entries col | cls << col_bits per image ----
+constexpr int kAsmMaxClasses = 256;
+struct AsmFillDev {
+  const uint64_t* rowptr;  // dim + 1 offsets of the rows' entries
+  uint32_t* ent;
+  const uint32_t* acls;    // per matvec term: amplitude class, 1-based
+  const uint32_t* pcls;    // per group element: phase class (nullptr: every phase 1)
+  int n_ph;
+  int col_bits;
+};
+struct AsmDev {
+  const uint64_t* rowptr;
+  const uint32_t* ent;
+  const double* w_re;      // n_cls weights, class 0 = 0
+  const double* w_im;
+  int n_cls;
+  int col_bits;
+};
+// counts[r] = entries of row r (terms matching it)
+hipError_t launch_asm_count(const OpDev& op, uint32_t* d_counts, hipStream_t st);
+hipError_t launch_asm_fill(const OpDev& op, const AsmFillDev& a, hipStream_t st);
+// xt = x / sqrt(orbit) (orbit != nullptr), in precision prec
+hipError_t launch_asm_scale(int prec, int64_t dim, const uint16_t* d_orbit, const void* x, void* xt, hipStream_t st);
+// y = (accumulate ? y : 0) + A x from the entries; xt = x scaled as above (x itself without orbits);
+// lanes = lanes per row (4, 8, 16 or 32), batch = entries per lane loaded together (4, 8 or 16)
+hipError_t launch_asm_spmv(const OpDev& op, const AsmDev& a, int prec, int lanes, int batch, int accumulate,
+                           const void* x, const void* xt, void* y, hipStream_t st);
+
 // Diagonal element of every row: re[r] (+ i im[r] when im != nullptr) = sum of
 // the amplitudes of the diagonal terms that match row r.
 hipError_t launch_diag_values(const OpDev& op, double* d_re, double* d_im, hipStream_t st);

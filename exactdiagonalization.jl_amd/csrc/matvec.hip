@@ -857,6 +857,169 @@ hipError_t launch_expect_v(const OpDev& op, const void* phi, const void* psi, do
   }
 }
 
+// ---------------------------------------------------------------------------
+// Assembled rows (row form): the off-diagonal images of every row, found once
+// by the matvec's own matching and canonicalisation, stored as 32-bit entries
+//   col | cls << col_bits
+// in the matvec's term order, rows one after another (rowptr).  Class 0 is an
+// image outside the space (weight 0); class c > 0 weighs a * phase, the term's
+// amplitude times the character phase of the element reaching the image.  The
+// orbit factor sqrt(N_r / N_s) is not stored: a product reads x / sqrt(N_s)
+// (prepared once per product) and scales each row's sum by sqrt(N_r).
+// ---------------------------------------------------------------------------
+
+// Bits of the terms seg .. seg + nseg - 1 (nseg <= 64) that match the word b,
+// as the matvec tests them (two terms per mask test on the paired copy).
+__device__ __forceinline__ uint64_t match_bits(const OpDev& op, const MvTerms& T, uint64_t b, int seg, int nseg) {
+  uint64_t bits = 0;
+  if (op.n_mv) {
+    for (int j = 0; j < nseg; j += 2) {
+      const uint64_t* m = T.mm + 3 * ((seg + j) >> 1);
+      const uint64_t xm = b & m[0];
+      bits |= (uint64_t)((uint32_t)(xm == m[1]) | ((uint32_t)(xm == m[2]) << 1)) << j;
+    }
+  } else {
+    for (int j = 0; j < nseg; ++j) {
+      const uint64_t* m = T.mm + 2 * (seg + j);
+      bits |= (uint64_t)((b & m[0]) == m[1]) << j;
+    }
+  }
+  return bits;
+}
+
+// counts[r] = number of terms matching row r (the entries it will hold)
+__global__ __launch_bounds__(kMvBlock) void k_asm_count(OpDev op, uint32_t* __restrict__ counts) {
+  extern __shared__ unsigned char lds[];
+  const MvTerms T = stage_mv_terms(op, lds, op.n_mv != 0);
+  __syncthreads();
+  const int64_t row = (int64_t)blockIdx.x * kMvBlock + threadIdx.x;
+  if (row >= op.dim) return;
+  const uint64_t b = op.rows[row];
+  const int n_terms = op.n_mv ? op.n_mv : op.n_off;
+  uint32_t n = 0;
+  for (int seg = 0; seg < n_terms; seg += 64)
+    n += (uint32_t)__builtin_popcountll(match_bits(op, T, b, seg, (n_terms - seg) < 64 ? (n_terms - seg) : 64));
+  counts[row] = n;
+}
+
+// Entries of every row: acls[t] is the amplitude class (1-based) of matvec
+// term t; with phases, cls = 1 + (acls - 1) * n_ph + pcls[h].
+template <int MODE, bool UNIT>
+__global__ __launch_bounds__(kMvBlock) void k_asm_fill(OpDev op, AsmFillDev a) {
+  extern __shared__ unsigned char lds[];
+  GroupLds L{};
+  if constexpr (MODE != kGroupNone) L = stage_group(op.g, op.lay, lds);
+  const MvTerms T = stage_mv_terms(op, lds, op.n_mv != 0);
+  lds_copy(lds + op.w_lds, op.ix.W, op.w_bytes);
+  const uint64_t* W = (const uint64_t*)(lds + op.w_lds);
+  __syncthreads();
+  const int64_t row = (int64_t)blockIdx.x * kMvBlock + threadIdx.x;
+  if (row >= op.dim) return;
+  const uint64_t b = op.rows[row];
+  uint32_t* out = a.ent + a.rowptr[row];
+  const int n_terms = op.n_mv ? op.n_mv : op.n_off;
+  for (int seg = 0; seg < n_terms; seg += 64) {
+    uint64_t bits = match_bits(op, T, b, seg, (n_terms - seg) < 64 ? (n_terms - seg) : 64);
+    while (bits) {
+      const int t = seg + __builtin_ctzll(bits);
+      bits &= bits - 1;
+      uint64_t s;
+      int h, cnt;
+      Canon<MODE>::template run<!UNIT>(op.g, L, b ^ T.fa[2 * t], s, h, cnt);
+      const int64_t idx = index_of(op.ix, W, s);
+      uint32_t e = 0;
+      if (idx >= 0) {
+        uint32_t cls = a.acls[t];
+        if (!UNIT && a.pcls) cls = 1u + (cls - 1u) * (uint32_t)a.n_ph + a.pcls[h];
+        e = (uint32_t)idx | (cls << a.col_bits);
+      }
+      *out++ = e;
+    }
+  }
+}
+
+// xt[i] = x[i] / sqrt(N_i)
+template <typename V>
+__global__ __launch_bounds__(kBlock) void k_asm_scale(int64_t dim, const uint16_t* __restrict__ orbit,
+                                                      const V* __restrict__ x, V* __restrict__ xt) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= dim) return;
+  using R = typename VT<V>::R;
+  const double s = 1.0 / sqrt((double)orbit[i]);
+  V v = x[i];
+  if constexpr (VT<V>::cx) {
+    v.re = (R)((double)v.re * s);
+    v.im = (R)((double)v.im * s);
+  } else {
+    v = (R)((double)v * s);
+  }
+  xt[i] = v;
+}
+
+template <typename V>
+__device__ __forceinline__ V vshfl_xor(const V& v, int o, int width) {
+  if constexpr (VT<V>::cx) {
+    return V{__shfl_xor(v.re, o, width), __shfl_xor(v.im, o, width)};
+  } else {
+    return __shfl_xor(v, o, width);
+  }
+}
+
+// y[r] = (accumulate ? y[r] : 0) + d_r x[r] + sqrt(N_r) sum_e W[cls_e] xt[col_e]:
+// GL lanes per row stride over its entries (a wave reads whole runs of
+// neighbouring rows' entries) in batches of U entries per lane, every load of
+// a batch issued before any is used (entries past the row's end are not
+// loaded and weigh nothing), then a butterfly sum over the GL lanes.  Blocks
+// are numbered over a 2D grid (a grid dimension holds fewer than 2^32 lanes).
+template <typename V, int GL, int U>
+__global__ __launch_bounds__(kBlock) void k_asm_spmv(OpDev op, AsmDev a, const V* __restrict__ x,
+                                                     const V* __restrict__ xt, V* __restrict__ y, int accumulate) {
+  using R = typename VT<V>::R;
+  __shared__ R w_re[kAsmMaxClasses], w_im[kAsmMaxClasses];
+  for (int i = threadIdx.x; i < a.n_cls; i += kBlock) {
+    w_re[i] = (R)a.w_re[i];
+    w_im[i] = (R)a.w_im[i];
+  }
+  __syncthreads();
+  const int64_t blk = (int64_t)blockIdx.y * gridDim.x + blockIdx.x;
+  const int64_t row = (blk * kBlock + threadIdx.x) / GL;
+  const int sub = (int)(threadIdx.x & (GL - 1));
+  const uint32_t cmask = (a.col_bits >= 32) ? ~0u : ((1u << a.col_bits) - 1u);
+  V acc = vzero<V>();
+  if (row < op.dim) {
+    const uint64_t end = a.rowptr[row + 1];
+    for (uint64_t e0 = a.rowptr[row] + (uint64_t)sub; e0 < end; e0 += (uint64_t)GL * U) {
+      uint32_t v[U];
+#pragma unroll
+      for (int k = 0; k < U; ++k) {
+        const uint64_t e = e0 + (uint64_t)k * GL;
+        v[k] = e < end ? __builtin_nontemporal_load(a.ent + e) : 0u;
+      }
+      V xv[U];
+#pragma unroll
+      for (int k = 0; k < U; ++k) xv[k] = xt[v[k] & cmask];
+#pragma unroll
+      for (int k = 0; k < U; ++k)
+        if (e0 + (uint64_t)k * GL < end) vfma(acc, w_re[v[k] >> a.col_bits], w_im[v[k] >> a.col_bits], xv[k]);
+    }
+  }
+#pragma unroll
+  for (int o = GL / 2; o > 0; o >>= 1) acc = vadd(acc, vshfl_xor(acc, o, GL));
+  if (row >= op.dim || sub != 0) return;
+  double dre = 0.0, dim_ = 0.0;
+  if (op.diag_re) {
+    dre = op.diag_re[row];
+    if (op.diag_im) dim_ = op.diag_im[row];
+  } else {
+    diag_sum(op, op.rows[row], dre, dim_);
+  }
+  V out = vzero<V>();
+  vfma(out, (R)dre, (R)dim_, x[row]);
+  const R so = op.orbit ? (R)sqrt((double)op.orbit[row]) : (R)1;
+  vfma(out, so, (R)0, acc);
+  y[row] = accumulate ? vadd(y[row], out) : out;
+}
+
 // Number of off-diagonal images generated over all rows (matched terms).
 __global__ __launch_bounds__(kBlock) void k_count_images(OpDev op, unsigned long long* __restrict__ total) {
   extern __shared__ unsigned char lds[];
@@ -873,7 +1036,116 @@ __global__ __launch_bounds__(kBlock) void k_count_images(OpDev op, unsigned long
   if ((threadIdx.x & 63) == 0 && c) atomicAdd(total, c);
 }
 
+template <int MODE>
+hipError_t launch_asm_fill_t(const OpDev& op, const AsmFillDev& a, hipStream_t st) {
+  dim3 grid((unsigned)((op.dim + kMvBlock - 1) / kMvBlock));
+  if (MODE == kGroupNone || op.g.unit_phases)
+    hipLaunchKernelGGL((k_asm_fill<MODE, true>), grid, dim3(kMvBlock), op.lds_total, st, op, a);
+  else
+    hipLaunchKernelGGL((k_asm_fill<MODE, false>), grid, dim3(kMvBlock), op.lds_total, st, op, a);
+  return hipGetLastError();
+}
+
+template <typename V>
+hipError_t launch_asm_spmv_v(const OpDev& op, const AsmDev& a, int lanes, int batch, int accumulate, const void* x,
+                             const void* xt, void* y, hipStream_t st) {
+  const uint64_t blocks = ((uint64_t)op.dim * (uint64_t)lanes + kBlock - 1) / kBlock;
+  const uint64_t gx = blocks < 65536 ? blocks : 65536;
+  dim3 grid((unsigned)gx, (unsigned)((blocks + gx - 1) / gx));
+#define EDHIP_ASM(G, B)                                                                                          \
+  hipLaunchKernelGGL((k_asm_spmv<V, G, B>), grid, dim3(kBlock), 0, st, op, a, (const V*)x, (const V*)xt, (V*)y, \
+                     accumulate)
+#define EDHIP_ASM_B(G)                 \
+  switch (batch) {                     \
+    case 4: EDHIP_ASM(G, 4); break;    \
+    case 8: EDHIP_ASM(G, 8); break;    \
+    case 16: EDHIP_ASM(G, 16); break;  \
+    default: return hipErrorInvalidValue; \
+  }
+  switch (lanes) {
+    case 4: EDHIP_ASM_B(4); break;
+    case 8: EDHIP_ASM_B(8); break;
+    case 16: EDHIP_ASM_B(16); break;
+    case 32: EDHIP_ASM_B(32); break;
+    default: return hipErrorInvalidValue;
+  }
+#undef EDHIP_ASM_B
+#undef EDHIP_ASM
+  return hipGetLastError();
+}
+
 }  // namespace
+
+hipError_t launch_asm_count(const OpDev& op, uint32_t* d_counts, hipStream_t st) {
+  if (op.dim == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_asm_count, dim3((unsigned)((op.dim + kMvBlock - 1) / kMvBlock)), dim3(kMvBlock), op.lds_total,
+                     st, op, d_counts);
+  return hipGetLastError();
+}
+
+hipError_t launch_asm_fill(const OpDev& op, const AsmFillDev& a, hipStream_t st) {
+  if (op.dim == 0) return hipSuccess;
+  switch (op.g.mode) {
+    case kGroupNone:
+      return launch_asm_fill_t<kGroupNone>(op, a, st);
+    case kGroupGeneric:
+      return launch_asm_fill_t<kGroupGeneric>(op, a, st);
+    default:
+      if (op.g.pair_table) {
+        const int Lx = op.g.Lx, Ly = op.g.Ly;
+        if (op.g.n_cosets == 1) {
+          if (Lx == 6 && Ly == 6) return launch_asm_fill_t<kTransPairs6x6>(op, a, st);
+          if (Lx == 4 && Ly == 6) return launch_asm_fill_t<kTransPairs4x6>(op, a, st);
+          if (Lx == 6 && Ly == 4) return launch_asm_fill_t<kTransPairs6x4>(op, a, st);
+          if (Lx == 4 && Ly == 4) return launch_asm_fill_t<kTransPairs4x4>(op, a, st);
+        }
+        return launch_asm_fill_t<kGroupTorusPairs>(op, a, st);
+      }
+      return launch_asm_fill_t<kGroupTorus>(op, a, st);
+  }
+}
+
+hipError_t launch_asm_scale(int prec, int64_t dim, const uint16_t* d_orbit, const void* x, void* xt, hipStream_t st) {
+  if (dim == 0) return hipSuccess;
+  dim3 grid((unsigned)((dim + kBlock - 1) / kBlock));
+  switch (prec) {
+    case kF64:
+      hipLaunchKernelGGL(k_asm_scale<double>, grid, dim3(kBlock), 0, st, dim, d_orbit, (const double*)x, (double*)xt);
+      break;
+    case kC128:
+      hipLaunchKernelGGL(k_asm_scale<Cx<double>>, grid, dim3(kBlock), 0, st, dim, d_orbit, (const Cx<double>*)x,
+                         (Cx<double>*)xt);
+      break;
+    case kF32:
+      hipLaunchKernelGGL(k_asm_scale<float>, grid, dim3(kBlock), 0, st, dim, d_orbit, (const float*)x, (float*)xt);
+      break;
+    case kC64:
+      hipLaunchKernelGGL(k_asm_scale<Cx<float>>, grid, dim3(kBlock), 0, st, dim, d_orbit, (const Cx<float>*)x,
+                         (Cx<float>*)xt);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_asm_spmv(const OpDev& op, const AsmDev& a, int prec, int lanes, int batch, int accumulate,
+                           const void* x, const void* xt, void* y, hipStream_t st) {
+  if (op.dim == 0) return hipSuccess;
+  if (a.n_cls > kAsmMaxClasses) return hipErrorInvalidValue;
+  switch (prec) {
+    case kF64:
+      return launch_asm_spmv_v<double>(op, a, lanes, batch, accumulate, x, xt, y, st);
+    case kC128:
+      return launch_asm_spmv_v<Cx<double>>(op, a, lanes, batch, accumulate, x, xt, y, st);
+    case kF32:
+      return launch_asm_spmv_v<float>(op, a, lanes, batch, accumulate, x, xt, y, st);
+    case kC64:
+      return launch_asm_spmv_v<Cx<float>>(op, a, lanes, batch, accumulate, x, xt, y, st);
+    default:
+      return hipErrorInvalidValue;
+  }
+}
 
 hipError_t launch_count_images(const OpDev& op, unsigned long long* d_total, hipStream_t st) {
   if (op.dim == 0) return hipSuccess;

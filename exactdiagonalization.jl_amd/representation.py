@@ -582,7 +582,7 @@ class AbstractOperatorRepresentation:
         return np.dtype(np.complex128) if self.is_complex else np.dtype(np.float64)
 
     # ---- strategy (performance knob; every choice gives the same matrix) -------
-    _KERNELS = {"auto": 0, "terms": 1, "pairs": 2}
+    _KERNELS = {"auto": 0, "terms": 1, "pairs": 2, "assembled": 3}
     _LOOKUPS = {"auto": 0, "search": 1, "hash": 2, "bitmap": 3, "xspread": 4}
 
     def set_strategy(self, kernel: str = "auto", lookup: str = "auto"):
@@ -591,8 +591,16 @@ class AbstractOperatorRepresentation:
     def last_strategy(self):
         k, l = ctypes.c_int(), ctypes.c_int()
         self.ctx.check(_lib.lib().edhip_op_last_strategy(self._handle, ctypes.byref(k), ctypes.byref(l)))
-        return ({0: None, 1: "terms", 2: "pairs"}[k.value],
+        return ({0: None, 1: "terms", 2: "pairs", 3: "assembled"}[k.value],
                 {0: None, 1: "search", 2: "hash", 3: "bitmap", 4: "xspread"}[l.value])
+
+    def assembled_info(self):
+        """(entries, device bytes, build seconds) of the assembled rows the
+        operator holds (kernel "assembled"); zeros when it holds none."""
+        n, b, t = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_double()
+        self.ctx.check(_lib.lib().edhip_op_assembled_info(self._handle, ctypes.byref(n), ctypes.byref(b),
+                                                          ctypes.byref(t)))
+        return int(n.value), int(b.value), float(t.value)
 
     # ---- apply ---------------------------------------------------------------
     def _apply(self, out, state, transpose: bool, accumulate: bool):

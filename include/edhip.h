@@ -200,7 +200,19 @@ int edhip_apply_host(edhip_op* op, int prec, int transpose, int accumulate, int6
  * against (test-only).
  *   kernel: 0 auto (term list), 1 term-list kernel, 2 bond-pair kernel
  *           (operators whose terms act on one or two bits, plain or
- *           torus-translation spaces; test-only)
+ *           torus-translation spaces; test-only),
+ *           3 assembled rows: the request itself finds every row's
+ *             off-diagonal images once, with the term-list kernel's
+ *             matching, canonicalisation and index, and keeps them as
+ *             4-byte entries (column | weight class << column bits, at
+ *             most 2^31 states and 256 classes: distinct term amplitudes
+ *             times distinct character phases); row-form products then
+ *             stream the entries (the column form keeps the term list).
+ *             The arrays take 4 bytes per image plus 8 per row and stay
+ *             until another kernel is requested or the operator is
+ *             destroyed; EDHIP_ERR_MEMORY when they do not fit (nothing
+ *             kept).  An eigensolver or MINRES workspace that does not fit
+ *             beside them drops them and sets the strategy back to auto.
  *   lookup: 0 auto (term-list kernel: x spread over word positions when
  *             it fits, for 4-byte elements, for full popcount sectors and
  *             where the spread is at most 256 MB; else bitmap, else sorted
@@ -220,6 +232,9 @@ int edhip_apply_host(edhip_op* op, int prec, int transpose, int accumulate, int6
  * edhip_op_last_strategy reports what the last product used. */
 int edhip_op_set_strategy(edhip_op* op, int kernel, int lookup);
 int edhip_op_last_strategy(const edhip_op* op, int* kernel, int* lookup);
+/* Assembled rows of the operator (kernel 3): entries, device bytes and the
+ * seconds their construction took (zeros when none are held). */
+int edhip_op_assembled_info(const edhip_op* op, uint64_t* entries, uint64_t* bytes, double* build_seconds);
 
 /* Number of off-diagonal images the row iterators generate over all rows
  * (terms whose bitrow matches), i.e. the off-diagonal elements one apply!
@@ -360,10 +375,18 @@ int edhip_event_elapsed(edhip_ctx* ctx, int slot0, int slot1, double* ms);
  * input vector over the word positions (the products then read x through
  * the rank index or the sorted search), bit 1 the eigensolver's Krylov
  * workspace (on a failure the spreads are freed and the allocation tried
- * once more; a second failure returns EDHIP_ERR_MEMORY). */
+ * once more; a second failure returns EDHIP_ERR_MEMORY), bit 2 the entries
+ * of an operator's assembled rows (the strategy request returns
+ * EDHIP_ERR_MEMORY and keeps nothing). */
 #define EDHIP_ALLOC_SPREAD 1
 #define EDHIP_ALLOC_WORKSPACE 2
+#define EDHIP_ALLOC_ASSEMBLED 4
 int edhip_debug_fail_allocs(int kinds, int count);
+/* Tuning (no counterpart in the reference): lanes per row of the
+ * assembled-row product, 4, 8, 16 or 32, and entries each lane loads
+ * together, 4, 8 or 16 (0: the defaults, lanes chosen from the mean entries
+ * per row).  Process-wide. */
+int edhip_debug_asm_shape(int lanes, int batch);
 
 /* Measurement (no counterpart in the reference): with timing on, every
  * term-list matvec kernel launch and every fused term pass (edhip_term_expect:
