@@ -21,8 +21,12 @@ configurations that path can build; the 6x6 workload is beyond it.
 ``time_to_E0`` times the FP32-mixed and the FP64 Lanczos to E0 on the
 workload's operator.
 
+``assembled_rows`` times the assembled-row strategy on the same operator:
+the build of its entries, its FP32 and FP64 products and both Lanczos modes
+on them.
+
 Options: --steps K --warmup W --gpus 1 [--config 4x6|6x6] [--no-cpu]
-[--no-grad] [--no-e0] [--space-group].
+[--no-grad] [--no-e0] [--no-assembled] [--space-group].
 """
 from __future__ import annotations
 
@@ -260,6 +264,49 @@ def lanczos_times(ed, opr, tol_fp32=0.0):
     return out
 
 
+def assembled_times(ed, opr, elems, steps, warmup, e0=True):
+    """The assembled-row strategy on the same workload: its one-time build
+    (every row's images found once, kept as 4-byte entries), the FP32 and
+    FP64 products that stream the entries, and the time to E0 of both
+    Lanczos modes on them (basis of 80 vectors: the FP64 workspace and the
+    entries then fit together; both modes converge before a restart).  The
+    matrix-free product stays the headline; this is the alternative for
+    repeated products when memory allows."""
+    from edhip import _lib
+
+    ed.release_workspace()  # the time-to-E0 workspace would leave no room for the entries
+    opr.set_strategy("assembled")
+    try:
+        n_ent, n_bytes, t_build = opr.assembled_info()
+        ms32, cs32 = time_matvecs(ed, opr, _lib.F32, steps, warmup)
+        ms64, cs64 = time_matvecs(ed, opr, _lib.F64, steps, warmup)
+        k32 = kernel_ms(ed, opr, _lib.F32, steps)
+        k64 = kernel_ms(ed, opr, _lib.F64, steps)
+        out = {
+            "entries": n_ent,
+            "device_bytes": n_bytes,
+            "build_seconds": t_build,
+            "f32": {"ms_per_step": ms32, "kernel_ms": k32, "Gelem_per_s": elems / (ms32 * 1e-3) / 1e9,
+                    "checksum": cs32},
+            "f64": {"ms_per_step": ms64, "kernel_ms": k64, "Gelem_per_s": elems / (ms64 * 1e-3) / 1e9,
+                    "checksum": cs64},
+            "kernels": "k_asm_scale (x / sqrt(N)) + k_asm_spmv (16 lanes per row, 8 entries per lane per batch)",
+        }
+        te = {}
+        for mode in ("fp64", "mixed") if e0 else ():
+            r = ed.eigs(opr, 1, mode=mode, tol=1e-10, max_basis=80, max_restarts=20, criterion="eigenvalue")
+            te[mode] = {"E0": float(r.values[0]), "seconds": r.t_total, "alloc_seconds": r.t_alloc,
+                        "matvecs": r.matvecs, "matvec_seconds": r.t_matvec, "converged": r.converged,
+                        "strategy": opr.last_strategy()[0]}
+        if e0:
+            te["speedup_mixed_vs_fp64"] = te["fp64"]["seconds"] / te["mixed"]["seconds"]
+            out["time_to_E0"] = te
+    finally:
+        opr.set_strategy()
+        ed.release_workspace()
+    return out
+
+
 def small_config_times(ed):
     """The CPU baseline's configurations on the GPU, for a like-for-like view
     beside it: the 16-site chain, the 24-site chain at k = 3 (4 levels), the
@@ -447,8 +494,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", default="6x6", choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-e0", action="store_true", help="skip the time-to-E0 runs (about 40 s)")
+    ap.add_argument("--no-e0", action="store_true",
+                    help="skip the time-to-E0 runs (about 40 s; also those on the assembled rows)")
     ap.add_argument("--no-grad", action="store_true")
+    ap.add_argument("--no-assembled", action="store_true",
+                    help="skip the assembled-row strategy (77 GB of entries at 6x6; about 20 s)")
     ap.add_argument("--tol-fp32", type=float, default=0.0,
                     help="mixed Lanczos: eigenvalue tolerance of the FP32 phase (0: max(tol, 1e-10))")
     ap.add_argument("--space-group", action="store_true",
@@ -540,6 +590,8 @@ def main():
     }
     if not args.no_e0:
         line["time_to_E0"] = lanczos_times(ed, opr, args.tol_fp32)
+    if not args.no_assembled:
+        line["assembled_rows"] = assembled_times(ed, opr, elems, args.steps, args.warmup, e0=not args.no_e0)
     if not args.no_grad:
         line["grad"] = grad_times(ed)
     if args.space_group:

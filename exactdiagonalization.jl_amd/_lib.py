@@ -118,6 +118,7 @@ SIGNATURES = {
     "edhip_get_stream": (_i32, [_vp, _P(_vp)]),
     "edhip_debug_fail_allocs": (_i32, [_i32, _i32]),
     "edhip_debug_asm_shape": (_i32, [_i32, _i32]),
+    "edhip_debug_block_order": (_i32, [_i32]),
     "edhip_synchronize": (_i32, [_vp]),
     "edhip_malloc": (_i32, [_vp, _u64, _P(_vp)]),
     "edhip_free": (_i32, [_vp, _vp]),

@@ -46,6 +46,20 @@ struct VT<Cx<float>> {
   static constexpr bool cx = true;
 };
 
+// Block order of the products.  Workgroups are dispatched to the 8 XCDs in
+// turn (dispatch b runs on XCD b mod 8), each XCD with its own L2.  With
+// chunk C > 1, a window of 8 C dispatches covers 8 C consecutive blocks and
+// XCD x takes blocks x C .. x C + C - 1 of it, so that neighbouring rows,
+// whose images share lines of x, meet in one L2 while every XCD still works
+// inside the same window of rows (which the shared last-level cache serves).
+// A bijection on [0, n_blocks); blocks of a last partial window keep their order.
+__device__ __forceinline__ int64_t xcd_chunked_block(int64_t b, int64_t n_blocks, int C) {
+  const int64_t w = 8 * (int64_t)C;
+  if (C <= 1 || b >= (n_blocks / w) * w) return b;
+  const int64_t r = b % w;
+  return b - r + (r & 7) * C + (r >> 3);
+}
+
 template <typename V>
 __device__ __forceinline__ V vzero() {
   if constexpr (VT<V>::cx) {

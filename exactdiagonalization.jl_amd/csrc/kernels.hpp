@@ -100,7 +100,7 @@ struct OpDev {
   int n_off;             // number of off-diagonal terms (in the row or column sense)
   int n_diag;
   int word_bits;         // bits of the words and of every diagonal term's mask and match
-  int pad_word_bits_;
+  int xcd_chunk;         // products: consecutive blocks each XCD takes per window of 8 xcd_chunk (<= 1: plain order)
   const OffTerm* off;    // row form: match = R, image = C ; column form: match = C, image = R
   // The matvec's copy of off: every run of equal masks padded to an even
   // length (padding terms carry the run's mask, a match outside it and no

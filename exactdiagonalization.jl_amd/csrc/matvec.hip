@@ -183,7 +183,7 @@ __global__ __launch_bounds__(kMvBlock) void k_matvec(OpDev op, const V* __restri
   uint16_t* codes = (uint16_t*)(lds + op.codes_lds);
   if (op.xm.values) xmap_stage(op.xm, codes);
   __syncthreads();
-  const int64_t row = (int64_t)blockIdx.x * kMvBlock + threadIdx.x;
+  const int64_t row = xcd_chunked_block(blockIdx.x, gridDim.x, op.xcd_chunk) * kMvBlock + threadIdx.x;
   if (row >= op.dim) return;
   using R = typename VT<V>::R;
   const uint64_t b = op.rows[row];
@@ -981,7 +981,8 @@ __global__ __launch_bounds__(kBlock) void k_asm_spmv(OpDev op, AsmDev a, const V
     w_im[i] = (R)a.w_im[i];
   }
   __syncthreads();
-  const int64_t blk = (int64_t)blockIdx.y * gridDim.x + blockIdx.x;
+  const int64_t blk = xcd_chunked_block((int64_t)blockIdx.y * gridDim.x + blockIdx.x,
+                                        (int64_t)gridDim.x * gridDim.y, op.xcd_chunk);
   const int64_t row = (blk * kBlock + threadIdx.x) / GL;
   const int sub = (int)(threadIdx.x & (GL - 1));
   const uint32_t cmask = (a.col_bits >= 32) ? ~0u : ((1u << a.col_bits) - 1u);

@@ -387,6 +387,12 @@ int edhip_debug_fail_allocs(int kinds, int count);
  * together, 4, 8 or 16 (0: the defaults, lanes chosen from the mean entries
  * per row).  Process-wide. */
 int edhip_debug_asm_shape(int lanes, int batch);
+/* Tuning (no counterpart in the reference): block order of the products
+ * (term-list and assembled-row kernels).  Workgroups go to the 8 XCDs in
+ * turn; with chunk C > 1 each XCD takes C consecutive blocks of every window
+ * of 8 C blocks (0 or 1: plain order; at most 64; default 32).
+ * Process-wide; every order gives the same results. */
+int edhip_debug_block_order(int chunk);
 
 /* Measurement (no counterpart in the reference): with timing on, every
  * term-list matvec kernel launch and every fused term pass (edhip_term_expect:

@@ -191,3 +191,36 @@ def test_lanczos_on_assembled_rows(ed, systems, mode):
     finally:
         opr.set_strategy()
     assert np.allclose(r.values, e_ref, rtol=0, atol=1e-10 * abs(e_ref[0]))
+
+
+@pytest.mark.parametrize("strategy", ["auto", "assembled"])
+def test_block_order_gives_bitwise_equal_products(ed, systems, strategy):
+    """The XCD-chunked block order (edhip_debug_block_order) only reorders the
+    blocks: every chunk gives bitwise the same products, on a sector large
+    enough for many windows (4x6 random-bond XXZ, 2,704,156 states) and on a
+    reduced one."""
+    from edhip import _lib
+
+    L = _lib.lib()
+    hs, _ = ed.spin_half_system(24)
+    j1, _ = ed.square_bonds(4, 6)
+    rng = np.random.default_rng(3)
+    H = _heis(ed, hs, j1, rng.uniform(0.5, 1.5, len(j1)), rng.uniform(0.8, 1.2, len(j1)))
+    big = ed.represent(ed.represent(ed.HilbertSpaceSector(hs, 0)), H)
+    try:
+        for opr in (big, systems["torus4x6_pi0"][0]):
+            opr.set_strategy(strategy)
+            x = rng.standard_normal(opr.dimension)
+            outs = []
+            for chunk in (1, 2, 5, 32, 64):
+                assert L.edhip_debug_block_order(chunk) == 0
+                y64 = opr @ x
+                y32 = np.zeros(opr.dimension, np.float32)
+                ed.mul_(y32, opr, x.astype(np.float32))
+                outs.append((y64, y32))
+            opr.set_strategy()
+            for y64, y32 in outs[1:]:
+                assert np.array_equal(y64, outs[0][0]) and np.array_equal(y32, outs[0][1])
+    finally:
+        assert L.edhip_debug_block_order(32) == 0
+    assert L.edhip_debug_block_order(65) != 0
