@@ -21,6 +21,9 @@ configurations that path can build; the 6x6 workload is beyond it.
 ``time_to_E0`` times the FP32-mixed and the FP64 Lanczos to E0 on the
 workload's operator.
 
+``row_shares`` times one GPU's share of a product sharded by row ranges
+over 8 GPUs (each eighth of the rows, the whole x resident).
+
 ``assembled_rows`` times the assembled-row strategy on the same operator:
 the build of its entries, its FP32 and FP64 products and both Lanczos modes
 on them.
@@ -307,6 +310,53 @@ def assembled_times(ed, opr, elems, steps, warmup, e0=True):
     return out
 
 
+def share_times(ed, opr, n_shares=8, reps=3):
+    """One GPU's share of a product sharded by contiguous row ranges over
+    ``n_shares`` GPUs: the FP32 kernel time of each share's rows (the whole x
+    resident, as after the all-gather), from the matrix-free kernel's own
+    launch events.  The slowest share sets the sharded product's kernel time;
+    the spread between shares is the load imbalance of equal row counts."""
+    import torch
+    from edhip import _lib
+
+    L = _lib.lib()
+    ctx = opr.ctx
+    D = opr.dimension
+    x = torch.randn(D, dtype=torch.float32, device="cuda", generator=torch.Generator(device="cuda").manual_seed(7))
+    cuts = [D * k // n_shares for k in range(n_shares + 1)]
+    ms = []
+    for a, b in zip(cuts[:-1], cuts[1:]):
+        y = torch.empty(b - a, dtype=torch.float32, device="cuda")
+        xp, yp = ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(y.data_ptr())
+        ctx.check(L.edhip_apply_rows(opr._handle, _lib.F32, 0, a, b, D, xp, b - a, yp))  # warm
+        ctx.synchronize()
+        tot, n = ctypes.c_double(), ctypes.c_int64()
+        ctx.check(L.edhip_op_kernel_time(opr._handle, 1, ctypes.byref(tot), ctypes.byref(n)))
+        ctx.check(L.edhip_set_kernel_timing(ctx.handle, 1))
+        try:
+            for _ in range(reps):
+                ctx.check(L.edhip_apply_rows(opr._handle, _lib.F32, 0, a, b, D, xp, b - a, yp))
+            ctx.synchronize()
+        finally:
+            ctx.check(L.edhip_set_kernel_timing(ctx.handle, 0))
+        ctx.check(L.edhip_op_kernel_time(opr._handle, 1, ctypes.byref(tot), ctypes.byref(n)))
+        ms.append(tot.value / max(n.value, 1))
+        del y
+    del x
+    torch.cuda.empty_cache()
+    return {
+        "shares": n_shares,
+        "rows_per_share": [b - a for a, b in zip(cuts[:-1], cuts[1:])],
+        "f32_kernel_ms": ms,
+        "max_ms": max(ms),
+        "mean_ms": sum(ms) / len(ms),
+        "sum_ms": sum(ms),
+        "imbalance_max_over_mean": max(ms) / (sum(ms) / len(ms)),
+        "note": "one GPU's share at its real shape (rows of D/8, full basis and x resident); "
+                "a sharded product would add the all-gather of x, not measured (one GPU)",
+    }
+
+
 def small_config_times(ed):
     """The CPU baseline's configurations on the GPU, for a like-for-like view
     beside it: the 16-site chain, the 24-site chain at k = 3 (4 levels), the
@@ -588,6 +638,7 @@ def main():
         "checksum_f32": cs32,
         "checksum_f64": cs64,
     }
+    line["row_shares"] = share_times(ed, opr)
     if not args.no_e0:
         line["time_to_E0"] = lanczos_times(ed, opr, args.tol_fp32)
     if not args.no_assembled:

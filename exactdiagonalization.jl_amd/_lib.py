@@ -148,6 +148,7 @@ SIGNATURES = {
     "edhip_op_count_images": (_i32, [_vp, _P(_i64)]),
     "edhip_op_set_strategy": (_i32, [_vp, _i32, _i32]),
     "edhip_op_last_strategy": (_i32, [_vp, _P(_i32), _P(_i32)]),
+    "edhip_apply_rows": (_i32, [_vp, _i32, _i32, _i64, _i64, _i64, _vp, _i64, _vp]),
     "edhip_op_assembled_info": (_i32, [_vp, _P(_u64), _P(_u64), _P(_dbl)]),
     "edhip_iterate": (_i32, [_vp, _i32, _i64, _P(_i64), _P(_i64), _P(_dbl), _P(_dbl)]),
     "edhip_sparse_build": (_i32, [_vp, _dbl, _P(_i64)]),

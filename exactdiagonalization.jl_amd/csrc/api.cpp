@@ -1324,6 +1324,39 @@ void asm_matvec(edhip_op* op, int prec, int accumulate, const void* x, void* y) 
   }
 }
 
+// Term-list kernel over the output rows [row_begin, row_end) (y holds them
+// from y[0]); x is the whole input vector.
+void term_list_matvec(edhip_op* op, int prec, int transpose, int accumulate, const void* x, void* y,
+                      int64_t row_begin, int64_t row_end) {
+  hipStream_t st = op->ctx->stream;
+  const edhip_space* sp = op->space;
+  const bool rank_mode = sp->is_sector && !sp->group;
+  const int idx_lookup = (op->strat_lookup == 2) ? 0 : op->strat_lookup;  // the hash applies to the pair kernel only
+  OpDev d = transpose ? op->dev_col : op->dev_row;
+  d.ix = sp->index_dev(idx_lookup == 4 ? 0 : idx_lookup);
+  d.xcd_chunk = g_xcd_chunk.load();
+  d.row_begin = row_begin;
+  d.dim = row_end;
+  if (ensure_diag(op)) {
+    d.diag_re = op->diag_re.as<double>();
+    d.diag_im = op->diag_im.as<double>();
+  }
+  op->used_kernel = 1;
+  op->used_lookup = rank_mode ? 1 : (d.ix.bm_bits ? 3 : 1);
+  if (attach_spread(op, prec, d, x, st)) op->used_lookup = 4;
+  if (idx_lookup == 4 && !d.xm.values) fail(EDHIP_ERR_ARGUMENT, "x cannot be spread over the words of this space");
+  if (op->ctx->kernel_timing) HIP_OR_THROW(hipEventRecord(op->ctx->ev[14], st));
+  HIP_OR_THROW(launch_matvec(d, prec, transpose ? 1 : 0, accumulate ? 1 : 0, x, y, st));
+  if (op->ctx->kernel_timing) {
+    HIP_OR_THROW(hipEventRecord(op->ctx->ev[15], st));
+    HIP_OR_THROW(hipEventSynchronize(op->ctx->ev[15]));
+    float f = 0.f;
+    HIP_OR_THROW(hipEventElapsedTime(&f, op->ctx->ev[14], op->ctx->ev[15]));
+    op->kt_ms += f;
+    ++op->kt_n;
+  }
+}
+
 void op_matvec(edhip_op* op, int prec, int transpose, int accumulate, const void* x, void* y) {
   hipStream_t st = op->ctx->stream;
   const edhip_space* sp = op->space;
@@ -1344,27 +1377,7 @@ void op_matvec(edhip_op* op, int prec, int transpose, int accumulate, const void
   const bool rank_mode = sp->is_sector && !sp->group;
   const int idx_lookup = (op->strat_lookup == 2) ? 0 : op->strat_lookup;  // the hash applies to the pair kernel only
   if (!fast) {
-    OpDev d = transpose ? op->dev_col : op->dev_row;
-    d.ix = sp->index_dev(idx_lookup == 4 ? 0 : idx_lookup);
-    d.xcd_chunk = g_xcd_chunk.load();
-    if (ensure_diag(op)) {
-      d.diag_re = op->diag_re.as<double>();
-      d.diag_im = op->diag_im.as<double>();
-    }
-    op->used_kernel = 1;
-    op->used_lookup = rank_mode ? 1 : (d.ix.bm_bits ? 3 : 1);
-    if (attach_spread(op, prec, d, x, st)) op->used_lookup = 4;
-    if (idx_lookup == 4 && !d.xm.values) fail(EDHIP_ERR_ARGUMENT, "x cannot be spread over the words of this space");
-    if (op->ctx->kernel_timing) HIP_OR_THROW(hipEventRecord(op->ctx->ev[14], st));
-    HIP_OR_THROW(launch_matvec(d, prec, transpose ? 1 : 0, accumulate ? 1 : 0, x, y, st));
-    if (op->ctx->kernel_timing) {
-      HIP_OR_THROW(hipEventRecord(op->ctx->ev[15], st));
-      HIP_OR_THROW(hipEventSynchronize(op->ctx->ev[15]));
-      float f = 0.f;
-      HIP_OR_THROW(hipEventElapsedTime(&f, op->ctx->ev[14], op->ctx->ev[15]));
-      op->kt_ms += f;
-      ++op->kt_n;
-    }
+    term_list_matvec(op, prec, transpose, accumulate, x, y, 0, sp->dim);
     return;
   }
   bool use_hash = false;
@@ -2314,6 +2327,24 @@ int edhip_apply(edhip_op* op, int prec, int transpose, int accumulate, int64_t n
     if (n_x != d)
       fail(EDHIP_ERR_DIMENSION, "state has length " + std::to_string(n_x) + " != dimension " + std::to_string(d));
     op_matvec(op, prec, transpose ? 1 : 0, accumulate ? 1 : 0, x, y);
+  });
+}
+
+int edhip_apply_rows(edhip_op* op, int prec, int transpose, int64_t row_begin, int64_t row_end, int64_t n_x,
+                     const void* x, int64_t n_y, void* y) {
+  if (!op) return EDHIP_ERR_ARGUMENT;
+  return guarded(op->ctx, [&] {
+    check_prec_for_op(op, prec);
+    const int64_t d = op->space->dim;
+    if (row_begin < 0 || row_end < row_begin || row_end > d)
+      fail(EDHIP_ERR_ARGUMENT, "rows [" + std::to_string(row_begin) + ", " + std::to_string(row_end) +
+                                   ") outside [0, " + std::to_string(d) + ")");
+    if (n_x != d)
+      fail(EDHIP_ERR_DIMENSION, "state has length " + std::to_string(n_x) + " != dimension " + std::to_string(d));
+    if (n_y != row_end - row_begin)
+      fail(EDHIP_ERR_DIMENSION, "out has length " + std::to_string(n_y) + " != " + std::to_string(row_end - row_begin) +
+                                    " rows");
+    term_list_matvec(op, prec, transpose ? 1 : 0, 0, x, y, row_begin, row_end);
   });
 }
 

@@ -96,7 +96,8 @@ struct OpDev {
   GroupLdsLayout lay;    // LDS layout of the group tables
   const uint64_t* rows;  // basis words (the space's sorted keys)
   const uint16_t* orbit; // orbit sizes (reduced) or nullptr
-  int64_t dim;
+  int64_t dim;           // rows of the space (products: the end of the rows computed)
+  int64_t row_begin;     // products: first row computed; y holds rows row_begin .. dim - 1 from y[0]
   int n_off;             // number of off-diagonal terms (in the row or column sense)
   int n_diag;
   int word_bits;         // bits of the words and of every diagonal term's mask and match

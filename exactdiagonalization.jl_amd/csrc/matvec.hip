@@ -183,7 +183,7 @@ __global__ __launch_bounds__(kMvBlock) void k_matvec(OpDev op, const V* __restri
   uint16_t* codes = (uint16_t*)(lds + op.codes_lds);
   if (op.xm.values) xmap_stage(op.xm, codes);
   __syncthreads();
-  const int64_t row = xcd_chunked_block(blockIdx.x, gridDim.x, op.xcd_chunk) * kMvBlock + threadIdx.x;
+  const int64_t row = op.row_begin + xcd_chunked_block(blockIdx.x, gridDim.x, op.xcd_chunk) * kMvBlock + threadIdx.x;
   if (row >= op.dim) return;
   using R = typename VT<V>::R;
   const uint64_t b = op.rows[row];
@@ -269,12 +269,13 @@ __global__ __launch_bounds__(kMvBlock) void k_matvec(OpDev op, const V* __restri
       }
     }
   }
-  y[row] = accumulate ? vadd(y[row], acc) : acc;
+  V* yr = y + (row - op.row_begin);
+  *yr = accumulate ? vadd(*yr, acc) : acc;
 }
 
 template <typename V, int MODE>
 hipError_t launch_mv_t(const OpDev& op, int transpose, int accumulate, const void* x, void* y, hipStream_t st) {
-  dim3 grid((unsigned)((op.dim + kMvBlock - 1) / kMvBlock));
+  dim3 grid((unsigned)((op.dim - op.row_begin + kMvBlock - 1) / kMvBlock));
   // unit-phase kernels only where the canonicaliser can drop the element (row-pair tori)
   const bool unit = MODE >= kGroupTorusPairs && op.g.unit_phases;
   const uint32_t lds = op.xm.values ? op.codes_lds + xmap_lds_bytes(op.xm) : op.lds_total;
@@ -1164,7 +1165,7 @@ hipError_t launch_diag_values(const OpDev& op, double* d_re, double* d_im, hipSt
 
 hipError_t launch_matvec(const OpDev& op, int prec, int transpose, int accumulate, const void* x, void* y,
                          hipStream_t st) {
-  if (op.dim == 0) return hipSuccess;
+  if (op.dim <= op.row_begin) return hipSuccess;
   switch (prec) {
     case kF64:
       return launch_mv_v<double>(op, transpose, accumulate, x, y, st);

@@ -635,6 +635,40 @@ class AbstractOperatorRepresentation:
             out_arr[...] = y
         return out_arr
 
+    def mul_rows(self, out, state, row_begin: int, row_end: int, transpose: bool = False):
+        """One share of a row-sharded product (no counterpart in the
+        reference): ``out[i] = (opr * state)[row_begin + i]`` for the rows
+        ``[row_begin, row_end)``, with the whole ``state`` given (the input a
+        GPU holds after an all-gather); ``transpose`` takes ``state * opr``.
+        Vectors are GPU tensors of one dtype, or numpy arrays (copied through
+        the device).  Every row equals the whole product's bitwise."""
+        import torch
+
+        lib = _lib.lib()
+        host = not (_is_torch(out) and _is_torch(state))
+        if host:
+            if _is_torch(out) or _is_torch(state):
+                raise ArgumentError("vectors must both be GPU tensors or both numpy arrays")
+            if not isinstance(out, np.ndarray) or _lib.DTYPE_PREC.get(out.dtype) is None:
+                raise ArgumentError("out must be a numpy array of float64/complex128/float32/complex64")
+            st = np.asarray(state)
+            if np.iscomplexobj(st) and not np.iscomplexobj(out):
+                raise ArgumentError("InexactError: complex state into a real output vector")
+            xs = torch.from_numpy(np.ascontiguousarray(st, dtype=out.dtype)).cuda()
+            ys = torch.empty(out.size, dtype=xs.dtype, device=xs.device)
+        else:
+            if not (out.is_cuda and state.is_cuda) or state.dtype != out.dtype or _torch_prec(out) is None:
+                raise ArgumentError("device vectors must be GPU tensors of one of float64/complex128/float32/complex64")
+            if not (out.is_contiguous() and state.is_contiguous()):
+                raise ArgumentError("device vectors must be contiguous")
+            xs, ys = state, out
+        _device_call(self.ctx, lib.edhip_apply_rows, self._handle, _torch_prec(ys), int(transpose), int(row_begin),
+                     int(row_end), xs.numel(), ctypes.c_void_p(xs.data_ptr()), ys.numel(),
+                     ctypes.c_void_p(ys.data_ptr()))
+        if host:
+            out[...] = ys.cpu().numpy()
+        return out
+
     def __matmul__(self, state):
         """opr * state (operator_representation.jl:122-129): a new vector of the promoted type."""
         st = np.asarray(state)

@@ -190,6 +190,16 @@ int edhip_apply(edhip_op* op, int prec, int transpose, int accumulate, int64_t n
 /* same on host vectors (copied to and from the device) */
 int edhip_apply_host(edhip_op* op, int prec, int transpose, int accumulate, int64_t n_x, const void* x_host,
                      int64_t n_y, void* y_host);
+/* One share of a product sharded by rows (no counterpart in the reference;
+ * the per-GPU part of a product over contiguous row ranges, with the whole
+ * input vector resident, as after an all-gather): y[i] = (opr * state)[row_begin
+ * + i] (transpose = 1: (state * opr)[row_begin + i]) for i < row_end -
+ * row_begin, overwriting y, by the term-list kernel whatever the strategy.
+ * Rows outside [0, dimension] give EDHIP_ERR_ARGUMENT; n_x must equal the
+ * dimension and n_y the row count (EDHIP_ERR_DIMENSION).  Every row comes out
+ * bitwise as in the whole product. */
+int edhip_apply_rows(edhip_op* op, int prec, int transpose, int64_t row_begin, int64_t row_end, int64_t n_x,
+                     const void* x, int64_t n_y, void* y);
 
 /* Kernel and lookup used by apply/mul/eigensolver calls (performance knob;
  * every choice gives the same matrix).  The automatic choice (0, 0) is the
