@@ -230,7 +230,7 @@ def cpu_baseline(ed):
     }
 
 
-def lanczos_times(ed, opr, tol_fp32=0.0):
+def lanczos_times(ed, opr, tol_fp32=0.0, reorth="auto"):
     """Time to E0 at 1e-10 relative (eigenvalue criterion r^2/gap) for the
     FP32-storage mixed Lanczos (with the FP64 residual correction) and the
     FP64 Lanczos.  ``seconds`` runs from the start vector to E0; the one-time
@@ -239,7 +239,7 @@ def lanczos_times(ed, opr, tol_fp32=0.0):
     out = {}
     for mode in ("fp64", "mixed"):
         r = ed.eigs(opr, 1, mode=mode, tol=1e-10, max_basis=100, max_restarts=20, criterion="eigenvalue",
-                    tol_fp32=tol_fp32)
+                    tol_fp32=tol_fp32, reorth=reorth)
         out[mode] = {
             "E0": float(r.values[0]),
             "seconds": r.t_total,
@@ -250,6 +250,8 @@ def lanczos_times(ed, opr, tol_fp32=0.0):
             "matvec_seconds": r.t_matvec,
             "refine_seconds": r.t_refine,
             "other_seconds": r.t_total - r.t_matvec - r.t_refine,
+            "reorth": reorth,
+            "full_reorth_steps": r.full_reorth_steps,
             "reorth_second_passes": r.second_passes,
             # orthogonalisation kernels (dot products and updates against the
             # basis): device time from HIP events, algorithmic bytes over it
@@ -641,6 +643,13 @@ def main():
     line["row_shares"] = share_times(ed, opr)
     if not args.no_e0:
         line["time_to_E0"] = lanczos_times(ed, opr, args.tol_fp32)
+        # the same runs with every step orthogonalised against the whole basis
+        full = lanczos_times(ed, opr, args.tol_fp32, reorth="adaptive")
+        line["time_to_E0_full_reorth"] = {
+            m: {k: full[m][k] for k in ("E0", "seconds", "matvecs", "ortho_kernel_seconds", "full_reorth_steps")}
+            for m in ("fp64", "mixed")
+        }
+        line["time_to_E0_full_reorth"]["speedup_mixed_vs_fp64"] = full["speedup_mixed_vs_fp64"]
     if not args.no_assembled:
         line["assembled_rows"] = assembled_times(ed, opr, elems, args.steps, args.warmup, e0=not args.no_e0)
     if not args.no_grad:

@@ -99,6 +99,7 @@ class LanczosStats(ctypes.Structure):
         ("t_alloc", ctypes.c_double),
         ("t_ortho", ctypes.c_double),
         ("ortho_bytes", ctypes.c_double),
+        ("full_reorth_steps", ctypes.c_int),
     ]
 
 

@@ -1519,6 +1519,18 @@ struct TRLanczos {
   // orthonormal B, ||w - B c||^2 = ||w||^2 - ||c||^2 needs no extra pass.
   static constexpr double kReorthKeep = 1e-2;
   int second_passes = 0;
+  // Partial reorthogonalisation (Simon, "The Lanczos algorithm with partial
+  // reorthogonalization", Math. Comp. 42 (1984) 115-142) in the first cycle:
+  // each step orthogonalises against the two latest vectors only, and the
+  // omega recurrence estimates the new vector's overlaps with the older ones
+  // from alpha and beta; when one exceeds sqrt(eps) (eps: the storage
+  // precision's unit roundoff) the vector, and the next one, are
+  // orthogonalised against the whole basis.  The basis stays semi-orthogonal,
+  // which keeps the Ritz values accurate to the working precision.  After a
+  // thick restart, an injected direction or with locked vectors every step
+  // orthogonalises against the whole basis.
+  bool partial = false;
+  int full_steps = 0;  // steps orthogonalised against the whole basis
 
   // w -= B (B^H w) over vectors B_0..B_{k-1} (stride ldb), once or twice;
   // returns the first-pass coefficients (host).
@@ -1633,6 +1645,11 @@ struct TRLanczos {
     int k = 0;  // kept Ritz vectors after a restart
     bool converged = false;
     std::vector<double> w, Z, prev;
+    // omega recurrence: om_cur[q] ~ <v_j, v_q>, om_prev[q] ~ <v_{j-1}, v_q>
+    const double eps = (prec == EDHIP_F32 || prec == EDHIP_C64) ? 0x1p-24 : 0x1p-53;
+    const double om_thr = std::sqrt(eps);
+    std::vector<double> om_prev, om_cur(1, 1.0), om_new;
+    bool om_force = false;
     int j = 0;
     double beta_last = 0.0;
     int size = 0;  // current basis size used for the final Ritz step
@@ -1651,10 +1668,48 @@ struct TRLanczos {
         matvec(vj, wv);
         ++iterations;
         if (n_locked) orthogonalize(locked, n, n_locked, wv);
-        std::vector<double> c = orthogonalize(Vp, ld, j + 1, wv);
-        T[(size_t)j * mm + j] = c[2 * (size_t)j];
+        const bool pro = partial && k == 0 && restarts == 0 && injections == 0 && n_locked == 0;
+        if (pro) {  // the three-term recurrence: v_{j-1} and v_j only
+          const int lo = j > 0 ? j - 1 : 0;
+          std::vector<double> c = orthogonalize(vec(lo), ld, j + 1 - lo, wv);
+          T[(size_t)j * mm + j] = c[2 * (size_t)(j - lo)];
+        } else {
+          std::vector<double> c = orthogonalize(Vp, ld, j + 1, wv);
+          T[(size_t)j * mm + j] = c[2 * (size_t)j];
+          ++full_steps;
+        }
         std::complex<double> nn = dot(wv, wv);
         double beta = std::sqrt(std::max(nn.real(), 0.0));
+        if (pro && beta > 0.0) {
+          // omega_{j+1,q} beta_j = beta_q omega_{j,q+1} + (alpha_q - alpha_j) omega_{j,q}
+          //                        + beta_{q-1} omega_{j,q-1} - beta_{j-1} omega_{j-1,q}
+          // plus a rounding term of the sign of the sum
+          om_new.assign((size_t)j + 2, 0.0);
+          const double a_j = T[(size_t)j * mm + j];
+          const double b_jm1 = j > 0 ? T[(size_t)j * mm + j - 1] : 0.0;
+          double worst = 0.0;
+          for (int q = 0; q < j; ++q) {
+            const double b_q = T[(size_t)(q + 1) * mm + q];
+            double v = b_q * om_cur[(size_t)q + 1] + (T[(size_t)q * mm + q] - a_j) * om_cur[(size_t)q] -
+                       b_jm1 * om_prev[(size_t)q];
+            if (q > 0) v += T[(size_t)q * mm + q - 1] * om_cur[(size_t)q - 1];
+            v += std::copysign(eps * (b_q + beta), v);
+            om_new[(size_t)q] = v / beta;
+            worst = std::max(worst, std::fabs(om_new[(size_t)q]));
+          }
+          om_new[(size_t)j] = eps;
+          om_new[(size_t)j + 1] = 1.0;
+          if (om_force || worst > om_thr) {
+            orthogonalize(Vp, ld, j + 1, wv);
+            ++full_steps;
+            nn = dot(wv, wv);
+            beta = std::sqrt(std::max(nn.real(), 0.0));
+            for (int q = 0; q <= j; ++q) om_new[(size_t)q] = eps;
+            om_force = !om_force;  // the next vector too
+          }
+          om_prev.swap(om_cur);
+          om_cur.swap(om_new);
+        }
         size = j + 1;
         const bool last = (j + 1 == mm);
         const bool exhausted = (int64_t)(j + 1) >= dim - n_locked;
@@ -2576,10 +2631,12 @@ int edhip_lanczos(edhip_op* op, const edhip_lanczos_opts* opts, double* evals, v
     edhip_lanczos_stats st{};
     std::vector<double> theta, res;
 
-    if (opts->reorth < 0 || opts->reorth > 1) fail(EDHIP_ERR_ARGUMENT, "reorth must be 0 or 1");
+    if (opts->reorth < 0 || opts->reorth > 3) fail(EDHIP_ERR_ARGUMENT, "reorth must be 0, 1, 2 or 3");
     TRLanczos L;
     L.op = op;
-    L.reorth = opts->reorth;
+    L.reorth = opts->reorth == 1 ? 1 : 0;
+    // auto: partial reorthogonalisation on bases over 2^22 elements
+    L.partial = opts->reorth == 2 || (opts->reorth == 0 && n > (int64_t)1 << 22);
     L.seed = opts->seed;
     L.prec = prec_store;
     L.set_size(n);
@@ -2628,6 +2685,7 @@ int edhip_lanczos(edhip_op* op, const edhip_lanczos_opts* opts, double* evals, v
     st.restarts = L.restarts;
     st.matvecs = L.matvecs;
     st.second_passes = L.second_passes;
+    st.full_reorth_steps = L.full_steps;
     st.t_matvec = L.t_matvec;
     st.t_ortho = L.t_ortho;
     st.ortho_bytes = L.ortho_bytes;
@@ -2648,7 +2706,7 @@ int edhip_lanczos(edhip_op* op, const edhip_lanczos_opts* opts, double* evals, v
       for (int i = 0; i < nev; ++i) {
         TRLanczos R;
         R.op = op;
-        R.reorth = opts->reorth;
+        R.reorth = opts->reorth == 1 ? 1 : 0;
         R.seed = opts->seed + 0x632BE59BD9B4E019ull * (uint64_t)(i + 1);
         R.prec = prec64;
         R.set_size(n);
@@ -2674,6 +2732,7 @@ int edhip_lanczos(edhip_op* op, const edhip_lanczos_opts* opts, double* evals, v
         res[i] = rs1.empty() ? 0.0 : rs1[0];
         st.matvecs_fp64 += R.matvecs;
         st.second_passes += R.second_passes;
+        st.full_reorth_steps += R.full_steps;
         st.t_ortho += R.t_ortho;
         st.ortho_bytes += R.ortho_bytes;
         conv = conv && ok;

@@ -36,12 +36,13 @@ class EigsResult:
     second_passes: int = 0
     t_ortho: float = 0.0
     ortho_bytes: float = 0.0
+    full_reorth_steps: int = 0
 
 
 def eigs(opr, nev: int = 1, which: str = "SR", *, mode: str = "fp64", tol: float = 1e-10, max_basis: int = 120,
          max_restarts: int = 50, seed: int = 20241202, return_vectors: bool = False, refine_basis: int = 12,
          max_refine: int = 20, check_every: int = 5, criterion: str = "residual",
-         reorth: str = "adaptive", tol_fp32: float = 0.0) -> EigsResult:
+         reorth: str = "auto", tol_fp32: float = 0.0) -> EigsResult:
     """Lowest ``nev`` eigenvalues of a Hermitian operator representation.
 
     mode="fp64": FP64 (ComplexF64) vectors.  mode="mixed": FP32 (ComplexF32)
@@ -61,7 +62,13 @@ def eigs(opr, nev: int = 1, which: str = "SR", *, mode: str = "fp64", tol: float
 
     reorth="adaptive": one Gram-Schmidt pass against the Krylov basis per step,
     a second when the first cancelled more than 99% of the vector's norm
-    squared; reorth="twice": always two passes.
+    squared; reorth="twice": always two passes; reorth="partial": partial
+    reorthogonalisation (Simon 1984) in the first cycle, the two latest
+    vectors at every step and the whole basis only when the omega recurrence
+    estimates an overlap above sqrt(eps) of the storage precision;
+    reorth="auto": "partial" for vectors over 2^22 elements, else "adaptive".
+    ``full_reorth_steps`` counts the steps orthogonalised against the whole
+    basis.
     """
     if which != "SR":
         raise ArgumentError("only which='SR' (smallest real) is supported")
@@ -76,9 +83,10 @@ def eigs(opr, nev: int = 1, which: str = "SR", *, mode: str = "fp64", tol: float
     o.max_refine = int(max_refine)
     o.check_every = int(check_every)
     o.criterion = 1 if criterion == "eigenvalue" else 0
-    if reorth not in ("adaptive", "twice"):
-        raise ArgumentError("reorth must be 'adaptive' or 'twice'")
-    o.reorth = 0 if reorth == "adaptive" else 1
+    modes = {"auto": 0, "twice": 1, "partial": 2, "adaptive": 3}
+    if reorth not in modes:
+        raise ArgumentError("reorth must be 'auto', 'adaptive', 'partial' or 'twice'")
+    o.reorth = modes[reorth]
     o.seed = int(seed)
     o.tol = float(tol)
     o.tol_fp32 = float(tol_fp32)
@@ -115,6 +123,7 @@ def eigs(opr, nev: int = 1, which: str = "SR", *, mode: str = "fp64", tol: float
         t_alloc=st.t_alloc,
         t_ortho=st.t_ortho,
         ortho_bytes=st.ortho_bytes,
+        full_reorth_steps=st.full_reorth_steps,
     )
 
 

@@ -302,9 +302,19 @@ typedef struct {
   int criterion;     /* 0: residual ||H y - theta y|| <= tol*max(1,|theta|);
                         1: also accept r^2/gap <= tol*max(1,|theta|) with theta
                         stationary between checks (eigenvalue accuracy) */
-  int reorth;        /* 0: one classical Gram-Schmidt pass against the basis,
-                        a second one when the first removed more than 99% of
-                        ||w||^2; 1: always two passes (CGS2) */
+  int reorth;        /* 0 (auto): 2 on bases over 2^22 elements, else 3;
+                        1: two classical Gram-Schmidt passes against the
+                        basis at every step (CGS2);
+                        2: partial reorthogonalisation (Simon 1984) in the
+                        first cycle: the two latest vectors at every step,
+                        the whole basis only when the omega recurrence
+                        estimates an overlap above sqrt(eps) of the storage
+                        precision (that step and the next); every step after
+                        a thick restart or an injected direction as 3;
+                        3: one pass against the basis at every step, a
+                        second one when the first removed more than 99% of
+                        ||w||^2.  The FP64 refinement of the mixed mode
+                        uses 1 or 3. */
   uint64_t seed;     /* start vector: SplitMix64 + Box-Muller Gaussian */
   double tol;        /* see criterion */
   double tol_fp32;   /* mixed mode: eigenvalue-criterion tolerance of the FP32
@@ -331,6 +341,8 @@ typedef struct {
                               counted for vectors over 2^22 elements */
   double ortho_bytes;      /* their algorithmic bytes: per pass (2k + 3)
                               vectors of n elements for k basis vectors */
+  int full_reorth_steps;   /* steps orthogonalised against the whole basis
+                              (every step unless reorth is partial) */
 } edhip_lanczos_stats;
 
 void edhip_lanczos_default_opts(edhip_lanczos_opts* opts);

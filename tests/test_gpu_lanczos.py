@@ -66,9 +66,11 @@ def test_eigenvectors(ed):
 
 
 @pytest.mark.parametrize("name,shape,k,nev", [("chain24_k3", (24, 1), (3, 0), 4), ("torus4x6_pi0", (4, 6), (2, 0), 1)])
-def test_full_size_configs_vs_arpack(ed, oracle, name, shape, k, nev):
+@pytest.mark.parametrize("reorth", ["auto", "partial"])
+def test_full_size_configs_vs_arpack(ed, oracle, name, shape, k, nev, reorth):
     """The 24-site configurations: FP64 Lanczos vs ARPACK on the oracle's
-    sparse matrix; mixed precision vs FP64 at 1e-10 relative."""
+    sparse matrix; mixed precision vs FP64 at 1e-10 relative; with full
+    (auto at this size) and partial reorthogonalisation."""
     n = shape[0] * shape[1]
     hs, _ = ed.spin_half_system(n)
     if shape[1] == 1:
@@ -81,10 +83,14 @@ def test_full_size_configs_vs_arpack(ed, oracle, name, shape, k, nev):
     perms = np.stack([ed.bit_permutation(hs, p) for p, _ in irrep.symops_and_amplitudes()])
     red = oracle.symmetry_reduce(oracle.sector_basis(n, 0), n, perms, [a for _, a in irrep.symops_and_amplitudes()])
     ref = oracle.lowest_eigenvalues(oracle.Opr(red, ed.terms_arrays(H)), nev)
-    r64 = ed.eigs(opr, nev, tol=1e-12, max_basis=200)
+    r64 = ed.eigs(opr, nev, tol=1e-12, max_basis=200, reorth=reorth)
     assert np.all(np.abs(r64.values - ref) < 1e-12 * np.abs(ref))
-    rmx = ed.eigs(opr, nev, tol=1e-11, mode="mixed", max_basis=200, criterion="eigenvalue")
+    rmx = ed.eigs(opr, nev, tol=1e-11, mode="mixed", max_basis=200, criterion="eigenvalue", reorth=reorth)
     assert np.all(np.abs(rmx.values - r64.values) < 1e-10 * np.abs(r64.values))
+    if reorth == "partial":
+        assert r64.full_reorth_steps < r64.iterations // 2
+    else:
+        assert r64.full_reorth_steps == r64.iterations
 
 
 @pytest.mark.parametrize("mode", ["fp64", "mixed"])
@@ -101,12 +107,16 @@ def test_reorth_adaptive_matches_twice(ed, golden, mode):
     kw = dict(tol=1e-12 if mode == "fp64" else 1e-11, mode=mode, max_basis=40, max_restarts=400)
     a = ed.eigs(opr, 4, reorth="adaptive", return_vectors=True, **kw)
     t = ed.eigs(opr, 4, reorth="twice", **kw)
+    p = ed.eigs(opr, 4, reorth="partial", return_vectors=True, **kw)
     bar = 1e-11 if mode == "fp64" else 1e-10
-    assert a.converged and t.converged and a.restarts > 0
+    assert a.converged and t.converged and p.converged and a.restarts > 0
     assert np.all(np.abs(a.values - ref) < bar * np.abs(ref))
     assert np.all(np.abs(t.values - ref) < bar * np.abs(ref))
+    assert np.all(np.abs(p.values - ref) < bar * np.abs(ref))
     assert a.second_passes < t.second_passes
     V = a.vectors.cpu().numpy()
+    assert np.abs(V @ V.T - np.eye(4)).max() < 1e-9
+    V = p.vectors.cpu().numpy()
     assert np.abs(V @ V.T - np.eye(4)).max() < 1e-9
     with pytest.raises(ed.ArgumentError):
         ed.eigs(opr, 1, reorth="never")
@@ -152,7 +162,8 @@ def _zz_chain(ed, L):
 
 @pytest.mark.parametrize("mode", ["fp64", "mixed"])
 @pytest.mark.parametrize("max_basis", [12, 120])
-def test_invariant_subspace_degenerate_levels(ed, mode, max_basis):
+@pytest.mark.parametrize("reorth", ["auto", "partial"])
+def test_invariant_subspace_degenerate_levels(ed, mode, max_basis, reorth):
     """A diagonal operator: the Krylov space of one start vector holds one copy
     of each distinct eigenvalue (4 here), so the lowest three levels with
     their multiplicities (-8, -8, -4 for sum_i sigma^z_i sigma^z_{i+1} on the
@@ -172,7 +183,8 @@ def test_invariant_subspace_degenerate_levels(ed, mode, max_basis):
         diag += 1 - 2 * (bi ^ bj)
     ref = np.sort(diag)[:3]
     assert ref.tolist() == [-8.0, -8.0, -4.0]
-    r = ed.eigs(opr, 3, tol=1e-12, mode=mode, max_basis=max_basis, max_restarts=400, return_vectors=True)
+    r = ed.eigs(opr, 3, tol=1e-12, mode=mode, max_basis=max_basis, max_restarts=400, return_vectors=True,
+                reorth=reorth)
     assert r.converged
     assert np.all(np.abs(r.values - ref) < 1e-11), r.values
     V = r.vectors.cpu().numpy()
@@ -182,12 +194,13 @@ def test_invariant_subspace_degenerate_levels(ed, mode, max_basis):
 
 
 @pytest.mark.parametrize("mode", ["fp64", "mixed"])
-def test_invariant_subspace_zero_operator(ed, mode):
+@pytest.mark.parametrize("reorth", ["auto", "partial"])
+def test_invariant_subspace_zero_operator(ed, mode, reorth):
     """sum_i sigma^z_i vanishes on the Sz = 0 sector: every Krylov block has one
     vector; four zero eigenvalues come back converged."""
     L = 10
     hs, s = ed.spin_half_system(L)
     op = sum((s(i, "z") for i in range(L)), ed.NullOperator())
     opr = ed.represent(ed.represent(ed.HilbertSpaceSector(hs, 0)), op)
-    r = ed.eigs(opr, 4, tol=1e-12, mode=mode, max_basis=20, max_restarts=50)
+    r = ed.eigs(opr, 4, tol=1e-12, mode=mode, max_basis=20, max_restarts=50, reorth=reorth)
     assert r.converged and np.all(np.abs(r.values) < 1e-12)
