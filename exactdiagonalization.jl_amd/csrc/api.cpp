@@ -1165,8 +1165,11 @@ bool ensure_diag(edhip_op* op) {
 // stays cache-sized, and where it is dense (at most kXMapDenseRatio positions
 // per basis word, as the colex form of single-popcount spaces gives: 6x6
 // FP64 127 -> 91 ms; the sparser plain form, 18 positions per word, is no
-// faster than the blocked rank index there).  Returns whether it did.
-bool attach_spread(edhip_op* op, int prec, OpDev& d, const void* x, hipStream_t st) {
+// faster than the blocked rank index there).  scaled: the spread holds
+// x / sqrt(N) on reduced spaces (the products then weigh an image without
+// its orbit size, and their canonicalisers skip counting the stabiliser).
+// Returns whether it did.
+bool attach_spread(edhip_op* op, int prec, OpDev& d, const void* x, hipStream_t st, bool scaled = false) {
   const edhip_space* sp = op->space;
   const bool rank_mode = sp->is_sector && !sp->group;
   const int idx_lookup = (op->strat_lookup == 2) ? 0 : op->strat_lookup;
@@ -1177,8 +1180,10 @@ bool attach_spread(edhip_op* op, int prec, OpDev& d, const void* x, hipStream_t 
   if (!want || !d.codes_lds) return false;
   const edhip_space::XMapHost* X = sp->ensure_xmap(es);
   if (!X) return false;
-  HIP_OR_THROW(launch_xmap_fill(prec, X->dev, sp->keys.as<uint64_t>(), sp->dim, x, st));
+  const uint16_t* orbit = (scaled && sp->group) ? sp->orbit.as<uint16_t>() : nullptr;
+  HIP_OR_THROW(launch_xmap_fill(prec, X->dev, sp->keys.as<uint64_t>(), sp->dim, x, orbit, st));
   d.xm = X->dev;
+  d.xm.scaled = orbit ? 1 : 0;
   // every image of a term that keeps the popcount has the space's popcount:
   // the products skip the test
   if (op->keeps_popcount) d.xm.pop = -1;
@@ -1343,7 +1348,7 @@ void term_list_matvec(edhip_op* op, int prec, int transpose, int accumulate, con
   }
   op->used_kernel = 1;
   op->used_lookup = rank_mode ? 1 : (d.ix.bm_bits ? 3 : 1);
-  if (attach_spread(op, prec, d, x, st)) op->used_lookup = 4;
+  if (attach_spread(op, prec, d, x, st, true)) op->used_lookup = 4;
   if (idx_lookup == 4 && !d.xm.values) fail(EDHIP_ERR_ARGUMENT, "x cannot be spread over the words of this space");
   if (op->ctx->kernel_timing) HIP_OR_THROW(hipEventRecord(op->ctx->ev[14], st));
   HIP_OR_THROW(launch_matvec(d, prec, transpose ? 1 : 0, accumulate ? 1 : 0, x, y, st));

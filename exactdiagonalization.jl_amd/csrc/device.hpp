@@ -924,7 +924,7 @@ struct XMapDev {
   int hbits;                // colex: top_shift - 12
   int pop;                  // colex: the popcount of every word (a word of another popcount has no position);
                             // -1 where every image keeps the popcount (no test needed)
-  int pad_;
+  int scaled;               // 1: the spread holds x / sqrt(N) of each representative (orbit size N)
 };
 
 // LDS bytes of the code table and, when used, the colex tables after it.

@@ -25,9 +25,10 @@ hipError_t launch_enumerate_qn(uint64_t count, int n_sites, const QnSite* d_site
                                const int32_t* d_nxt, uint64_t* d_out, hipStream_t st);
 // flags[key >> shift] = 1 for every key
 hipError_t launch_mark_tops(const uint64_t* d_keys, int64_t dim, int shift, uint32_t* d_flags, hipStream_t st);
-// x spread over the word positions of X (before each product).
+// x spread over the word positions of X (before each product); with d_orbit,
+// x / sqrt(orbit) (the orbit factor of the images, X.scaled).
 hipError_t launch_xmap_fill(int prec, const XMapDev& X, const uint64_t* d_keys, int64_t dim, const void* x,
-                            hipStream_t st);
+                            const uint16_t* d_orbit, hipStream_t st);
 // prefix[j] = lower_bound(keys, j << shift), j <= n_buckets
 hipError_t launch_build_prefix(const uint64_t* d_keys, int64_t dim, int shift, uint64_t n_buckets, uint32_t* d_prefix,
                                hipStream_t st);

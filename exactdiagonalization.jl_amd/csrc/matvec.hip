@@ -110,14 +110,15 @@ __device__ __forceinline__ MvTerms stage_mv_terms(const OpDev& op, unsigned char
 }
 
 // Weight of an off-diagonal element: a * phase^(conj if TRANS) * sqrt(N_own) / sqrt(N_img)
-template <int MODE, bool TRANS, bool UNIT = false>
+// (SCALED: without 1 / sqrt(N_img), which the spread x / sqrt(N) carries)
+template <int MODE, bool TRANS, bool UNIT = false, bool SCALED = false>
 __device__ __forceinline__ void element_weight_a(const OpDev& op, const GroupLds& L, double are, double aim, int h,
                                                  int cnt, double sqrt_own, double& wr, double& wi) {
   if constexpr (MODE == kGroupNone) {
     wr = are;
     wi = aim;
   } else {
-    const double sc = sqrt_own * L.sqcnt[cnt];
+    const double sc = SCALED ? sqrt_own : sqrt_own * L.sqcnt[cnt];
     if (UNIT || op.g.unit_phases) {
       wr = are * sc;
       wi = aim * sc;
@@ -171,7 +172,9 @@ constexpr int kMvBlock = 512;
 
 // UNIT: every character is 1 (op.g.unit_phases): the weight needs no phase,
 // so the canonicaliser does not track the element reaching the image.
-template <typename V, int MODE, bool TRANS, bool UNIT>
+// SCALED: x is read from its spread holding x / sqrt(N) (op.xm.scaled), so the
+// weight needs no orbit size of the image and the stabiliser goes uncounted.
+template <typename V, int MODE, bool TRANS, bool UNIT, bool SCALED>
 __global__ __launch_bounds__(kMvBlock) void k_matvec(OpDev op, const V* __restrict__ x, V* __restrict__ y, int accumulate) {
   extern __shared__ unsigned char lds[];
   GroupLds L{};
@@ -262,8 +265,8 @@ __global__ __launch_bounds__(kMvBlock) void k_matvec(OpDev op, const V* __restri
       for (int k = 0; k < NW; ++k) {
         if (ik[k] >= 0) {
           double wr, wi;
-          element_weight_a<MODE, TRANS, UNIT>(op, L, __longlong_as_double((long long)fa[k].y),
-                                        VT<V>::cx ? T.ai[t[k]] : 0.0, hk[k], ck[k], sqrt_own, wr, wi);
+          element_weight_a<MODE, TRANS, UNIT, SCALED>(op, L, __longlong_as_double((long long)fa[k].y),
+                                                VT<V>::cx ? T.ai[t[k]] : 0.0, hk[k], ck[k], sqrt_own, wr, wi);
           vfma(acc, (R)wr, (R)wi, xv[k]);
         }
       }
@@ -279,12 +282,22 @@ hipError_t launch_mv_t(const OpDev& op, int transpose, int accumulate, const voi
   // unit-phase kernels only where the canonicaliser can drop the element (row-pair tori)
   const bool unit = MODE >= kGroupTorusPairs && op.g.unit_phases;
   const uint32_t lds = op.xm.values ? op.codes_lds + xmap_lds_bytes(op.xm) : op.lds_total;
-#define EDHIP_MV(T, U)                                                                                                 \
-  hipLaunchKernelGGL((k_matvec<V, MODE, T, U>), grid, dim3(kMvBlock), lds, st, op, (const V*)x, (V*)y, accumulate)
+#define EDHIP_MV(T, U, S)                                                                                              \
+  hipLaunchKernelGGL((k_matvec<V, MODE, T, U, S>), grid, dim3(kMvBlock), lds, st, op, (const V*)x, (V*)y, accumulate)
+  if constexpr (MODE != kGroupNone) {
+    if (op.xm.values && op.xm.scaled) {
+      if (transpose) {
+        if (unit) EDHIP_MV(true, MODE >= kGroupTorusPairs, true); else EDHIP_MV(true, false, true);
+      } else {
+        if (unit) EDHIP_MV(false, MODE >= kGroupTorusPairs, true); else EDHIP_MV(false, false, true);
+      }
+      return hipGetLastError();
+    }
+  }
   if (transpose) {
-    if (unit) EDHIP_MV(true, MODE >= kGroupTorusPairs); else EDHIP_MV(true, false);
+    if (unit) EDHIP_MV(true, MODE >= kGroupTorusPairs, false); else EDHIP_MV(true, false, false);
   } else {
-    if (unit) EDHIP_MV(false, MODE >= kGroupTorusPairs); else EDHIP_MV(false, false);
+    if (unit) EDHIP_MV(false, MODE >= kGroupTorusPairs, false); else EDHIP_MV(false, false, false);
   }
 #undef EDHIP_MV
   return hipGetLastError();

@@ -314,12 +314,23 @@ __global__ __launch_bounds__(kBlock) void k_mark_tops(const uint64_t* __restrict
 // Before a product: x spread over the word positions (XMapDev).
 template <typename V>
 __global__ __launch_bounds__(kBlock) void k_xmap_fill(XMapDev X, const uint64_t* __restrict__ keys, int64_t dim,
-                                                      const V* __restrict__ x) {
+                                                      const V* __restrict__ x, const uint16_t* __restrict__ orbit) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= dim) return;
   const uint64_t k = keys[i];
   const uint64_t pos = xmap_pos(X, X.colex, X.topcode[k >> X.top_shift], k);
-  ((V*)X.values)[pos] = x[i];
+  V v = x[i];
+  if (orbit) {  // x / sqrt(N), scaled in double
+    using R = typename VT<V>::R;
+    const double sc = 1.0 / sqrt((double)orbit[i]);
+    if constexpr (VT<V>::cx) {
+      v.re = (R)((double)v.re * sc);
+      v.im = (R)((double)v.im * sc);
+    } else {
+      v = (R)((double)v * sc);
+    }
+  }
+  ((V*)X.values)[pos] = v;
 }
 
 __global__ void k_u32_to_u64(const uint32_t* __restrict__ in, uint64_t* __restrict__ out, uint64_t n) {
@@ -424,21 +435,23 @@ hipError_t launch_mark_tops(const uint64_t* d_keys, int64_t dim, int shift, uint
 }
 
 hipError_t launch_xmap_fill(int prec, const XMapDev& X, const uint64_t* d_keys, int64_t dim, const void* x,
-                            hipStream_t st) {
+                            const uint16_t* d_orbit, hipStream_t st) {
   if (dim == 0) return hipSuccess;
   dim3 grid(grid_for((uint64_t)dim));
   switch (prec) {
     case kF64:
-      hipLaunchKernelGGL(k_xmap_fill<double>, grid, dim3(kBlock), 0, st, X, d_keys, dim, (const double*)x);
+      hipLaunchKernelGGL(k_xmap_fill<double>, grid, dim3(kBlock), 0, st, X, d_keys, dim, (const double*)x, d_orbit);
       break;
     case kC128:
-      hipLaunchKernelGGL(k_xmap_fill<Cx<double>>, grid, dim3(kBlock), 0, st, X, d_keys, dim, (const Cx<double>*)x);
+      hipLaunchKernelGGL(k_xmap_fill<Cx<double>>, grid, dim3(kBlock), 0, st, X, d_keys, dim, (const Cx<double>*)x,
+                         d_orbit);
       break;
     case kF32:
-      hipLaunchKernelGGL(k_xmap_fill<float>, grid, dim3(kBlock), 0, st, X, d_keys, dim, (const float*)x);
+      hipLaunchKernelGGL(k_xmap_fill<float>, grid, dim3(kBlock), 0, st, X, d_keys, dim, (const float*)x, d_orbit);
       break;
     case kC64:
-      hipLaunchKernelGGL(k_xmap_fill<Cx<float>>, grid, dim3(kBlock), 0, st, X, d_keys, dim, (const Cx<float>*)x);
+      hipLaunchKernelGGL(k_xmap_fill<Cx<float>>, grid, dim3(kBlock), 0, st, X, d_keys, dim, (const Cx<float>*)x,
+                         d_orbit);
       break;
     default:
       return hipErrorInvalidValue;

@@ -155,4 +155,7 @@ def test_torus6x6_properties(ed):
     Hv32b = torch.empty_like(v32)
     ed.mul_(Hv32b, opr, v32)
     opr.set_strategy()
-    assert torch.equal(Hv32, Hv32b)  # same products in the same order: bitwise equal
+    # same images in the same order; the spread holds x / sqrt(N) rounded to
+    # FP32 where the rank path rounds the weight a sqrt(N_r / N_s): equal to
+    # FP32 rounding
+    assert (torch.linalg.norm(Hv32 - Hv32b) / torch.linalg.norm(Hv32b)).item() < 2e-7
