@@ -599,10 +599,12 @@ def main():
     # the dominant kernel alone (the product also spreads x before it)
     achieved = b_alg(4) / (kms32 * 1e-3) / 1e9
     traffic = None
+    traffic_all = {}
     tfile = os.path.join(ROOT, "profiles", f"traffic_matvec_{args.config}.json")
     if os.path.exists(tfile):
         with open(tfile) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch_f32")
+            traffic_all = json.load(f)
+        traffic = traffic_all.get("hbm_bytes_per_launch_f32")
 
     line = {
         "metric": "matvec Gelem/s + time-to-E0 @1e-10 rel, FP32-mixed vs FP64 GPU; grad time",
@@ -634,8 +636,11 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
-            "kernel": "k_matvec<float> (term list, 6x6 row-pair canonicaliser, x spread over word positions)",
+            "kernel": "k_matvec<float> (term list, 6x6 row-pair canonicaliser, x / sqrt(N) spread over word positions)",
             "kernel_ms": kms32,
+            # measured HBM bytes (rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated) over this run's kernel time
+            "traffic_GBps": traffic / (kms32 * 1e-3) / 1e9 if traffic else None,
+            "traffic_frac": traffic / (kms32 * 1e-3) / 1e9 / HBM_PEAK_GBS if traffic else None,
         },
         "checksum_f32": cs32,
         "checksum_f64": cs64,
@@ -651,7 +656,14 @@ def main():
         }
         line["time_to_E0_full_reorth"]["speedup_mixed_vs_fp64"] = full["speedup_mixed_vs_fp64"]
     if not args.no_assembled:
-        line["assembled_rows"] = assembled_times(ed, opr, elems, args.steps, args.warmup, e0=not args.no_e0)
+        asm = assembled_times(ed, opr, elems, args.steps, args.warmup, e0=not args.no_e0)
+        for sfx in ("f32", "f64"):
+            tb = traffic_all.get(f"assembled_hbm_bytes_per_launch_{sfx}")
+            if tb:
+                asm[sfx]["hbm_bytes"] = tb
+                asm[sfx]["hbm_GBps"] = tb / (asm[sfx]["kernel_ms"] * 1e-3) / 1e9
+                asm[sfx]["hbm_frac"] = asm[sfx]["hbm_GBps"] / HBM_PEAK_GBS
+        line["assembled_rows"] = asm
     if not args.no_grad:
         line["grad"] = grad_times(ed)
     if args.space_group:
