@@ -437,13 +437,15 @@ __device__ __forceinline__ void canonicalize_torus_pairs_t(const GroupDev& g, co
     bool full = true;
     if (Ly >= 3 && !first) {
       const int y3 = ((bit >> 3) + Ly - 2) % Ly;
-      const uint32_t r3 = (uint32_t)(b >> (y3 * Lx)) & rmask;
+      // row offsets as 24-bit products (v_mul_u32_u24, full rate; a 32-bit
+      // v_mul_lo_u32 issues at a quarter of the rate)
+      const uint32_t r3 = (uint32_t)(b >> __umul24((uint32_t)y3, (uint32_t)Lx)) & rmask;
       const uint32_t t3 = ((r3 << a) | (r3 >> (Lx - a))) & rmask;
       full = t3 <= ((uint32_t)(s >> ((Ly - 3) * Lx)) & rmask);
     }
     first = false;
     if (!full) continue;
-    const int sh = (Ly - 1 - (bit >> 3)) * Lx;  // row y moves to the top
+    const int sh = (int)__umul24((uint32_t)(Ly - 1 - (bit >> 3)), (uint32_t)Lx);  // row y moves to the top
     const uint64_t t = ((b << sh) | (b >> (NB - sh))) & FULL;
     const uint64_t img = ((t << a) & L.rot_hi[a]) | ((t >> (Lx - a)) & L.rot_lo[a]);
     const bool lt = img < s, eq = img == s;
