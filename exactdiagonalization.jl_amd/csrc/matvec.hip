@@ -10,6 +10,8 @@
 // and the element is a * phase[h] * sqrt(N_r / N_s) (row form) with
 // phase[h] = conj(chi(h^-1)); this equals a * U[b'] / U[b_r] of
 // reduced_operator_representation.jl:57-85 with U[g(s)] = conj(chi(g))/sqrt(N_s).
+#include <type_traits>
+
 #include "kernels.hpp"
 
 namespace edhip {
@@ -82,8 +84,10 @@ struct MvTerms {
 };
 
 // paired: the padded copy (op.n_mv > 0) with one mask per pair of terms;
-// else the terms of op.off one by one, indexed as in op.off.
-__device__ __forceinline__ MvTerms stage_mv_terms(const OpDev& op, unsigned char* lds, bool paired) {
+// else the terms of op.off one by one, indexed as in op.off.  amp32: the real
+// amplitude is kept as an FP32 value in the low half of its slot.
+__device__ __forceinline__ MvTerms stage_mv_terms(const OpDev& op, unsigned char* lds, bool paired,
+                                                  bool amp32 = false) {
   const int n = paired ? op.n_mv : op.n_off;
   const OffTerm* src = paired ? op.mv_off : op.off;
   uint64_t* mm = (uint64_t*)(lds + op.off_lds);  // at most 40 bytes per term, as reserved
@@ -103,7 +107,7 @@ __device__ __forceinline__ MvTerms stage_mv_terms(const OpDev& op, unsigned char
       mm[2 * t + 1] = o.match;
     }
     fa[2 * t] = o.match ^ o.image;
-    fa[2 * t + 1] = (uint64_t)__double_as_longlong(o.are);
+    fa[2 * t + 1] = amp32 ? (uint64_t)__float_as_uint((float)o.are) : (uint64_t)__double_as_longlong(o.are);
     ai[t] = o.aim;
   }
   return MvTerms{mm, fa, ai};
@@ -179,7 +183,11 @@ __global__ __launch_bounds__(kMvBlock) void k_matvec(OpDev op, const V* __restri
   extern __shared__ unsigned char lds[];
   GroupLds L{};
   if constexpr (MODE != kGroupNone) L = stage_group(op.g, op.lay, lds);
-  const MvTerms T = stage_mv_terms(op, lds, op.n_mv != 0);
+  // FP32 weights: a real FP32 product whose images need no phase and no orbit
+  // size (unit characters, scaled spread) weighs each image a * sqrt(N_r) in
+  // FP32, from FP32 amplitudes (no FP64 multiply and conversion per image)
+  constexpr bool kW32 = UNIT && SCALED && std::is_same<V, float>::value;
+  const MvTerms T = stage_mv_terms(op, lds, op.n_mv != 0, kW32);
   lds_copy(lds + op.w_lds, op.ix.W, op.w_bytes);
   const uint64_t* W = (const uint64_t*)(lds + op.w_lds);
   // code table of the spread x (the host sets xm only with room for it)
@@ -204,6 +212,7 @@ __global__ __launch_bounds__(kMvBlock) void k_matvec(OpDev op, const V* __restri
 
   double sqrt_own = 1.0;
   if constexpr (MODE != kGroupNone) sqrt_own = sqrt((double)op.orbit[row]);
+  const float so32 = (float)sqrt_own;
 
   const int n_terms = op.n_mv ? op.n_mv : op.n_off;
   for (int seg = 0; seg < n_terms; seg += 64) {
@@ -264,10 +273,14 @@ __global__ __launch_bounds__(kMvBlock) void k_matvec(OpDev op, const V* __restri
 #pragma unroll
       for (int k = 0; k < NW; ++k) {
         if (ik[k] >= 0) {
-          double wr, wi;
-          element_weight_a<MODE, TRANS, UNIT, SCALED>(op, L, __longlong_as_double((long long)fa[k].y),
-                                                VT<V>::cx ? T.ai[t[k]] : 0.0, hk[k], ck[k], sqrt_own, wr, wi);
-          vfma(acc, (R)wr, (R)wi, xv[k]);
+          if constexpr (kW32) {
+            vfma(acc, __uint_as_float((uint32_t)fa[k].y) * so32, 0.f, xv[k]);
+          } else {
+            double wr, wi;
+            element_weight_a<MODE, TRANS, UNIT, SCALED>(op, L, __longlong_as_double((long long)fa[k].y),
+                                                  VT<V>::cx ? T.ai[t[k]] : 0.0, hk[k], ck[k], sqrt_own, wr, wi);
+            vfma(acc, (R)wr, (R)wi, xv[k]);
+          }
         }
       }
     }
