@@ -215,6 +215,10 @@ constexpr uint64_t kXMapRatio = 32;
 // Spreads of 8- and 16-byte elements are used by default when they hold at
 // most this many positions per basis word
 constexpr uint64_t kXMapDenseRatio = 8;
+// and 4-byte spreads with at most this many (6x6 J1-J2 FP32 products: the
+// translation sector's 4.1 and 18.2 positions per word (colex and plain
+// forms) beat the rank index, 83 and 97 against 116 ms)
+constexpr uint64_t kXMapDenseRatio32 = 24;
 // Spreads of 8- and 16-byte elements are used by default only up to this
 // size (about what the last-level cache holds)
 constexpr uint64_t kXMapCachedBytes = 256ull << 20;
@@ -1158,14 +1162,15 @@ bool ensure_diag(edhip_op* op) {
 }
 
 // Points d at the spread of x over the word positions, filled with x, when
-// the operator's lookup strategy calls for it: lookup 4, and auto for 4-byte
-// elements (6x6 J1-J2 FP32: 118 -> 87 ms); for wider ones on full sectors,
-// whose other lookup is rank arithmetic, a chain of table reads per image
-// (4x7 J1-J2: FP32 14.9 -> 4.6 ms, FP64 14.9 -> 8.1 ms), where the spread
-// stays cache-sized, and where it is dense (at most kXMapDenseRatio positions
-// per basis word, as the colex form of single-popcount spaces gives: 6x6
-// FP64 127 -> 91 ms; the sparser plain form, 18 positions per word, is no
-// faster than the blocked rank index there).  scaled: the spread holds
+// the operator's lookup strategy calls for it: lookup 4, and auto on full
+// sectors, whose other lookup is rank arithmetic, a chain of table reads per
+// image (4x7 J1-J2: FP32 14.9 -> 4.6 ms, FP64 14.9 -> 8.1 ms), where the
+// spread stays cache-sized, and where it is dense (at most kXMapDenseRatio32
+// positions per basis word for 4-byte elements, 6x6 J1-J2 FP32: 118 -> 87
+// ms; kXMapDenseRatio for wider ones, the colex form of single-popcount
+// spaces: 6x6 FP64 127 -> 91 ms, where the sparser plain form, 18 positions
+// per word, is no faster than the blocked rank index) and x itself does not
+// fit the last-level cache.  scaled: the spread holds
 // x / sqrt(N) on reduced spaces (the products then weigh an image without
 // its orbit size, and their canonicalisers skip counting the stabiliser).
 // Returns whether it did.
@@ -1174,9 +1179,14 @@ bool attach_spread(edhip_op* op, int prec, OpDev& d, const void* x, hipStream_t 
   const bool rank_mode = sp->is_sector && !sp->group;
   const int idx_lookup = (op->strat_lookup == 2) ? 0 : op->strat_lookup;
   const size_t es = elem_size(prec);
+  // a dense spread only for vectors larger than the last-level cache: where x
+  // itself fits it, the rank index's x gathers hit it and the index was
+  // measured faster (6x6 space-group A1 sector, 31.6M words, 126 MB of FP32
+  // x: 57.8 ms against 67.0 ms through its spread)
+  const uint64_t ratio = es == 4 ? kXMapDenseRatio32 : kXMapDenseRatio;
+  const bool dense = sp->ensure_xtop() <= ratio * (uint64_t)sp->dim && (uint64_t)sp->dim * es > kXMapCachedBytes;
   const bool want = idx_lookup == 4 ||
-                    (idx_lookup == 0 && (es == 4 || rank_mode || sp->ensure_xtop() * es <= kXMapCachedBytes ||
-                                           sp->ensure_xtop() <= kXMapDenseRatio * (uint64_t)sp->dim));
+                    (idx_lookup == 0 && (rank_mode || sp->ensure_xtop() * es <= kXMapCachedBytes || dense));
   if (!want || !d.codes_lds) return false;
   const edhip_space::XMapHost* X = sp->ensure_xmap(es);
   if (!X) return false;

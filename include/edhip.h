@@ -224,9 +224,12 @@ int edhip_apply_rows(edhip_op* op, int prec, int transpose, int64_t row_begin, i
  *             kept).  An eigensolver or MINRES workspace that does not fit
  *             beside them drops them and sets the strategy back to auto.
  *   lookup: 0 auto (term-list kernel: x spread over word positions when
- *             it fits, for 4-byte elements, for full popcount sectors and
- *             where the spread is at most 256 MB; else bitmap, else sorted
- *             search, or rank arithmetic for a full popcount sector),
+ *             it fits, for full popcount sectors, where the spread is at
+ *             most 256 MB, and where it holds at most 24 (4-byte
+ *             elements) or 8 (wider ones) positions per basis word and
+ *             the vector itself exceeds 256 MB; else
+ *             bitmap, else sorted search, or rank arithmetic for a full
+ *             popcount sector),
  *           1 sorted search behind the prefix table (rank arithmetic for a
  *             full popcount sector),
  *           2 open-addressing table of the basis with the input vector
