@@ -532,10 +532,23 @@ def space_group_times(ed, cfg, steps, warmup):
         opr = ed.represent(r, H)
         ms32, _ = time_matvecs(ed, opr, _lib.F32, steps, warmup)
         ms64, _ = time_matvecs(ed, opr, _lib.F64, steps, warmup)
+        # eigs assembles the rows of these sectors for the solve (entries of a
+        # few GB); the matrix-free solve beside it
         res = ed.eigs(opr, 1, mode="mixed", tol=1e-10, max_basis=100, max_restarts=20, criterion="eigenvalue")
+        mf = ed.eigs(opr, 1, mode="mixed", tol=1e-10, max_basis=100, max_restarts=20, criterion="eigenvalue",
+                     assemble="off")
+        opr.set_strategy("assembled")
+        n_ent, n_bytes, t_build = opr.assembled_info()
+        a32 = kernel_ms(ed, opr, _lib.F32, steps)
+        a64 = kernel_ms(ed, opr, _lib.F64, steps)
+        opr.set_strategy()
         out[name] = {"group_order": order, "dim": r.dimension, "reduce_seconds": t_red, "matvec_ms_f32": ms32,
                      "matvec_ms_f64": ms64, "E0": float(res.values[0]), "time_to_E0_mixed_s": res.t_total,
-                     "matvecs": res.matvecs, "matvecs_fp64": res.matvecs_fp64}
+                     "assembled": res.assembled, "matvecs": res.matvecs, "matvecs_fp64": res.matvecs_fp64,
+                     "time_to_E0_mixed_matrix_free_s": mf.t_total, "E0_matrix_free": float(mf.values[0]),
+                     "assembled_entries": n_ent, "assembled_bytes": n_bytes, "assembled_build_s": t_build,
+                     "assembled_kernel_ms_f32": a32, "assembled_kernel_ms_f64": a64}
+        ed.release_workspace()
     return out
 
 

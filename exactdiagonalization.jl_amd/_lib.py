@@ -80,6 +80,7 @@ class LanczosOpts(ctypes.Structure):
         ("seed", ctypes.c_uint64),
         ("tol", ctypes.c_double),
         ("tol_fp32", ctypes.c_double),
+        ("assemble", ctypes.c_int),
     ]
 
 
@@ -100,6 +101,7 @@ class LanczosStats(ctypes.Structure):
         ("t_ortho", ctypes.c_double),
         ("ortho_bytes", ctypes.c_double),
         ("full_reorth_steps", ctypes.c_int),
+        ("assembled", ctypes.c_int),
     ]
 
 

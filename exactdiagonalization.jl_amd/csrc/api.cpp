@@ -1205,6 +1205,12 @@ bool attach_spread(edhip_op* op, int prec, OpDev& d, const void* x, hipStream_t 
   return true;
 }
 
+// The eigensolver assembles the rows of a reduced space over 2^22 words for
+// the time of a solve (kernel auto) when their arrays take at most this much
+// and fit beside its workspace (6x6 space-group A1 sector: 9.6 GB built in
+// 0.1 s, products 57.9 -> 9.8 ms, mixed time to E0 3.40 -> 0.61 s)
+constexpr uint64_t kAutoAsmMaxBytes = 16ull << 30;
+
 // Lanes per row and entries per lane per batch of the assembled product
 // (0: chosen from the mean entries per row)
 std::atomic<int> g_asm_lanes{0}, g_asm_batch{0};
@@ -1226,9 +1232,10 @@ void asm_shape(const edhip_op* op, int& lanes, int& batch) {
 // (matched terms), their offsets, then the entries written by the matvec's own
 // matching and canonicalisation.  Weight classes: the distinct amplitudes of
 // the terms times the distinct character phases of the group.  Leaves nothing
-// allocated when it fails (EDHIP_ERR_MEMORY when the arrays do not fit).
-void ensure_assembled(edhip_op* op) {
-  if (op->asm_built) return;
+// allocated when it fails (EDHIP_ERR_MEMORY when the arrays do not fit), or
+// when the arrays would take more than max_bytes (then returns false).
+bool ensure_assembled(edhip_op* op, uint64_t max_bytes = ~0ull) {
+  if (op->asm_built) return true;
   const auto t0 = std::chrono::steady_clock::now();
   const edhip_space* sp = op->space;
   hipStream_t st = op->ctx->stream;
@@ -1292,6 +1299,10 @@ void ensure_assembled(edhip_op* op) {
       HIP_OR_THROW(hipMemcpyAsync(&op->asm_nnz, op->asm_rowptr.as<uint64_t>() + D, 8, hipMemcpyDeviceToHost, st));
       HIP_OR_THROW(hipStreamSynchronize(st));
     }
+    if (op->asm_nnz * 4 + op->asm_rowptr.bytes > max_bytes) {
+      op->release_assembled();
+      return false;
+    }
     injected_alloc_failure(kAllocAssembled);
     op->asm_ent.alloc((size_t)op->asm_nnz * 4);
     upload(op->asm_acls, acls, st);
@@ -1310,6 +1321,7 @@ void ensure_assembled(edhip_op* op) {
                        op->asm_w.as<double>() + n_cls, n_cls, col_bits};
   op->asm_built = true;
   op->asm_build_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return true;
 }
 
 // Row-form product from the assembled rows: x / sqrt(N) first (reduced spaces),
@@ -2652,6 +2664,7 @@ int edhip_lanczos(edhip_op* op, const edhip_lanczos_opts* opts, double* evals, v
     std::vector<double> theta, res;
 
     if (opts->reorth < 0 || opts->reorth > 3) fail(EDHIP_ERR_ARGUMENT, "reorth must be 0, 1, 2 or 3");
+    if (opts->assemble < 0 || opts->assemble > 1) fail(EDHIP_ERR_ARGUMENT, "assemble must be 0 or 1");
     TRLanczos L;
     L.op = op;
     L.reorth = opts->reorth == 1 ? 1 : 0;
@@ -2679,6 +2692,33 @@ int edhip_lanczos(edhip_op* op, const edhip_lanczos_opts* opts, double* evals, v
     // the time to E0 starts once the workspace is there (a first call
     // allocates it, later calls of the same size reuse it)
     Timer total;
+    // assembled rows for the solve (kernel auto, opts->assemble 0), released
+    // when it ends (the guard also on errors)
+    struct AsmGuard {
+      edhip_op* op = nullptr;
+      ~AsmGuard() {
+        if (op) {
+          op->release_assembled();
+          op->strat_kernel = 0;
+        }
+      }
+    } asm_guard;
+    if (opts->assemble == 0 && op->strat_kernel == 0 && n > ((int64_t)1 << 22) && op->space->group) {
+      size_t free_b = 0, total_b = 0;
+      HIP_OR_THROW(hipMemGetInfo(&free_b, &total_b));
+      const uint64_t margin = 4ull << 30;
+      const uint64_t room = free_b > margin ? (uint64_t)free_b - margin : 0;
+      try {
+        if (ensure_assembled(op, std::min<uint64_t>(kAutoAsmMaxBytes, room))) {
+          op->strat_kernel = 3;
+          asm_guard.op = op;
+          st.assembled = 1;
+        }
+      } catch (const Error& e) {
+        if (e.code != EDHIP_ERR_MEMORY) throw;
+        (void)hipGetLastError();  // no room: the products stay matrix-free
+      }
+    }
     void* ritz = arena;
     L.Vp = arena + ritz_b;
     L.tmpp = arena + ritz_b + main_v;

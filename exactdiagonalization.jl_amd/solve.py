@@ -37,12 +37,13 @@ class EigsResult:
     t_ortho: float = 0.0
     ortho_bytes: float = 0.0
     full_reorth_steps: int = 0
+    assembled: bool = False
 
 
 def eigs(opr, nev: int = 1, which: str = "SR", *, mode: str = "fp64", tol: float = 1e-10, max_basis: int = 120,
          max_restarts: int = 50, seed: int = 20241202, return_vectors: bool = False, refine_basis: int = 12,
          max_refine: int = 20, check_every: int = 5, criterion: str = "residual",
-         reorth: str = "auto", tol_fp32: float = 0.0) -> EigsResult:
+         reorth: str = "auto", tol_fp32: float = 0.0, assemble: str = "auto") -> EigsResult:
     """Lowest ``nev`` eigenvalues of a Hermitian operator representation.
 
     mode="fp64": FP64 (ComplexF64) vectors.  mode="mixed": FP32 (ComplexF32)
@@ -69,6 +70,12 @@ def eigs(opr, nev: int = 1, which: str = "SR", *, mode: str = "fp64", tol: float
     reorth="auto": "partial" for vectors over 2^22 elements, else "adaptive".
     ``full_reorth_steps`` counts the steps orthogonalised against the whole
     basis.
+
+    assemble="auto": with the operator's kernel strategy on auto, a reduced
+    space over 2^22 states gets assembled rows for the time of the solve
+    (every row's images found once, then streamed by each product) when they
+    take at most 16 GiB and fit beside the workspace; ``assembled`` reports
+    it.  assemble="off": never.
     """
     if which != "SR":
         raise ArgumentError("only which='SR' (smallest real) is supported")
@@ -87,6 +94,9 @@ def eigs(opr, nev: int = 1, which: str = "SR", *, mode: str = "fp64", tol: float
     if reorth not in modes:
         raise ArgumentError("reorth must be 'auto', 'adaptive', 'partial' or 'twice'")
     o.reorth = modes[reorth]
+    if assemble not in ("auto", "off"):
+        raise ArgumentError("assemble must be 'auto' or 'off'")
+    o.assemble = 0 if assemble == "auto" else 1
     o.seed = int(seed)
     o.tol = float(tol)
     o.tol_fp32 = float(tol_fp32)
@@ -124,6 +134,7 @@ def eigs(opr, nev: int = 1, which: str = "SR", *, mode: str = "fp64", tol: float
         t_ortho=st.t_ortho,
         ortho_bytes=st.ortho_bytes,
         full_reorth_steps=st.full_reorth_steps,
+        assembled=bool(st.assembled),
     )
 
 

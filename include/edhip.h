@@ -322,6 +322,11 @@ typedef struct {
   double tol;        /* see criterion */
   double tol_fp32;   /* mixed mode: eigenvalue-criterion tolerance of the FP32
                         phase, max(tol, tol_fp32); 0 selects max(tol, 1e-10) */
+  int assemble;      /* 0 (auto): with the kernel strategy on auto, a reduced
+                        space over 2^22 words gets assembled rows (kernel 3)
+                        for the time of the solve when they take at most
+                        16 GiB and fit beside the workspace with 4 GiB to
+                        spare; 1: never */
 } edhip_lanczos_opts;
 
 typedef struct {
@@ -346,6 +351,7 @@ typedef struct {
                               vectors of n elements for k basis vectors */
   int full_reorth_steps;   /* steps orthogonalised against the whole basis
                               (every step unless reorth is partial) */
+  int assembled;           /* 1 if the solve assembled the operator's rows */
 } edhip_lanczos_stats;
 
 void edhip_lanczos_default_opts(edhip_lanczos_opts* opts);

@@ -224,3 +224,46 @@ def test_block_order_gives_bitwise_equal_products(ed, systems, strategy):
     finally:
         assert L.edhip_debug_block_order(32) == 0
     assert L.edhip_debug_block_order(65) != 0
+
+
+def test_eigs_assembles_large_reduced_spaces(ed):
+    """eigs assembles the rows of a reduced space over 2^22 states for the time
+    of the solve (6x6 J1-J2 space-group A1 sector, 31,566,122 states, about
+    9.6 GB of entries) and releases them; the eigenvalue equals the
+    matrix-free solve's; an allocation failure of the entries leaves the solve
+    matrix-free; an explicit assembled strategy is kept."""
+    from edhip import _lib
+
+    L = _lib.lib()
+    hs, _ = ed.spin_half_system(36)
+    j1, j2 = ed.square_bonds(6, 6)
+    H = _heis(ed, hs, j1 + j2, [1.0] * len(j1) + [0.5] * len(j2))
+    r = ed.symmetry_reduce(ed.represent(ed.HilbertSpaceSector(hs, 0)), ed.SquareSpaceIrrep(6, (0, 0), "A1"))
+    opr = ed.represent(r, H)
+    assert opr.dimension == 31566122
+    kw = dict(tol=1e-10, max_basis=100, criterion="eigenvalue")
+    try:
+        a = ed.eigs(opr, 1, mode="mixed", **kw)
+        assert a.assembled and a.converged
+        assert opr.assembled_info() == (0, 0, 0.0)
+        x = np.random.default_rng(1).standard_normal(opr.dimension).astype(np.float32)
+        y = np.zeros(opr.dimension, np.float32)
+        ed.mul_(y, opr, x)
+        assert opr.last_strategy()[0] == "terms"
+        m = ed.eigs(opr, 1, mode="mixed", assemble="off", **kw)
+        assert not m.assembled
+        assert abs(a.values[0] - m.values[0]) < 1e-10 * abs(m.values[0])
+        assert abs(m.values[0] - (-18.13714754)) < 1e-7
+        assert L.edhip_debug_fail_allocs(4, 1) == 0
+        f = ed.eigs(opr, 1, mode="fp64", **kw)
+        assert not f.assembled and abs(f.values[0] - m.values[0]) < 1e-10 * abs(m.values[0])
+        assert L.edhip_debug_fail_allocs(0, 0) == 0
+        opr.set_strategy("assembled")
+        e = ed.eigs(opr, 1, mode="fp64", **kw)
+        assert not e.assembled and opr.last_strategy() == ("assembled", None)
+        assert opr.assembled_info()[0] > 0
+        assert abs(e.values[0] - m.values[0]) < 1e-10 * abs(m.values[0])
+    finally:
+        L.edhip_debug_fail_allocs(0, 0)
+        opr.set_strategy()
+        ed.release_workspace()
