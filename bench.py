@@ -29,7 +29,7 @@ the build of its entries, its FP32 and FP64 products and both Lanczos modes
 on them.
 
 Options: --steps K --warmup W --gpus 1 [--config 4x6|6x6] [--no-cpu]
-[--no-grad] [--no-e0] [--no-assembled] [--space-group].
+[--no-grad] [--no-e0] [--no-assembled] [--no-shares] [--space-group].
 """
 from __future__ import annotations
 
@@ -562,6 +562,7 @@ def main():
     ap.add_argument("--no-e0", action="store_true",
                     help="skip the time-to-E0 runs (about 40 s; also those on the assembled rows)")
     ap.add_argument("--no-grad", action="store_true")
+    ap.add_argument("--no-shares", action="store_true", help="skip the eight row shares of the FP32 product")
     ap.add_argument("--no-assembled", action="store_true",
                     help="skip the assembled-row strategy (77 GB of entries at 6x6; about 20 s)")
     ap.add_argument("--tol-fp32", type=float, default=0.0,
@@ -658,7 +659,8 @@ def main():
         "checksum_f32": cs32,
         "checksum_f64": cs64,
     }
-    line["row_shares"] = share_times(ed, opr)
+    if not args.no_shares:
+        line["row_shares"] = share_times(ed, opr)
     if not args.no_e0:
         line["time_to_E0"] = lanczos_times(ed, opr, args.tol_fp32)
         # the same runs with every step orthogonalised against the whole basis
