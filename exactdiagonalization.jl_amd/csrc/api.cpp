@@ -2715,8 +2715,10 @@ int edhip_lanczos(edhip_op* op, const edhip_lanczos_opts* opts, double* evals, v
           st.assembled = 1;
         }
       } catch (const Error& e) {
-        if (e.code != EDHIP_ERR_MEMORY) throw;
-        (void)hipGetLastError();  // no room: the products stay matrix-free
+        // no room, or rows an entry cannot hold (2^31 states or more, more
+        // weight classes than its bits name): the products stay matrix-free
+        if (e.code != EDHIP_ERR_MEMORY && e.code != EDHIP_ERR_ARGUMENT) throw;
+        (void)hipGetLastError();
       }
     }
     void* ritz = arena;

@@ -267,3 +267,33 @@ def test_eigs_assembles_large_reduced_spaces(ed):
         L.edhip_debug_fail_allocs(0, 0)
         opr.set_strategy()
         ed.release_workspace()
+
+
+def test_eigs_stays_matrix_free_when_entries_cannot_name_the_weights(ed):
+    """A reduced space over 2^22 states whose weights outnumber what an entry
+    can name (5x6 torus at k = (1, 1): 5,170,604 states, 23 column bits and
+    256 classes; 12 couplings of distinct strength, one per displacement,
+    times 30 character phases make 361 weights): eigs solves it matrix-free
+    instead of failing, with the same eigenvalue as assemble="off"."""
+    Lx, Ly = 5, 6
+    n = Lx * Ly
+    hs, _ = ed.spin_half_system(n)
+    disp = [(1, 0), (0, 1), (1, 1), (1, -1), (2, 0), (0, 2), (2, 1), (1, 2), (2, 2), (2, -1), (-1, 2), (2, -2)]
+    bonds, w = [], []
+    for k, (dx, dy) in enumerate(disp):
+        for y in range(Ly):
+            for x in range(Lx):
+                bonds.append((x + Lx * y, (x + dx) % Lx + Lx * ((y + dy) % Ly)))
+                w.append(1.0 / (1 + 0.37 * k))
+    H = _heis(ed, hs, bonds, w)
+    r = ed.symmetry_reduce(ed.represent(ed.HilbertSpaceSector(hs, 0)), ed.TranslationIrrep((Lx, Ly), (1, 1)))
+    opr = ed.represent(r, H)
+    assert opr.dimension > 1 << 22 and opr.is_complex
+    kw = dict(tol=1e-9, max_basis=80, criterion="eigenvalue")
+    a = ed.eigs(opr, 1, **kw)
+    b = ed.eigs(opr, 1, assemble="off", **kw)
+    assert a.converged and not a.assembled and not b.assembled
+    assert a.values[0] == b.values[0]
+    with pytest.raises(ed.ArgumentError):
+        opr.set_strategy("assembled")
+    ed.release_workspace()
