@@ -1127,6 +1127,10 @@ hipError_t launch_asm_fill(const OpDev& op, const AsmFillDev& a, hipStream_t st)
           if (Lx == 6 && Ly == 4) return launch_asm_fill_t<kTransPairs6x4>(op, a, st);
           if (Lx == 4 && Ly == 4) return launch_asm_fill_t<kTransPairs4x4>(op, a, st);
         }
+        if (Lx == 6 && Ly == 6) return launch_asm_fill_t<kTorusPairs6x6>(op, a, st);
+        if (Lx == 4 && Ly == 6) return launch_asm_fill_t<kTorusPairs4x6>(op, a, st);
+        if (Lx == 6 && Ly == 4) return launch_asm_fill_t<kTorusPairs6x4>(op, a, st);
+        if (Lx == 4 && Ly == 4) return launch_asm_fill_t<kTorusPairs4x4>(op, a, st);
         return launch_asm_fill_t<kGroupTorusPairs>(op, a, st);
       }
       return launch_asm_fill_t<kGroupTorus>(op, a, st);

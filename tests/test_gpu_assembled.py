@@ -2,7 +2,8 @@
 streamed by every later product give the oracle's products (GPU).
 
 Covered: unreduced sectors (rank arithmetic), translation sectors with unit,
-real and complex characters (row-pair tori, 1D chains), a dihedral group on
+real and complex characters (row-pair tori, 1D chains), space groups (cosets
+of the translations, unit and +-1 characters), a dihedral group on
 the generic canonicaliser, images that leave the sector (weight class 0),
 add-into (apply!) products, every lane count of the streaming kernel, the
 column form (which keeps the term list), an injected allocation failure of
@@ -54,6 +55,12 @@ def _system(ed, oracle, name):
         H = _heis(ed, hs, j1 + j2, [1.0] * len(j1) + [0.5] * len(j2))
         H = ed.simplify(H + sum((0.3 * s(i, "z") + 0.15 * s(i, "x") for i in range(n)), ed.NullOperator()))
         return _reduced(ed, oracle, hs, n, H, ed.TranslationIrrep((Lx, Ly), k))
+    if name.startswith("sg4x4"):  # space group: cosets of the translations (unit and +-1 characters)
+        hs, _ = ed.spin_half_system(16)
+        j1, j2 = ed.square_bonds(4, 4)
+        H = _heis(ed, hs, j1 + j2, [1.0] * len(j1) + [0.5] * len(j2))
+        irrep = ed.SquareSpaceIrrep(4, (0, 0), "A1" if name == "sg4x4_a1" else "B1")
+        return _reduced(ed, oracle, hs, 16, H, irrep)
     if name == "dihedral14":  # generic canonicaliser
         L = 14
         hs, _ = ed.spin_half_system(L)
@@ -64,7 +71,7 @@ def _system(ed, oracle, name):
     raise KeyError(name)
 
 
-NAMES = ["xxz14", "chain16_k3", "torus4x6_00", "torus4x6_pi0", "torus4x4_k12", "dihedral14"]
+NAMES = ["xxz14", "chain16_k3", "torus4x6_00", "torus4x6_pi0", "torus4x4_k12", "sg4x4_a1", "sg4x4_b1", "dihedral14"]
 
 
 @pytest.fixture(scope="module")
