@@ -106,7 +106,8 @@ class LanczosStats(ctypes.Structure):
 
 
 class MinresStats(ctypes.Structure):
-    _fields_ = [("iterations", ctypes.c_int), ("converged", ctypes.c_int), ("residual", ctypes.c_double)]
+    _fields_ = [("iterations", ctypes.c_int), ("converged", ctypes.c_int), ("residual", ctypes.c_double),
+                ("assembled", ctypes.c_int)]
 
 
 # name -> (restype, argtypes); every symbol declared in include/edhip.h

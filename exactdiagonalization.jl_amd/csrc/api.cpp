@@ -1492,6 +1492,41 @@ char* krylov_arena_for(edhip_op* op, size_t bytes) {
   }
 }
 
+// Assembled rows for the time of a solve (eigensolver, MINRES) with the
+// operator's kernel strategy on auto: reduced spaces over 2^22 words whose
+// arrays take at most kAutoAsmMaxBytes and fit beside the solver's workspace
+// (already allocated) with 4 GiB to spare.  Spaces whose rows an entry cannot
+// hold, or that do not fit, stay matrix-free.  The destructor releases the
+// arrays and puts the strategy back to auto.
+struct SolveAssembly {
+  edhip_op* op = nullptr;
+  bool begin(edhip_op* o) {
+    if (o->strat_kernel != 0 || o->space->dim <= ((int64_t)1 << 22) || !o->space->group) return false;
+    size_t free_b = 0, total_b = 0;
+    HIP_OR_THROW(hipMemGetInfo(&free_b, &total_b));
+    const uint64_t margin = 4ull << 30;
+    const uint64_t room = free_b > margin ? (uint64_t)free_b - margin : 0;
+    try {
+      if (!ensure_assembled(o, std::min<uint64_t>(kAutoAsmMaxBytes, room))) return false;
+    } catch (const Error& e) {
+      // no room, or rows an entry cannot hold (2^31 states or more, more
+      // weight classes than its bits name)
+      if (e.code != EDHIP_ERR_MEMORY && e.code != EDHIP_ERR_ARGUMENT) throw;
+      (void)hipGetLastError();
+      return false;
+    }
+    o->strat_kernel = 3;
+    op = o;
+    return true;
+  }
+  ~SolveAssembly() {
+    if (op) {
+      op->release_assembled();
+      op->strat_kernel = 0;
+    }
+  }
+};
+
 struct TRLanczos {
   edhip_op* op;
   int prec;        // storage precision of the Krylov vectors
@@ -2693,34 +2728,9 @@ int edhip_lanczos(edhip_op* op, const edhip_lanczos_opts* opts, double* evals, v
     // allocates it, later calls of the same size reuse it)
     Timer total;
     // assembled rows for the solve (kernel auto, opts->assemble 0), released
-    // when it ends (the guard also on errors)
-    struct AsmGuard {
-      edhip_op* op = nullptr;
-      ~AsmGuard() {
-        if (op) {
-          op->release_assembled();
-          op->strat_kernel = 0;
-        }
-      }
-    } asm_guard;
-    if (opts->assemble == 0 && op->strat_kernel == 0 && n > ((int64_t)1 << 22) && op->space->group) {
-      size_t free_b = 0, total_b = 0;
-      HIP_OR_THROW(hipMemGetInfo(&free_b, &total_b));
-      const uint64_t margin = 4ull << 30;
-      const uint64_t room = free_b > margin ? (uint64_t)free_b - margin : 0;
-      try {
-        if (ensure_assembled(op, std::min<uint64_t>(kAutoAsmMaxBytes, room))) {
-          op->strat_kernel = 3;
-          asm_guard.op = op;
-          st.assembled = 1;
-        }
-      } catch (const Error& e) {
-        // no room, or rows an entry cannot hold (2^31 states or more, more
-        // weight classes than its bits name): the products stay matrix-free
-        if (e.code != EDHIP_ERR_MEMORY && e.code != EDHIP_ERR_ARGUMENT) throw;
-        (void)hipGetLastError();
-      }
-    }
+    // when it ends (also on errors)
+    SolveAssembly solve_asm;
+    if (opts->assemble == 0) st.assembled = solve_asm.begin(op) ? 1 : 0;
     void* ritz = arena;
     L.Vp = arena + ritz_b;
     L.tmpp = arena + ritz_b + main_v;
@@ -2915,6 +2925,8 @@ int edhip_minres(edhip_op* op, int prec, double shift, int n_deflate, const void
     H.set_size(n);
     const size_t vb = align256(bytes);
     char* buf = krylov_arena_for(op, vb * 7);
+    SolveAssembly solve_asm;  // assembled rows for the solve, as the eigensolver takes them
+    const bool assembled = solve_asm.begin(op);
     auto V = [&](int i) { return (void*)(buf + (size_t)i * vb); };
     void *r1 = V(0), *r2 = V(1), *y = V(2), *v = V(3), *w = V(4), *w1 = V(5), *w2 = V(6);
     auto project = [&](void* z) {
@@ -2935,6 +2947,7 @@ int edhip_minres(edhip_op* op, int prec, double shift, int n_deflate, const void
     HIP_OR_THROW(vec_fill_zero(prec, n, w2, s));
     double beta1 = std::sqrt(std::max(H.dot(r1, r1).real(), 0.0));
     edhip_minres_stats out{};
+    out.assembled = assembled ? 1 : 0;
     if (beta1 == 0.0) {
       out.converged = 1;
       if (stats) *stats = out;

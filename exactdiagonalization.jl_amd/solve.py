@@ -144,10 +144,13 @@ def release_workspace(ctx=None):
     ctx.check(_lib.lib().edhip_release_workspace(ctx.handle))
 
 
-def minres(opr, b, *, shift: float = 0.0, deflate=None, tol: float = 1e-10, max_iter: int = 2000):
+def minres(opr, b, *, shift: float = 0.0, deflate=None, tol: float = 1e-10, max_iter: int = 2000,
+           stats: bool = False):
     """Solve P (H - shift) P x = P b with MINRES, P = 1 - sum d d^H over the
     rows of ``deflate`` (orthonormal).  b, deflate: GPU tensors (float64 or
-    complex128).  Returns (x, iterations, residual)."""
+    complex128).  Returns (x, iterations, residual), and whether the solve
+    assembled the operator's rows when ``stats`` (it does, as eigs, on
+    reduced spaces over 2^22 states with the kernel strategy on auto)."""
     import torch
 
     if not (torch.is_tensor(b) and b.is_cuda):
@@ -167,6 +170,8 @@ def minres(opr, b, *, shift: float = 0.0, deflate=None, tol: float = 1e-10, max_
     st = _lib.MinresStats()
     _device_call(opr.ctx, _lib.lib().edhip_minres, opr._handle, prec, float(shift), nd, dptr, ctypes.c_void_p(b.data_ptr()),
                  ctypes.c_void_p(x.data_ptr()), float(tol), int(max_iter), ctypes.byref(st))
+    if stats:
+        return x, st.iterations, st.residual, bool(st.assembled)
     return x, st.iterations, st.residual
 
 

@@ -384,11 +384,15 @@ typedef struct {
   int iterations;
   int converged;
   double residual; /* final ||b - A x|| / ||b|| estimate */
+  int assembled;   /* 1 if the solve assembled the operator's rows */
 } edhip_minres_stats;
 
 /* MINRES (Paige & Saunders) on A = P (H - shift) P, P = 1 - sum_i d_i d_i^H
  * over n_deflate orthonormal device vectors d (may be 0):
- * solves A x = P b.  FP64/C128 vectors (prec EDHIP_F64 or EDHIP_C128). */
+ * solves A x = P b.  FP64/C128 vectors (prec EDHIP_F64 or EDHIP_C128).
+ * With the kernel strategy on auto, a reduced space over 2^22 words gets
+ * assembled rows for the solve as in edhip_lanczos (at most 16 GiB, beside
+ * the workspace with 4 GiB to spare); another strategy turns that off. */
 int edhip_minres(edhip_op* op, int prec, double shift, int n_deflate, const void* deflate, const void* b, void* x,
                  double tol, int max_iter, edhip_minres_stats* stats);
 
