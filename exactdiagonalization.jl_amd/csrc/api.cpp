@@ -1748,24 +1748,9 @@ struct TRLanczos {
         std::complex<double> nn = dot(wv, wv);
         double beta = std::sqrt(std::max(nn.real(), 0.0));
         if (pro && beta > 0.0) {
-          // omega_{j+1,q} beta_j = beta_q omega_{j,q+1} + (alpha_q - alpha_j) omega_{j,q}
-          //                        + beta_{q-1} omega_{j,q-1} - beta_{j-1} omega_{j-1,q}
-          // plus a rounding term of the sign of the sum
-          om_new.assign((size_t)j + 2, 0.0);
-          const double a_j = T[(size_t)j * mm + j];
-          const double b_jm1 = j > 0 ? T[(size_t)j * mm + j - 1] : 0.0;
-          double worst = 0.0;
-          for (int q = 0; q < j; ++q) {
-            const double b_q = T[(size_t)(q + 1) * mm + q];
-            double v = b_q * om_cur[(size_t)q + 1] + (T[(size_t)q * mm + q] - a_j) * om_cur[(size_t)q] -
-                       b_jm1 * om_prev[(size_t)q];
-            if (q > 0) v += T[(size_t)q * mm + q - 1] * om_cur[(size_t)q - 1];
-            v += std::copysign(eps * (b_q + beta), v);
-            om_new[(size_t)q] = v / beta;
-            worst = std::max(worst, std::fabs(om_new[(size_t)q]));
-          }
-          om_new[(size_t)j] = eps;
-          om_new[(size_t)j + 1] = 1.0;
+          const double worst = omega_step(
+              j, [&](int q) { return T[(size_t)q * mm + q]; }, [&](int q) { return T[(size_t)(q + 1) * mm + q]; },
+              beta, om_cur, om_prev, eps, om_new);
           if (om_force || worst > om_thr) {
             orthogonalize(Vp, ld, j + 1, wv);
             ++full_steps;

@@ -164,6 +164,36 @@ inline std::string group_inverses(int G, int nb, const std::vector<int32_t>& per
 // hi[v * 7 + m] = sum_i C(6 + q_i, m + i) over the set bits q_1 < ... of v.
 // Returns the 64 + 64 * 7 entries (lo, then hi).
 constexpr int kColexStride = 924;  // C(12, 6): room for every popcount class
+// One step of the omega recurrence of partial reorthogonalisation (Simon,
+// "The Lanczos algorithm with partial reorthogonalization", Math. Comp. 42
+// (1984) 115-142).  Given the estimates om_cur[q] ~ <v_j, v_q> (q <= j) and
+// om_prev[q] ~ <v_{j-1}, v_q> (q < j), the Lanczos coefficients alpha(q)
+// (q <= j) and beta(q) (q < j: the coupling of v_q and v_{q+1}) and the new
+// beta_j > 0, writes om_new[q] ~ <v_{j+1}, v_q> for q <= j + 1:
+//   beta_j om_new[q] = beta(q) om_cur[q+1] + (alpha(q) - alpha(j)) om_cur[q]
+//                      + beta(q-1) om_cur[q-1] - beta(j-1) om_prev[q]
+// plus a rounding term eps (beta(q) + beta_j) of the sum's sign, with
+// om_new[j] = eps and om_new[j+1] = 1.  Returns max |om_new[q]| over q < j.
+template <typename Alpha, typename Beta>
+inline double omega_step(int j, Alpha alpha, Beta beta, double beta_j, const std::vector<double>& om_cur,
+                         const std::vector<double>& om_prev, double eps, std::vector<double>& om_new) {
+  om_new.assign((size_t)j + 2, 0.0);
+  const double a_j = alpha(j);
+  const double b_jm1 = j > 0 ? beta(j - 1) : 0.0;
+  double worst = 0.0;
+  for (int q = 0; q < j; ++q) {
+    const double b_q = beta(q);
+    double v = b_q * om_cur[(size_t)q + 1] + (alpha(q) - a_j) * om_cur[(size_t)q] - b_jm1 * om_prev[(size_t)q];
+    if (q > 0) v += beta(q - 1) * om_cur[(size_t)q - 1];
+    v += std::copysign(eps * (b_q + beta_j), v);
+    om_new[(size_t)q] = v / beta_j;
+    worst = std::max(worst, std::fabs(om_new[(size_t)q]));
+  }
+  om_new[(size_t)j] = eps;
+  om_new[(size_t)j + 1] = 1.0;
+  return worst;
+}
+
 inline std::vector<uint16_t> colex12_tables() {
   auto binom = [](int n, int k) -> uint32_t {
     if (k < 0 || k > n) return 0;

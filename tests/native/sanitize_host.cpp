@@ -1,7 +1,8 @@
 // Host-only checks built with AddressSanitizer and UndefinedBehaviorSanitizer
 // (tests/test_sanitizers.py): the product's host logic (the eigensolver's
-// tridiagonal / arrowhead eigensolve, the group closure and inverses, the
-// colex ranks of the spread of x) and the oracle's C restatement on
+// tridiagonal / arrowhead eigensolve, the omega recurrence of the partial
+// reorthogonalisation, the group closure and inverses, the colex ranks of
+// the spread of x) and the oracle's C restatement on
 // small inputs, so that an out-of-bounds access or undefined behaviour on
 // these paths aborts the run.  Returns 0 when every check holds.
 #include <stdint.h>
@@ -247,8 +248,120 @@ static void test_oracle() {
   }
 }
 
+// Lanczos on a dense random symmetric matrix (n = 300) with the three-term
+// recurrence only, beside the omega recurrence of the partial
+// reorthogonalisation.  The recurrence without its rounding term is an
+// identity for the actual overlaps up to local rounding: fed the measured
+// overlaps of v_j and v_{j-1}, it predicts those of v_{j+1} to 1e-3 where
+// they exceed 1e-9 (a dropped or wrong term misses by far more).  Its
+// estimates (rounding term included) do not underestimate the overlaps by
+// more than a factor 10 while they stay below 1e-4.  With the partial
+// reorthogonalisation (the whole basis when an estimate exceeds sqrt(eps),
+// and the next vector too) the basis stays semi-orthogonal and the lowest
+// Ritz value matches the dense eigensolve.
+static void test_omega_recurrence() {
+  const int n = 300, m = 120;
+  std::mt19937_64 rng(11);
+  std::normal_distribution<double> nd;
+  std::vector<double> A((size_t)n * n);
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j <= i; ++j) A[(size_t)i * n + j] = A[(size_t)j * n + i] = nd(rng) / std::sqrt((double)n);
+  for (int i = 0; i < n; ++i) A[(size_t)i * n + i] += 0.05 * i;  // spread spectrum, a clear lowest eigenvalue
+  std::vector<double> w_ref, Z;
+  {
+    std::vector<double> B = A, d(n), e(n, 0.0);
+    edhip::householder_tridiag(n, B, Z, d, e);
+    CHECK(edhip::tridiag_ql(n, d, e, Z));
+    w_ref = d;
+  }
+  const double eps = 0x1p-53, thr = std::sqrt(eps);
+  for (int partial = 0; partial < 2; ++partial) {
+    std::vector<std::vector<double>> V(1, std::vector<double>(n));
+    double nrm = 0.0;
+    for (double& x : V[0]) nrm += (x = nd(rng)) * x;
+    for (double& x : V[0]) x /= std::sqrt(nrm);
+    std::vector<double> alpha, beta, om_prev, om_cur(1, 1.0), om_new;
+    std::vector<double> W_prev, W_cur(1, 1.0), W_pred;  // measured overlaps, and their prediction
+    int predicted = 0;
+    bool force = false;
+    int full = 0;
+    double worst_actual = 0.0;
+    for (int j = 0; j < m; ++j) {
+      std::vector<double> w(n, 0.0);
+      for (int r = 0; r < n; ++r)
+        for (int c = 0; c < n; ++c) w[(size_t)r] += A[(size_t)r * n + c] * V[(size_t)j][(size_t)c];
+      double a = 0.0;
+      for (int r = 0; r < n; ++r) a += w[(size_t)r] * V[(size_t)j][(size_t)r];
+      for (int r = 0; r < n; ++r) w[(size_t)r] -= a * V[(size_t)j][(size_t)r] + (j ? beta[(size_t)j - 1] * V[(size_t)j - 1][(size_t)r] : 0.0);
+      alpha.push_back(a);
+      double b = 0.0;
+      for (double x : w) b += x * x;
+      b = std::sqrt(b);
+      const double est = edhip::omega_step(
+          j, [&](int q) { return alpha[(size_t)q]; }, [&](int q) { return beta[(size_t)q]; }, b, om_cur, om_prev, eps,
+          om_new);
+      if (!partial)
+        (void)edhip::omega_step(
+            j, [&](int q) { return alpha[(size_t)q]; }, [&](int q) { return beta[(size_t)q]; }, b, W_cur, W_prev, 0.0,
+            W_pred);
+      if (partial && (force || est > thr)) {
+        for (int pass = 0; pass < 2; ++pass)
+          for (int q = 0; q <= j; ++q) {
+            double c = 0.0;
+            for (int r = 0; r < n; ++r) c += V[(size_t)q][(size_t)r] * w[(size_t)r];
+            for (int r = 0; r < n; ++r) w[(size_t)r] -= c * V[(size_t)q][(size_t)r];
+          }
+        b = 0.0;
+        for (double x : w) b += x * x;
+        b = std::sqrt(b);
+        for (int q = 0; q <= j; ++q) om_new[(size_t)q] = eps;
+        force = !force;
+        ++full;
+      }
+      beta.push_back(b);
+      std::vector<double> v(n);
+      for (int r = 0; r < n; ++r) v[(size_t)r] = w[(size_t)r] / b;
+      double actual = 0.0;  // max overlap of the new vector with the older ones
+      std::vector<double> W_next((size_t)j + 2, 0.0);
+      for (int q = 0; q <= j; ++q) {
+        double c = 0.0;
+        for (int r = 0; r < n; ++r) c += V[(size_t)q][(size_t)r] * v[(size_t)r];
+        W_next[(size_t)q] = c;
+        if (q < j) actual = std::max(actual, std::fabs(c));
+        if (!partial && q < j && std::fabs(c) > 1e-9) {
+          CHECK(std::fabs(W_pred[(size_t)q] - c) <= 1e-3 * std::fabs(c));
+          ++predicted;
+        }
+      }
+      W_next[(size_t)j + 1] = 1.0;
+      W_prev.swap(W_cur);
+      W_cur.swap(W_next);
+      double est_now = 0.0;
+      for (int q = 0; q < j; ++q) est_now = std::max(est_now, std::fabs(om_new[(size_t)q]));
+      if (!partial && est_now < 1e-4) CHECK(actual <= 10.0 * est_now + 1e-14);
+      worst_actual = std::max(worst_actual, actual);
+      V.push_back(v);
+      om_prev.swap(om_cur);
+      om_cur.swap(om_new);
+    }
+    if (partial) {
+      CHECK(worst_actual < 10.0 * thr);
+      CHECK(full > 0 && full < m / 2);
+      std::vector<double> d(alpha), e(m, 0.0), Zt((size_t)m * m, 0.0);
+      for (int i = 0; i + 1 < m; ++i) e[(size_t)i] = beta[(size_t)i];
+      for (int i = 0; i < m; ++i) Zt[(size_t)i * m + i] = 1.0;
+      CHECK(edhip::tridiag_ql(m, d, e, Zt));
+      CHECK(std::fabs(d[0] - w_ref[0]) < 1e-10 * std::max(1.0, std::fabs(w_ref[0])));
+    } else {
+      CHECK(worst_actual > thr);  // without it the basis does lose orthogonality here
+      CHECK(predicted > 100);     // overlaps above 1e-9 were predicted and checked
+    }
+  }
+}
+
 int main() {
   test_linear_algebra();
+  test_omega_recurrence();
   test_group_closure();
   test_colex();
   test_oracle();
